@@ -1,0 +1,707 @@
+// C ABI of the solver (include/lbm.h): handles, device buffers, the step loop and readbacks.
+//
+// Stands in for the host halves of the reference's Solver (src/solver.ts:50-340) and Solver3D
+// (src/solver3d.ts:42-319): parameter derivation (writeParams), geometry rebuild (setSdf /
+// rebuildFlags), initField, resetForces, encodeSteps / encodeMacro / run, readMacro and the force
+// ring bookkeeping of readForces. WebGPU buffers become hipMalloc allocations, the queue becomes
+// one HIP stream per handle, and uniform-buffer writes become kernel arguments, so setters are
+// stream-ordered exactly as queue.writeBuffer is.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "kernels.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      return fail(LBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    }                                                                                     \
+  } while (0)
+
+constexpr double MAGIC_LAMBDA = 3.0 / 16.0;
+
+// Inlet ramp evaluated per step on the host (an extension; the reference's drivers ramp per chunk,
+// src/cases.ts:52-54): kind 0 off, 1 linear, 2 smoothstep t^2 (3 - 2t).
+struct Ramp {
+  int kind = 0;
+  double u_target = 0, steps = 0;
+  double at(int64_t k) const {
+    double r = steps > 0 ? std::min(1.0, double(k) / steps) : 1.0;
+    if (kind == 2) r = r * r * (3 - 2 * r);
+    return u_target * r;
+  }
+};
+
+struct Ticket {
+  double* ring = nullptr;    // pinned, HISTORY_LEN * dim
+  uint32_t* state = nullptr; // pinned
+  hipEvent_t ev = nullptr;
+  int64_t generation = 0;
+};
+
+}  // namespace
+
+struct lbm_solver {
+  int nd = 2;  // 2 or 3
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int W = 1, H = 1, D = 1;
+  uint32_t N = 0;
+  int Q = 9;
+  int prec = LBM_FP32;
+  size_t elem = 4;
+  lbm_config2d cfg2{};
+  lbm_config3d cfg3{};
+  Params2 p2{};
+  Params3 p3{};
+  Ramp ramp;
+  int bulk_block = 256;
+
+  void* f[2] = {nullptr, nullptr};
+  uint32_t* flags = nullptr;
+  uint8_t* touches = nullptr;
+  float* sdf = nullptr;
+  float4* mac = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t nlist = 0;
+  int64_t n_bulk = 0, n_solid = 0;
+  double* partials = nullptr;
+  uint32_t* counter = nullptr;
+  double* ring = nullptr;
+  uint32_t* state = nullptr;
+
+  int64_t step = 0;
+  int parity = 0;
+
+  // readForces bookkeeping (src/solver.ts:57-61, 236-241, 315-335)
+  int64_t samples_read = 0;
+  std::deque<int64_t> force_steps;
+  int64_t generation = 0;
+  std::mutex mu;
+  std::map<int64_t, Ticket> tickets;
+  int64_t next_ticket = 1;
+
+  int dim() const { return nd == 2 ? 2 : 3; }
+  int force_every() const { return nd == 2 ? cfg2.force_every : cfg3.force_every; }
+};
+
+namespace {
+
+void free_all(lbm_solver* s) {
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
+                  (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state})
+    if (p) (void)hipFree(p);
+  for (auto& kv : s->tickets) {
+    if (kv.second.ring) (void)hipHostFree(kv.second.ring);
+    if (kv.second.state) (void)hipHostFree(kv.second.state);
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+  }
+  s->tickets.clear();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+// Solver.writeParams (src/solver.ts:180-197).
+void write_params2(lbm_solver* s) {
+  const lbm_config2d& c = s->cfg2;
+  Params2& p = s->p2;
+  p.W = s->W;
+  p.H = s->H;
+  p.N = s->N;
+  p.left = c.left;
+  p.right = c.right;
+  p.bottom = c.bottom;
+  p.top = c.top;
+  p.tau = float(c.tau);
+  p.lambda = float(c.lambda);
+  p.smagC2 = float(c.smagorinsky * c.smagorinsky);
+  p.uIn = float(c.u_in);
+  p.parabolic = c.parabolic ? 1 : 0;
+  p.spongeStart = float(c.sponge_fraction > 0 ? (s->W - 1) * (1 - c.sponge_fraction) : 1e9);
+  p.tauSponge = float(std::max(c.sponge_tau, c.tau));
+  p.uLid = float(c.u_lid);
+  p.fx = float(c.body_force[0]);
+  p.fy = float(c.body_force[1]);
+  p.forcing = (c.body_force[0] != 0 || c.body_force[1] != 0) ? 1 : 0;
+  p.absorb = float(c.absorb);
+  p.inletLayer = float(c.inlet_layer_fraction * (s->W - 1));
+}
+
+// Solver3D.writeParams (src/solver3d.ts:197-208); lambda defaults to trtLambda3(tau) (src/lattice3d.ts:29-31).
+void write_params3(lbm_solver* s) {
+  const lbm_config3d& c = s->cfg3;
+  Params3& p = s->p3;
+  p.W = s->W;
+  p.H = s->H;
+  p.D = s->D;
+  p.N = s->N;
+  p.WH = uint32_t(s->W) * uint32_t(s->H);
+  p.tau = float(c.tau);
+  p.lambda = float(std::isnan(c.lambda) ? std::min(MAGIC_LAMBDA, 50.0 * (c.tau - 0.5) * (c.tau - 0.5)) : c.lambda);
+  p.smagC2 = float(c.smagorinsky * c.smagorinsky);
+  p.uIn = float(c.u_in);
+  p.spongeStart = float(c.sponge_fraction > 0 ? (s->W - 1) * (1 - c.sponge_fraction) : 1e9);
+  p.tauSponge = float(std::max(c.sponge_tau, c.tau));
+  p.absorb = float(c.absorb);
+}
+
+// feq(i, 1, (u, 0, 0)) in float, evaluated in the source's operation order without contraction
+// (src/shaders/common3d.ts:78-81).
+#pragma clang fp contract(off)
+void inlet_feq3(float u, float* out) {
+  static const int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+  static const float Wt[19] = {1.0f / 3,  1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18,
+                               1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36,
+                               1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36};
+  const float rho = 1.0f, uy = 0.0f, uz = 0.0f;
+  for (int i = 0; i < 19; i++) {
+    float cu = 3.0f * (float(CX[i]) * u + 0.0f * uy + 0.0f * uz);
+    out[i] = Wt[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (u * u + uy * uy + uz * uz)));
+  }
+}
+
+// rebuildFlags (src/solver.ts:205-217, src/solver3d.ts:216-228): macro on the current buffer, flags
+// and refill on the GPU, then the boundary list (and 2D force slots) from the flags, in cell order.
+int rebuild_flags(lbm_solver* s) {
+  const uint32_t N = s->N;
+  if (s->nd == 2) {
+    launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
+    launch_flags2(s->sdf, s->flags, s->touches, static_cast<float*>(s->f[0]), static_cast<float*>(s->f[1]), s->mac,
+                  s->p2, s->stream);
+  } else {
+    launch_macro3(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
+    launch_flags3(s->prec, s->sdf, s->flags, s->f[0], s->f[1], s->mac, s->p3, s->stream);
+  }
+  HIP_TRY(hipGetLastError());
+  std::vector<uint32_t> fl(N);
+  HIP_TRY(hipMemcpyAsync(fl.data(), s->flags, sizeof(uint32_t) * N, hipMemcpyDeviceToHost, s->stream));
+  std::vector<uint8_t> touch;
+  if (s->nd == 2) {
+    touch.resize(N);
+    HIP_TRY(hipMemcpyAsync(touch.data(), s->touches, N, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+
+  uint32_t solid, inlet_flag, outlet_flag, plain_in, plain_out;
+  if (s->nd == 2) {
+    solid = FLAG_SOLID2;
+    inlet_flag = FLAG_INLET2;
+    outlet_flag = FLAG_OUTLET2;
+    plain_in = FLAG_INLET2 | INLET_DIRS2;
+    plain_out = FLAG_OUTLET2 | OUTLET_DIRS2;
+    // Force slots in cell order (the reference's atomicAdd order is arbitrary, src/shaders/aux.ts:63-66).
+    uint32_t slot = 0;
+    bool changed = false;
+    for (uint32_t k = 0; k < N; k++) {
+      if (!touch[k]) continue;
+      ++slot;
+      if (slot < (1u << (32 - BIDX_SHIFT2))) {
+        fl[k] |= slot << BIDX_SHIFT2;
+        changed = true;
+      }
+    }
+    if (changed) HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s->stream));
+  } else {
+    solid = FLAG_SOLID3;
+    inlet_flag = FLAG_INLET3;
+    outlet_flag = FLAG_OUTLET3;
+    plain_in = FLAG_INLET3 | INLET_DIRS3;
+    plain_out = FLAG_OUTLET3 | OUTLET_DIRS3;
+  }
+  // Boundary list: nodes the bulk kernel skips, grouped as inlet, outlet, other (uniform waves).
+  std::vector<uint32_t> lst;
+  s->n_bulk = 0;
+  s->n_solid = 0;
+  for (uint32_t k = 0; k < N; k++) {
+    const uint32_t v = fl[k];
+    if (v & solid) s->n_solid++;
+    else if (v == 0 || v == plain_in || v == plain_out) s->n_bulk++;
+  }
+  for (int pass = 0; pass < 3; pass++) {
+    for (uint32_t k = 0; k < N; k++) {
+      const uint32_t v = fl[k];
+      if (v == 0 || (v & solid) || v == plain_in || v == plain_out) continue;
+      const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
+      if (kind == pass) lst.push_back(k);
+    }
+  }
+  s->nlist = uint32_t(lst.size());
+  if (s->nlist) HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * lst.size(), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int alloc_common(lbm_solver* s) {
+  const size_t N = s->N;
+  HIP_TRY(hipSetDevice(s->device));
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  const size_t fbytes = size_t(s->Q) * N * s->elem;
+  HIP_TRY(hipMalloc(&s->f[0], fbytes));
+  HIP_TRY(hipMalloc(&s->f[1], fbytes));
+  HIP_TRY(hipMalloc(&s->flags, 4 * N));
+  if (s->nd == 2) HIP_TRY(hipMalloc(&s->touches, N));
+  HIP_TRY(hipMalloc(&s->sdf, 4 * N));
+  HIP_TRY(hipMalloc(&s->mac, 16 * N));
+  HIP_TRY(hipMalloc(&s->list, 4 * N));
+  const size_t maxblocks = (N + 255) / 256 + 1;
+  HIP_TRY(hipMalloc(&s->partials, sizeof(double) * 3 * maxblocks));
+  HIP_TRY(hipMalloc(&s->counter, 4));
+  HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
+  HIP_TRY(hipMalloc(&s->state, 4));
+  HIP_TRY(hipMemsetAsync(s->f[0], 0, fbytes, s->stream));
+  HIP_TRY(hipMemsetAsync(s->f[1], 0, fbytes, s->stream));
+  HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * N, s->stream));
+  HIP_TRY(hipMemsetAsync(s->mac, 0, 16 * N, s->stream));
+  HIP_TRY(hipMemsetAsync(s->counter, 0, 4, s->stream));
+  HIP_TRY(hipMemsetAsync(s->state, 0, 4, s->stream));
+  HIP_TRY(hipMemsetAsync(s->ring, 0, sizeof(double) * 3 * HISTORY_LEN, s->stream));
+  std::vector<float> far(N, 1e6f);  // src/solver.ts:133, src/solver3d.ts:142 (FAR)
+  HIP_TRY(hipMemcpyAsync(s->sdf, far.data(), 4 * N, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int bulk_block_for(int wg) {
+  if (wg <= 0) return 256;
+  if (wg % 64 != 0 || wg > 1024) return -1;
+  return wg;
+}
+
+int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
+  const int every = s->force_every();
+  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0};
+  fs.sample = (every > 0 && (s->step + 1) % every == 0) ? 1 : 0;
+  void* src = s->f[s->parity];
+  void* dst = s->f[s->parity ^ 1];
+  if (s->nd == 2) {
+    if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
+    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, s->list, s->nlist, s->p2, fs,
+                 s->bulk_block, s->stream, mid);
+  } else {
+    if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
+    float fin[19];
+    inlet_feq3(s->p3.uIn, fin);
+    launch_step3(s->prec, src, dst, s->flags, s->sdf, s->list, s->nlist, s->p3, fin, fs, s->bulk_block, s->stream, mid);
+  }
+  s->parity ^= 1;
+  s->step++;
+  if (fs.sample) s->force_steps.push_back(s->step);
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_abi_version(void) { return LBM_ABI_VERSION; }
+const char* lbm_last_error(void) { return g_err.c_str(); }
+
+void lbm_config2d_defaults(lbm_config2d* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->lambda = MAGIC_LAMBDA;
+  c->sponge_tau = 1.0;
+  c->workgroup_size = 128;
+  c->force_every = 1;
+}
+
+void lbm_config3d_defaults(lbm_config3d* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->lambda = std::nan("");
+  c->sponge_tau = 1.0;
+  c->workgroup_size = 128;
+  c->force_every = 1;
+}
+
+int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
+  if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->width < 1 || cfg->height < 1) return fail(LBM_ERR_INVALID, "width and height must be positive");
+  const uint64_t N = uint64_t(cfg->width) * uint64_t(cfg->height);
+  if (9 * N >= (1ull << 32)) return fail(LBM_ERR_INVALID, "grid too large: 9 * N must stay below 2^32");
+  const int bb = bulk_block_for(cfg->workgroup_size);
+  if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
+  lbm_solver* s = new lbm_solver();
+  s->nd = 2;
+  s->device = device;
+  s->W = cfg->width;
+  s->H = cfg->height;
+  s->N = uint32_t(N);
+  s->Q = 9;
+  s->elem = 4;
+  s->cfg2 = *cfg;
+  s->bulk_block = bb;
+  int rc = alloc_common(s);
+  if (rc == LBM_OK) {
+    write_params2(s);
+    rc = rebuild_flags(s);
+  }
+  if (rc != LBM_OK) {
+    std::string msg = g_err;
+    free_all(s);
+    delete s;
+    return fail(rc == LBM_ERR_DEVICE ? LBM_ERR_ALLOC : rc, "Solver allocation failed: " + msg);
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
+  if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (cfg->precision < LBM_FP32 || cfg->precision > LBM_BF16) return fail(LBM_ERR_UNSUPPORTED, "unknown precision");
+  if (cfg->width < 3 || cfg->height < 1 || cfg->depth < 1)
+    return fail(LBM_ERR_INVALID, "width must be at least 3, height and depth positive");
+  const uint64_t N = uint64_t(cfg->width) * uint64_t(cfg->height) * uint64_t(cfg->depth);
+  const size_t elem = cfg->precision == LBM_FP32 ? 4 : 2;
+  if (19 * N >= (1ull << 32)) {
+    return fail(LBM_ERR_ALLOC, "Solver3D allocation failed: one distribution buffer needs " +
+                                   std::to_string(19 * elem * N) + " bytes; this device allows " +
+                                   std::to_string((1ull << 32) / 19 * elem) + " per storage binding and " +
+                                   std::to_string((1ull << 32) / 19 * elem) + " per buffer");
+  }
+  const int bb = bulk_block_for(cfg->workgroup_size);
+  if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
+  lbm_solver* s = new lbm_solver();
+  s->nd = 3;
+  s->device = device;
+  s->W = cfg->width;
+  s->H = cfg->height;
+  s->D = cfg->depth;
+  s->N = uint32_t(N);
+  s->Q = 19;
+  s->prec = cfg->precision;
+  s->elem = elem;
+  s->cfg3 = *cfg;
+  s->bulk_block = bb;
+  int rc = alloc_common(s);
+  if (rc == LBM_OK) {
+    write_params3(s);
+    rc = rebuild_flags(s);
+  }
+  if (rc != LBM_OK) {
+    std::string msg = g_err;
+    free_all(s);
+    delete s;
+    return fail(rc == LBM_ERR_DEVICE ? LBM_ERR_ALLOC : rc, "Solver3D allocation failed: " + msg);
+  }
+  *out = s;
+  return LBM_OK;
+}
+
+void lbm_destroy(lbm_solver* s) {
+  if (!s) return;
+  free_all(s);
+  delete s;
+}
+
+int lbm_update_2d(lbm_solver* s, const lbm_config2d* cfg) {
+  if (!s || !cfg || s->nd != 2) return fail(LBM_ERR_INVALID, "lbm_update_2d needs a 2D solver and a config");
+  if (cfg->width != s->W || cfg->height != s->H) return fail(LBM_ERR_INVALID, "width and height cannot change");
+  const int bb = bulk_block_for(cfg->workgroup_size);
+  if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
+  s->cfg2 = *cfg;
+  s->bulk_block = bb;
+  write_params2(s);
+  return LBM_OK;
+}
+
+int lbm_update_3d(lbm_solver* s, const lbm_config3d* cfg) {
+  if (!s || !cfg || s->nd != 3) return fail(LBM_ERR_INVALID, "lbm_update_3d needs a 3D solver and a config");
+  if (cfg->width != s->W || cfg->height != s->H || cfg->depth != s->D || cfg->precision != s->prec)
+    return fail(LBM_ERR_INVALID, "grid size and precision cannot change");
+  const int bb = bulk_block_for(cfg->workgroup_size);
+  if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
+  s->cfg3 = *cfg;
+  s->bulk_block = bb;
+  write_params3(s);
+  return LBM_OK;
+}
+
+int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps) {
+  if (!s || kind < 0 || kind > 2) return fail(LBM_ERR_INVALID, "bad ramp");
+  s->ramp.kind = kind;
+  s->ramp.u_target = u_target;
+  s->ramp.steps = steps;
+  if (!kind) {
+    if (s->nd == 2) write_params2(s);
+    else write_params3(s);
+  }
+  return LBM_OK;
+}
+
+int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n) {
+  if (!s || !sdf || n != s->N) return fail(LBM_ERR_INVALID, "sdf must hold N floats");
+  HIP_TRY(hipMemcpyAsync(s->sdf, sdf, 4 * size_t(s->N), hipMemcpyHostToDevice, s->stream));
+  return rebuild_flags(s);
+}
+
+int lbm_init_field(lbm_solver* s, const float* macro, size_t n) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (s->nd == 2) {
+    std::vector<float> rest;
+    if (!macro) {
+      rest.assign(size_t(s->N) * 4, 0.0f);
+      for (uint32_t k = 0; k < s->N; k++) rest[size_t(k) * 4] = 1.0f;
+      macro = rest.data();
+    } else if (n != size_t(s->N) * 4) {
+      return fail(LBM_ERR_INVALID, "macro must hold N * 4 floats");
+    }
+    HIP_TRY(hipMemcpyAsync(s->mac, macro, 16 * size_t(s->N), hipMemcpyHostToDevice, s->stream));
+    launch_init2(static_cast<float*>(s->f[0]), static_cast<float*>(s->f[1]), s->mac, s->p2, s->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  } else {
+    if (macro) return fail(LBM_ERR_INVALID, "the 3D initField takes no macro field");
+    launch_init3(s->prec, s->f[0], s->f[1], s->flags, s->p3, s->stream);
+    HIP_TRY(hipGetLastError());
+  }
+  s->parity = 0;
+  s->step = 0;
+  return lbm_reset_forces(s);
+}
+
+int lbm_reset_forces(lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  launch_reset_state(s->state, s->stream);
+  HIP_TRY(hipGetLastError());
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->samples_read = 0;
+  s->force_steps.clear();
+  s->generation++;
+  return LBM_OK;
+}
+
+int lbm_enqueue_steps(lbm_solver* s, int64_t n) {
+  if (!s || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  for (int64_t k = 0; k < n; k++) enqueue_one(s);
+  HIP_TRY(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_enqueue_macro(lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (s->nd == 2) launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
+  else launch_macro3(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
+  HIP_TRY(hipGetLastError());
+  return LBM_OK;
+}
+
+int lbm_run(lbm_solver* s, int64_t n, int64_t chunk) {
+  if (!s || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (chunk <= 0) chunk = 500;
+  for (int64_t done = 0; done < n;) {
+    const int64_t k = std::min(chunk, n - done);
+    int rc = lbm_enqueue_steps(s, k);
+    if (rc) return rc;
+    rc = lbm_enqueue_macro(s);
+    if (rc) return rc;
+    done += k;
+  }
+  return lbm_sync(s);
+}
+
+int lbm_sync(lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_macro(lbm_solver* s, float* out, size_t n) {
+  if (!s || !out || n < size_t(s->N) * 4) return fail(LBM_ERR_INVALID, "out must hold N * 4 floats");
+  HIP_TRY(hipMemcpyAsync(out, s->mac, 16 * size_t(s->N), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_forces_begin(lbm_solver* s, int64_t* ticket) {
+  if (!s || !ticket) return fail(LBM_ERR_INVALID, "bad arguments");
+  Ticket t;
+  const size_t rb = sizeof(double) * size_t(s->dim()) * HISTORY_LEN;
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&t.ring), rb, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&t.state), 4, hipHostMallocDefault));
+  HIP_TRY(hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
+  // The device ring is HISTORY_LEN * 3 doubles wide; copy dim components per sample.
+  if (s->dim() == 3) {
+    HIP_TRY(hipMemcpyAsync(t.ring, s->ring, rb, hipMemcpyDeviceToHost, s->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(t.ring, s->ring, rb, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(t.state, s->state, 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipEventRecord(t.ev, s->stream));
+  std::lock_guard<std::mutex> lk(s->mu);
+  t.generation = s->generation;
+  *ticket = s->next_ticket++;
+  s->tickets[*ticket] = t;
+  return LBM_OK;
+}
+
+int lbm_read_forces_wait(lbm_solver* s, int64_t ticket) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  hipEvent_t ev;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(LBM_ERR_INVALID, "unknown ticket");
+    ev = it->second.ev;
+  }
+  HIP_TRY(hipEventSynchronize(ev));
+  return LBM_OK;
+}
+
+int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* values, int64_t cap, int64_t* count) {
+  if (!s || !count) return fail(LBM_ERR_INVALID, "bad arguments");
+  *count = 0;
+  Ticket t;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(LBM_ERR_INVALID, "unknown ticket");
+    t = it->second;
+    s->tickets.erase(it);
+  }
+  HIP_TRY(hipEventSynchronize(t.ev));
+  const int dim = s->dim();
+  int rc = LBM_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    // A reset while the copy was in flight makes these samples stale (src/solver.ts:322-323).
+    if (t.generation == s->generation && !s->force_steps.empty()) {
+      const int64_t written = int64_t(*t.state);
+      const int64_t first = std::max(s->samples_read, written - int64_t(HISTORY_LEN));
+      const int64_t avail = written - first;
+      if (avail > cap || (avail > 0 && (!steps || !values))) {
+        rc = fail(LBM_ERR_INVALID, "output capacity too small for " + std::to_string(avail) + " samples");
+      } else {
+        int64_t n = 0;
+        for (int64_t k = first; k < written; k++, n++) {
+          const int64_t slot = k % HISTORY_LEN;
+          const int64_t pos = k - s->samples_read;
+          steps[n] = pos < int64_t(s->force_steps.size()) ? s->force_steps[size_t(pos)] : -1;
+          for (int d = 0; d < dim; d++) values[n * dim + d] = t.ring[slot * dim + d];
+        }
+        *count = n;
+        const int64_t drop = std::min<int64_t>(written - s->samples_read, int64_t(s->force_steps.size()));
+        s->force_steps.erase(s->force_steps.begin(), s->force_steps.begin() + drop);
+        s->samples_read = written;
+      }
+    }
+  }
+  (void)hipHostFree(t.ring);
+  (void)hipHostFree(t.state);
+  (void)hipEventDestroy(t.ev);
+  return rc;
+}
+
+int lbm_read_forces(lbm_solver* s, int64_t* steps, double* values, int64_t cap, int64_t* count) {
+  int64_t t = 0;
+  int rc = lbm_read_forces_begin(s, &t);
+  if (rc) return rc;
+  rc = lbm_read_forces_wait(s, t);
+  if (rc) return rc;
+  return lbm_read_forces_end(s, t, steps, values, cap, count);
+}
+
+int lbm_info(const lbm_solver* s, int32_t* dims, int64_t* n_cells, int64_t* step, int32_t* lattice_q,
+             int32_t* precision) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (dims) dims[0] = s->W, dims[1] = s->H, dims[2] = s->D;
+  if (n_cells) *n_cells = s->N;
+  if (step) *step = s->step;
+  if (lattice_q) *lattice_q = s->Q;
+  if (precision) *precision = s->prec;
+  return LBM_OK;
+}
+
+int lbm_read_populations(lbm_solver* s, void* out, size_t bytes) {
+  const size_t need = size_t(s ? s->Q : 0) * (s ? s->N : 0) * (s ? s->elem : 0);
+  if (!s || !out || bytes < need) return fail(LBM_ERR_INVALID, "out too small");
+  HIP_TRY(hipMemcpyAsync(out, s->f[s->parity], need, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n) {
+  if (!s || !out || n < s->N) return fail(LBM_ERR_INVALID, "out must hold N words");
+  HIP_TRY(hipMemcpyAsync(out, s->flags, 4 * size_t(s->N), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
+int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms) {
+  if (!s || n < 0 || !elapsed_ms) return fail(LBM_ERR_INVALID, "bad arguments");
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  HIP_TRY(hipEventRecord(a, s->stream));
+  for (int64_t k = 0; k < n; k++) enqueue_one(s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(b, s->stream));
+  HIP_TRY(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  *elapsed_ms = ms;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return LBM_OK;
+}
+
+int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_ms, double* list_ms) {
+  if (!s || n < 0 || !total_ms || !bulk_ms || !list_ms) return fail(LBM_ERR_INVALID, "bad arguments");
+  std::vector<hipEvent_t> ev(size_t(2 * n + 1));
+  for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(ev[0], s->stream));
+  for (int64_t k = 0; k < n; k++) {
+    enqueue_one(s, ev[size_t(2 * k + 1)]);
+    HIP_TRY(hipEventRecord(ev[size_t(2 * k + 2)], s->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(ev.back()));
+  double tb = 0, tl = 0;
+  for (int64_t k = 0; k < n; k++) {
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, ev[size_t(2 * k)], ev[size_t(2 * k + 1)]));
+    HIP_TRY(hipEventElapsedTime(&b, ev[size_t(2 * k + 1)], ev[size_t(2 * k + 2)]));
+    tb += a;
+    tl += b;
+  }
+  float tot = 0;
+  HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev.back()));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *total_ms = tot;
+  *bulk_ms = tb;
+  *list_ms = tl;
+  return LBM_OK;
+}
+
+int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (bulk) *bulk = s->n_bulk;
+  if (list) *list = s->nlist;
+  if (solid) *solid = s->n_solid;
+  return LBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,35 @@
+// Host-side launchers of the gfx950 kernels (defined in kernels2d.hip / kernels3d.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+// ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 ----
+// One time step: the bulk kernel (plain fluid and plain inlet/outlet nodes, one thread per cell)
+// followed by the boundary-list kernel (every other non-solid node, plus the force sample).
+void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf,
+                  const uint32_t* list, uint32_t nlist, const Params3& p, const float* feq_in,
+                  const ForceSink& fs, int bulk_block, hipStream_t st, hipEvent_t mid = nullptr);
+void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);
+void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st);
+void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
+                   const Params3& p, hipStream_t st);
+uint32_t list_grid3(uint32_t nlist);
+
+// ---- D2Q9 (FP32 storage) ----
+void launch_step2(const float* src, float* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
+                  uint32_t nlist, const Params2& p, const ForceSink& fs, int bulk_block, hipStream_t st,
+                  hipEvent_t mid = nullptr);
+void launch_macro2(const float* f, const uint32_t* flags, float4* mac, const Params2& p, hipStream_t st);
+void launch_init2(float* fa, float* fb, const float4* mac, const Params2& p, hipStream_t st);
+void launch_flags2(const float* sdf, uint32_t* flags, uint8_t* touches, float* fa, float* fb, const float4* mac,
+                   const Params2& p, hipStream_t st);
+uint32_t list_grid2(uint32_t nlist);
+
+// ---- shared ----
+void launch_reset_state(uint32_t* state, hipStream_t st);
+
+}  // namespace lbm

@@ -1,0 +1,491 @@
+// D3Q19 TRT kernels for gfx950: fused pull-stream + boundaries + collide, flags, init, macro.
+//
+// Restates src/shaders/step3d.ts:24-225 and src/shaders/aux3d.ts:20-152 for HIP, re-organised for
+// wave64 hardware: instead of one kernel in which the x-boundary and face nodes of every row take
+// a divergent slow path, a step is two launches on the same stream:
+//   * step3_bulk: one thread per cell over the whole grid in cell order. It handles plain fluid
+//     nodes (flags == 0) and plain inlet / outlet nodes (only the off-grid x bits set), which sit
+//     in the first and last wave of each x row, so every wave streams fully coalesced loads and
+//     stores. Other nodes (solid, faces, body links) are skipped.
+//   * step3_list: one thread per node of a list built once per geometry change (every non-solid
+//     node the bulk kernel skips, grouped by kind, in cell order within a kind). It runs the
+//     reference's per-direction boundary rules (pullSlow3), and fuses the momentum-exchange force
+//     reduction in FP64 (reduce.hpp).
+// Storage is structure-of-arrays f[i * N + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16;
+// arithmetic is FP32 throughout, as in the reference.
+#include "kernels.hpp"
+#include "reduce.hpp"
+
+namespace lbm {
+
+__device__ __forceinline__ float feq3(int i, float rho, float ux, float uy, float uz) {
+  // src/shaders/common3d.ts:78-81
+  const float cu = 3.0f * (float(CX3[i]) * ux + float(CY3[i]) * uy + float(CZ3[i]) * uz);
+  return W3[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (ux * ux + uy * uy + uz * uz)));
+}
+
+struct Mom3 {
+  float drho, rho, invRho, ux, uy, uz, usq;
+};
+
+// src/shaders/step3d.ts:38-45: sums in index order.
+__device__ __forceinline__ Mom3 moments3(const float (&f)[19]) {
+  Mom3 m;
+  float d = f[0];
+#pragma unroll
+  for (int i = 1; i < 19; i++) d = d + f[i];
+  float jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+  for (int i = 1; i < 19; i++) {
+    if (CX3[i] > 0) jx = jx + f[i];
+    if (CX3[i] < 0) jx = jx - f[i];
+    if (CY3[i] > 0) jy = jy + f[i];
+    if (CY3[i] < 0) jy = jy - f[i];
+    if (CZ3[i] > 0) jz = jz + f[i];
+    if (CZ3[i] < 0) jz = jz - f[i];
+  }
+  m.drho = d;
+  m.rho = 1.0f + d;
+  m.invRho = 1.0f / m.rho;
+  m.ux = jx * m.invRho;
+  m.uy = jy * m.invRho;
+  m.uz = jz * m.invRho;
+  m.usq = 1.5f * (m.ux * m.ux + m.uy * m.uy + m.uz * m.uz);
+  return m;
+}
+
+// TRT collision with sponge, Smagorinsky and the outlet absorbing layer: src/shaders/step3d.ts:47-102,
+// 210-222. Writes the post-collision populations to out.
+__device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, float x, const Params3& p,
+                                         float (&out)[19]) {
+  float cu[19], eqS[19], eqA[19];
+#pragma unroll
+  for (int a = 1; a < 19; a += 2) {
+    float acc = 0.0f;
+    if (CX3[a] > 0) acc = acc + m.ux;
+    if (CX3[a] < 0) acc = acc - m.ux;
+    if (CY3[a] > 0) acc = acc + m.uy;
+    if (CY3[a] < 0) acc = acc - m.uy;
+    if (CZ3[a] > 0) acc = acc + m.uz;
+    if (CZ3[a] < 0) acc = acc - m.uz;
+    cu[a] = 3.0f * acc;
+    eqS[a] = W3[a] * (m.drho + m.rho * (0.5f * cu[a] * cu[a] - m.usq));
+    eqA[a] = W3[a] * m.rho * cu[a];
+  }
+  float omegaBase = 1.0f / p.tau;
+  float sOut = 0.0f;
+  const float wm1 = float(p.W - 1);
+  if (p.spongeStart < wm1) {
+    sOut = smoothstep(p.spongeStart, wm1, x);
+    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
+  }
+  const float sigma = p.absorb * sOut * sOut;
+  float omP = omegaBase;
+  if (p.smagC2 > 0.0f) {
+    float n[19];
+    n[0] = 0.0f;
+#pragma unroll
+    for (int a = 1; a < 19; a += 2) {
+      n[a] = f[a] - (eqS[a] + eqA[a]);
+      n[a + 1] = f[a + 1] - (eqS[a] - eqA[a]);
+    }
+    float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+      if (CX3[i] != 0) pxx = pxx + n[i];
+      if (CY3[i] != 0) pyy = pyy + n[i];
+      if (CZ3[i] != 0) pzz = pzz + n[i];
+      if (CX3[i] * CY3[i] > 0) pxy = pxy + n[i];
+      if (CX3[i] * CY3[i] < 0) pxy = pxy - n[i];
+      if (CX3[i] * CZ3[i] > 0) pxz = pxz + n[i];
+      if (CX3[i] * CZ3[i] < 0) pxz = pxz - n[i];
+      if (CY3[i] * CZ3[i] > 0) pyz = pyz + n[i];
+      if (CY3[i] * CZ3[i] < 0) pyz = pyz - n[i];
+    }
+    const float pi = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float t0 = 1.0f / omegaBase;
+    const float tEff = 0.5f * (t0 + sqrtf(t0 * t0 + 25.455844122715714f * p.smagC2 * pi * m.invRho));
+    omP = 1.0f / tEff;
+  }
+  const float omM = 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f));
+  float s0 = 0.0f;
+  float sS[19], sA[19];
+#pragma unroll
+  for (int a = 1; a < 19; a += 2) sS[a] = sA[a] = 0.0f;
+  if (sigma > 0.0f) {
+    const float ut = p.uIn;
+    const float dq = 1.5f * m.rho * (ut * ut - m.ux * m.ux - m.uy * m.uy - m.uz * m.uz);
+    s0 += sigma * W3[0] * -dq;
+#pragma unroll
+    for (int a = 1; a < 19; a += 2) {
+      const float cut = 3.0f * float(CX3[a]) * ut;
+      sS[a] += sigma * W3[a] * (0.5f * m.rho * (cut * cut - cu[a] * cu[a]) - dq);
+      sA[a] += sigma * W3[a] * m.rho * (cut - cu[a]);
+    }
+  }
+  out[0] = f[0] - omP * (f[0] - W3[0] * (m.drho - m.rho * m.usq)) + s0;
+#pragma unroll
+  for (int a = 1; a < 19; a += 2) {
+    const int b = a + 1;
+    const float sym = omP * (0.5f * (f[a] + f[b]) - eqS[a]);
+    const float asym = omM * (0.5f * (f[a] - f[b]) - eqA[a]);
+    out[a] = f[a] - sym - asym + sS[a] + sA[a];
+    out[b] = f[b] - sym + asym + sS[a] - sA[a];
+  }
+}
+
+// Inlet equilibrium feq(i, 1, (uIn, 0, 0)) for all i, computed on the host per step (it is uniform
+// over the inlet plane) and passed by value.
+struct FeqIn {
+  float v[19];
+};
+
+// ============================================================================ bulk kernel
+template <class S>
+__global__ __launch_bounds__(1024) void step3_bulk(const typename S::elem* __restrict__ src,
+                                                   typename S::elem* __restrict__ dst,
+                                                   const uint32_t* __restrict__ flags, Params3 p, FeqIn fin) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p.N) return;
+  const uint32_t fl = flags[idx];
+  const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3);
+  const bool outlet = fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
+  if (fl != 0u && !inlet && !outlet) return;  // handled by the boundary list (or solid)
+  const uint32_t N = p.N;
+  if (inlet) {
+    // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
+#pragma unroll
+    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
+    return;
+  }
+  const uint32_t z = idx / p.WH;
+  const uint32_t rem = idx - z * p.WH;
+  const uint32_t y = rem / uint32_t(p.W);
+  const uint32_t x = rem - y * uint32_t(p.W);
+  float f[19];
+#pragma unroll
+  for (int i = 0; i < 19; i++) {
+    const int delta = CZ3[i] * int(p.WH) + CY3[i] * p.W + CX3[i];
+    uint32_t k = idx - uint32_t(delta);
+    // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z).
+    if (CX3[i] < 0 && outlet) k = idx - 1u;
+    f[i] = S::load(src + size_t(i) * N, k);
+  }
+  const Mom3 m = moments3(f);
+  if (outlet) {
+    // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
+#pragma unroll
+    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+    return;
+  }
+  float out[19];
+  collide3(f, m, float(x), p, out);
+#pragma unroll
+  for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+}
+
+// ============================================================================ boundary-list kernel
+template <class S>
+__device__ __forceinline__ bool solid3(const uint32_t* flags, uint32_t s) {
+  return (flags[s] & FLAG_SOLID3) != 0u;
+}
+
+// src/shaders/step3d.ts:119-168
+template <class S, int i>
+__device__ __forceinline__ float pull_slow3(const typename S::elem* __restrict__ src, const uint32_t* __restrict__ flags,
+                                            const float* __restrict__ sdf, int x, int y, int z, uint32_t idx,
+                                            uint32_t fl, bool outlet, const Params3& p, float (&force)[3]) {
+  constexpr int j = opp3(i);
+  const uint32_t N = p.N;
+  const int W = p.W, H = p.H, D = p.D;
+  const int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
+  auto at = [&](int ax, int ay, int az) { return uint32_t(ax + W * (ay + H * az)); };
+  const typename S::elem* srci = src + size_t(i) * N;
+  const typename S::elem* srcj = src + size_t(j) * N;
+  if ((fl & (1u << i)) == 0u) return S::load(srci, at(sx, sy, sz));
+  const bool outX = sx < 0 || sx >= W;
+  const bool outY = sy < 0 || sy >= H;
+  const bool outZ = sz < 0 || sz >= D;
+  if (outX && outlet) return S::load(srci, at(W - 2, y, z));
+  if (outY && outZ) return S::load(srcj, idx);
+  if (outY || outZ) {
+    int rx = sx, ry = sy, rz = sz;
+    int mdir;
+    if (outY) {
+      ry = y;
+      mdir = mirror_y3(i);
+    } else {
+      rz = z;
+      mdir = mirror_z3(i);
+    }
+    const uint32_t rs = at(rx, ry, rz);
+    if (solid3<S>(flags, rs)) {
+      const float fj = S::load(srcj, idx);
+      force[0] += float(CX3[j]) * (2.0f * fj);
+      force[1] += float(CY3[j]) * (2.0f * fj);
+      force[2] += float(CZ3[j]) * (2.0f * fj);
+      return fj;
+    }
+    return S::load(src + size_t(mdir) * N, rs);
+  }
+  if (outX) return S::load(srcj, idx);
+  const uint32_t s = at(sx, sy, sz);
+  if (!solid3<S>(flags, s)) return S::load(srci, s);
+  // Bouzidi linear interpolated bounce-back with q from the SDF.
+  const float fj = S::load(srcj, idx);
+  const float dF = sdf[idx];
+  const float dS = sdf[s];
+  const float q = wclamp(dF / fmaxf(dF - dS, 1e-6f), 1e-3f, 1.0f);
+  float fi = fj;
+  if (q < 0.5f) {
+    const int nx = x + CX3[i], ny = y + CY3[i], nz = z + CZ3[i];
+    if (nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D) {
+      const uint32_t n = at(nx, ny, nz);
+      if (!solid3<S>(flags, n)) fi = 2.0f * q * fj + (1.0f - 2.0f * q) * S::load(srcj, n);
+    }
+  } else {
+    fi = fj / (2.0f * q) + (2.0f * q - 1.0f) / (2.0f * q) * S::load(srci, idx);
+  }
+  force[0] += float(CX3[j]) * (fj + fi);
+  force[1] += float(CY3[j]) * (fj + fi);
+  force[2] += float(CZ3[j]) * (fj + fi);
+  return fi;
+}
+
+template <class S, int i>
+struct PullAll3 {
+  __device__ static __forceinline__ void run(const typename S::elem* src, const uint32_t* flags, const float* sdf, int x,
+                                             int y, int z, uint32_t idx, uint32_t fl, bool outlet, const Params3& p,
+                                             float (&f)[19], float (&force)[3]) {
+    f[i] = pull_slow3<S, i>(src, flags, sdf, x, y, z, idx, fl, outlet, p, force);
+    PullAll3<S, i + 1>::run(src, flags, sdf, x, y, z, idx, fl, outlet, p, f, force);
+  }
+};
+template <class S>
+struct PullAll3<S, 19> {
+  __device__ static __forceinline__ void run(const typename S::elem*, const uint32_t*, const float*, int, int, int,
+                                             uint32_t, uint32_t, bool, const Params3&, float (&)[19], float (&)[3]) {}
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void step3_list(const typename S::elem* __restrict__ src,
+                                                  typename S::elem* __restrict__ dst,
+                                                  const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
+                                                  const uint32_t* __restrict__ list, uint32_t nlist, Params3 p,
+                                                  FeqIn fin, ForceSink fs) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  float force[3] = {0.0f, 0.0f, 0.0f};
+  if (k < nlist) {
+    const uint32_t idx = list[k];
+    const uint32_t fl = flags[idx];
+    const uint32_t N = p.N;
+    if (fl & FLAG_INLET3) {
+#pragma unroll
+      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
+    } else {
+      const bool outlet = (fl & FLAG_OUTLET3) != 0u;
+      const int z = int(idx / p.WH);
+      const uint32_t rem = idx - uint32_t(z) * p.WH;
+      const int y = int(rem / uint32_t(p.W));
+      const int x = int(rem - uint32_t(y) * uint32_t(p.W));
+      float f[19];
+      f[0] = S::load(src, idx);
+      PullAll3<S, 1>::run(src, flags, sdf, x, y, z, idx, fl, outlet, p, f, force);
+      const Mom3 m = moments3(f);
+      if (outlet) {
+#pragma unroll
+        for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+      } else {
+        float out[19];
+        collide3(f, m, float(x), p, out);
+#pragma unroll
+        for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+      }
+    }
+  }
+  if (fs.sample) force_sample<3>(force, fs);
+}
+
+// ============================================================================ aux kernels
+// src/shaders/aux3d.ts:137-150: (ux, uy, uz, rho), zero for solids.
+template <class S>
+__global__ __launch_bounds__(256) void macro3_kernel(const typename S::elem* __restrict__ f,
+                                                     const uint32_t* __restrict__ flags, float4* __restrict__ mac,
+                                                     Params3 p) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p.N) return;
+  if (flags[idx] & FLAG_SOLID3) {
+    mac[idx] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 19; i++) {
+    const float v = S::load(f + size_t(i) * p.N, idx);
+    drho += v;
+    jx += float(CX3[i]) * v;
+    jy += float(CY3[i]) * v;
+    jz += float(CZ3[i]) * v;
+  }
+  const float rho = 1.0f + drho;
+  mac[idx] = make_float4(jx / rho, jy / rho, jz / rho, rho);
+}
+
+// src/shaders/aux3d.ts:113-123
+template <class S>
+__global__ __launch_bounds__(256) void init3_kernel(typename S::elem* __restrict__ fa, typename S::elem* __restrict__ fb,
+                                                    const uint32_t* __restrict__ flags, Params3 p) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p.N) return;
+  const bool solid = (flags[idx] & FLAG_SOLID3) != 0u;
+#pragma unroll
+  for (int i = 0; i < 19; i++) {
+    const float v = solid ? 0.0f : feq3(i, 1.0f, p.uIn, 0.0f, 0.0f);
+    S::store(fa + size_t(i) * p.N, idx, v);
+    S::store(fb + size_t(i) * p.N, idx, v);
+  }
+}
+
+// src/shaders/aux3d.ts:32-100: flags from the SDF and refill of cells the body uncovered. Force
+// slots are not needed here (the list kernel reduces per block), so none are assigned.
+template <class S>
+__global__ __launch_bounds__(256) void flags3_kernel(const float* __restrict__ sdf, uint32_t* __restrict__ flags,
+                                                     typename S::elem* __restrict__ fa, typename S::elem* __restrict__ fb,
+                                                     const float4* __restrict__ mac, Params3 p) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= p.N) return;
+  const int W = p.W, H = p.H, D = p.D;
+  const int z = int(idx / p.WH);
+  const int y = int((idx - uint32_t(z) * p.WH) / uint32_t(W));
+  const int x = int(idx - uint32_t(z) * p.WH - uint32_t(y) * uint32_t(W));
+  auto in_range = [&](int qx, int qy, int qz) { return qx >= 0 && qy >= 0 && qz >= 0 && qx < W && qy < H && qz < D; };
+  auto solid_at = [&](int qx, int qy, int qz) {
+    if (qx == 0 || qx == W - 1) return false;
+    return sdf[uint32_t(qx + W * (qy + H * qz))] < 0.0f;
+  };
+  const uint32_t old = flags[idx];
+  if (solid_at(x, y, z)) {
+    flags[idx] = FLAG_SOLID3;
+    return;
+  }
+  uint32_t bits = 0u;
+  if (x == 0) bits = FLAG_INLET3;
+  if (x == W - 1) bits = FLAG_OUTLET3;
+#pragma unroll
+  for (int i = 1; i < 19; i++) {
+    const int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
+    if (in_range(sx, sy, sz)) {
+      if (solid_at(sx, sy, sz)) bits |= 1u << i;
+      continue;
+    }
+    bits |= 1u << i;
+  }
+  flags[idx] = bits;
+  if (old & FLAG_SOLID3) {
+    float rho = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f, n = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+      const int qx = x + CX3[i], qy = y + CY3[i], qz = z + CZ3[i];
+      if (!in_range(qx, qy, qz)) continue;
+      const float4 m = mac[uint32_t(qx + W * (qy + H * qz))];
+      if (m.w != 0.0f) {
+        rho += m.w;
+        ux += m.x;
+        uy += m.y;
+        uz += m.z;
+        n += 1.0f;
+      }
+    }
+    if (n > 0.0f) {
+      rho /= n;
+      ux /= n;
+      uy /= n;
+      uz /= n;
+    } else {
+      rho = 1.0f;
+      ux = p.uIn;
+      uy = 0.0f;
+      uz = 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+      const float v = feq3(i, rho, ux, uy, uz);
+      S::store(fa + size_t(i) * p.N, idx, v);
+      S::store(fb + size_t(i) * p.N, idx, v);
+    }
+  }
+}
+
+__global__ void reset_state_kernel(uint32_t* state) { state[0] = 0u; }
+
+// ============================================================================ launchers
+static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+uint32_t list_grid3(uint32_t nlist) { return nlist ? blocks_for(nlist, 256) : 1u; }
+
+template <class S>
+static void step3_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
+                    uint32_t nlist, const Params3& p, const float* feq_in, const ForceSink& fs, int bb,
+                    hipStream_t st, hipEvent_t mid) {
+  using E = typename S::elem;
+  FeqIn fin;
+  for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
+  step3_bulk<S><<<blocks_for(p.N, bb), bb, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, p, fin);
+  if (mid) (void)hipEventRecord(mid, st);
+  if (nlist || fs.sample)
+    step3_list<S><<<list_grid3(nlist), 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, sdf, list,
+                                                      nlist, p, fin, fs);
+}
+
+void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
+                  uint32_t nlist, const Params3& p, const float* feq_in, const ForceSink& fs, int bulk_block,
+                  hipStream_t st, hipEvent_t mid) {
+  switch (prec) {
+    case 1: step3_t<StoreF16>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
+    default: step3_t<StoreF32>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
+  }
+}
+
+template <class S>
+static void macro3_t(const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st) {
+  macro3_kernel<S><<<blocks_for(p.N, 256), 256, 0, st>>>(static_cast<const typename S::elem*>(f), flags, mac, p);
+}
+void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st) {
+  switch (prec) {
+    case 1: macro3_t<StoreF16>(f, flags, mac, p, st); break;
+    case 2: macro3_t<StoreBF16>(f, flags, mac, p, st); break;
+    default: macro3_t<StoreF32>(f, flags, mac, p, st); break;
+  }
+}
+
+template <class S>
+static void init3_t(void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st) {
+  using E = typename S::elem;
+  init3_kernel<S><<<blocks_for(p.N, 256), 256, 0, st>>>(static_cast<E*>(fa), static_cast<E*>(fb), flags, p);
+}
+void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st) {
+  switch (prec) {
+    case 1: init3_t<StoreF16>(fa, fb, flags, p, st); break;
+    case 2: init3_t<StoreBF16>(fa, fb, flags, p, st); break;
+    default: init3_t<StoreF32>(fa, fb, flags, p, st); break;
+  }
+}
+
+template <class S>
+static void flags3_t(const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac, const Params3& p,
+                     hipStream_t st) {
+  using E = typename S::elem;
+  flags3_kernel<S><<<blocks_for(p.N, 256), 256, 0, st>>>(sdf, flags, static_cast<E*>(fa), static_cast<E*>(fb), mac, p);
+}
+void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
+                   const Params3& p, hipStream_t st) {
+  switch (prec) {
+    case 1: flags3_t<StoreF16>(sdf, flags, fa, fb, mac, p, st); break;
+    case 2: flags3_t<StoreBF16>(sdf, flags, fa, fb, mac, p, st); break;
+    default: flags3_t<StoreF32>(sdf, flags, fa, fb, mac, p, st); break;
+  }
+}
+
+void launch_reset_state(uint32_t* state, hipStream_t st) { reset_state_kernel<<<1, 1, 0, st>>>(state); }
+
+}  // namespace lbm

@@ -1,0 +1,113 @@
+// Device-side lattice tables, storage codecs and kernel parameter blocks (gfx950).
+//
+// Tables: D2Q9 src/lattice.ts:2-36, D3Q19 src/lattice3d.ts:4-26. Codecs: FP32 and FP16 x 2^15
+// follow src/shaders/common3d.ts:92-103; BF16 (round to nearest even of the float32 bit pattern,
+// no scale, no clamp) has no counterpart in the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+// ---------------------------------------------------------------- D2Q9
+constexpr int CX2[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+constexpr int CY2[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+constexpr int OPP2[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr int MIRROR_Y2[9] = {0, 1, 4, 3, 2, 8, 7, 6, 5};
+constexpr float W2[9] = {4.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36};
+constexpr uint32_t FLAG_SOLID2 = 1u << 9;
+constexpr uint32_t FLAG_INLET2 = 1u << 10;
+constexpr uint32_t FLAG_OUTLET2 = 1u << 11;
+constexpr uint32_t BIDX_SHIFT2 = 12;
+// Direction bits a plain inlet / outlet column node carries: the upstream node is off the grid
+// exactly for the directions with c_x = +1 (inlet) or c_x = -1 (outlet).
+constexpr uint32_t INLET_DIRS2 = (1u << 1) | (1u << 5) | (1u << 8);
+constexpr uint32_t OUTLET_DIRS2 = (1u << 3) | (1u << 6) | (1u << 7);
+
+// ---------------------------------------------------------------- D3Q19
+constexpr int CX3[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+constexpr int CY3[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+constexpr int CZ3[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+constexpr float W3[19] = {1.0f / 3,  1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18,
+                          1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36,
+                          1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36};
+constexpr uint32_t FLAG_SOLID3 = 1u << 19;
+constexpr uint32_t FLAG_INLET3 = 1u << 20;
+constexpr uint32_t FLAG_OUTLET3 = 1u << 21;
+constexpr uint32_t INLET_DIRS3 = (1u << 1) | (1u << 7) | (1u << 9) | (1u << 13) | (1u << 15);
+constexpr uint32_t OUTLET_DIRS3 = (1u << 2) | (1u << 8) | (1u << 10) | (1u << 14) | (1u << 16);
+
+__host__ __device__ constexpr int opp3(int i) { return i == 0 ? 0 : (i % 2 ? i + 1 : i - 1); }
+__host__ __device__ constexpr int find3(int cx, int cy, int cz) {
+  int r = -1;
+  for (int i = 0; i < 19; i++)
+    if (CX3[i] == cx && CY3[i] == cy && CZ3[i] == cz) r = i;
+  return r;
+}
+__host__ __device__ constexpr int mirror_y3(int i) { return find3(CX3[i], -CY3[i], CZ3[i]); }
+__host__ __device__ constexpr int mirror_z3(int i) { return find3(CX3[i], CY3[i], -CZ3[i]); }
+
+// ---------------------------------------------------------------- WGSL built-ins
+__device__ __forceinline__ float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
+__device__ __forceinline__ float smoothstep(float lo, float hi, float x) {
+  float t = wclamp((x - lo) / (hi - lo), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// ---------------------------------------------------------------- storage codecs
+struct StoreF32 {
+  using elem = float;
+  __device__ static __forceinline__ float load(const elem* p, uint32_t k) { return p[k]; }
+  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) { p[k] = v; }
+};
+struct StoreF16 {
+  using elem = _Float16;
+  __device__ static __forceinline__ float load(const elem* p, uint32_t k) {
+    return float(p[k]) * (1.0f / 32768.0f);
+  }
+  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) {
+    p[k] = _Float16(wclamp(v, -1.99f, 1.99f) * 32768.0f);  // v_cvt_f16_f32: round to nearest even
+  }
+};
+struct StoreBF16 {
+  using elem = uint16_t;
+  __device__ static __forceinline__ float load(const elem* p, uint32_t k) {
+    return __uint_as_float(uint32_t(p[k]) << 16);
+  }
+  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) {
+    uint32_t x = __float_as_uint(v);
+    uint32_t r = ((x & 0x7fffffffu) > 0x7f800000u) ? ((x >> 16) | 0x40u) : ((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+    p[k] = uint16_t(r);
+  }
+};
+
+// ---------------------------------------------------------------- parameter blocks
+// Values of the reference's Params uniform (src/shaders/common.ts:55-63) after writeParams
+// (src/solver.ts:180-197), passed by value as kernel arguments.
+struct Params2 {
+  int32_t W, H;
+  uint32_t N;
+  int32_t left, right, bottom, top, parabolic, forcing;
+  float tau, lambda, smagC2, uIn, spongeStart, tauSponge, uLid, fx, fy, absorb, inletLayer;
+};
+// Params3 (src/shaders/common3d.ts:49-56) after Solver3D.writeParams (src/solver3d.ts:197-208).
+struct Params3 {
+  int32_t W, H, D;
+  uint32_t N, WH;
+  float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
+};
+
+// Where the step kernels put a force sample: per-block FP64 partials, a completion counter,
+// the FP64 history ring and its sample counter. sample == 0 skips the reduction.
+struct ForceSink {
+  double* partials;
+  uint32_t* counter;
+  double* ring;
+  uint32_t* state;
+  int32_t sample;
+};
+
+constexpr int HISTORY_LEN = 8192;
+
+}  // namespace lbm

@@ -1,0 +1,79 @@
+// Deterministic FP64 force reduction used by the boundary-list step kernels.
+//
+// Replaces the reference's single-workgroup FP32 reduce pass (src/shaders/aux.ts:144-172,
+// src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every block of the list kernel
+// sums its cells' momentum-exchange forces in double with wave64 butterfly shuffles and LDS, writes
+// one partial, and the last block to finish (completion counter) sums the partials in block order
+// and appends the sample to the history ring. The summation order is fixed, so a sample is
+// bit-reproducible run to run.
+#pragma once
+#include "lbm_device.hpp"
+
+namespace lbm {
+
+template <int DIM>
+__device__ __forceinline__ void wave_sum(double (&v)[DIM]) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) v[d] += __shfl_xor(v[d], off, 64);
+}
+
+// Block-wide sum; the result is valid in thread 0. blockDim.x must be a multiple of 64, <= 1024.
+template <int DIM>
+__device__ __forceinline__ void block_sum(double (&v)[DIM], double (*lds)[16]) {
+  wave_sum<DIM>(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int d = 0; d < DIM; d++) lds[d][wave] = v[d];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+      double s = 0.0;
+      for (int w = 0; w < nwaves; w++) s += lds[d][w];
+      v[d] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Every thread of every block of the grid must call this (no early return before it).
+template <int DIM>
+__device__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
+  __shared__ double lds[DIM][16];
+  __shared__ int is_last;
+  double v[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) v[d] = double(force[d]);
+  block_sum<DIM>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < DIM; d++) fs.partials[size_t(blockIdx.x) * DIM + d] = v[d];
+    __threadfence();
+    const uint32_t prev = atomicAdd(fs.counter, 1u);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) acc[d] = 0.0;
+  for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+      acc[d] += __hip_atomic_load(fs.partials + size_t(k) * DIM + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<DIM>(acc, lds);
+  if (threadIdx.x == 0) {
+    const uint32_t st = *fs.state;
+    const uint32_t slot = st % HISTORY_LEN;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) fs.ring[size_t(slot) * DIM + d] = acc[d];
+    *fs.state = st + 1;
+    atomicExch(fs.counter, 0u);
+  }
+}
+
+}  // namespace lbm

@@ -1,0 +1,395 @@
+"""ctypes binding of the C ABI (include/lbm.h) with the reference's Solver / Solver3D surface.
+
+Method names and argument meaning follow src/solver.ts:50-340 and src/solver3d.ts:42-319 (both the
+camelCase names of the reference and snake_case aliases exist). All compute runs in the gfx950
+library ``cells_amd/liblbm_hip.so``; there is no CPU path: if the library is missing or no GPU is
+present, creating a solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from collections import namedtuple
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+
+LBM_OK = 0
+X_PERIODIC, X_WALL, X_OPEN = 0, 1, 2  # XMode (src/lattice.ts:17-21)
+Y_PERIODIC, Y_NOSLIP, Y_SLIP, Y_MOVING = 0, 1, 2, 3  # YMode (src/lattice.ts:23-28)
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+HISTORY_LEN = 8192
+MAGIC_LAMBDA = 3 / 16
+
+ForceSample = namedtuple("ForceSample", "step fx fy")
+ForceSample3 = namedtuple("ForceSample3", "step fx fy fz")
+
+
+class Config2D(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32),
+        ("left", C.c_int32), ("right", C.c_int32), ("bottom", C.c_int32), ("top", C.c_int32),
+        ("parabolic", C.c_int32), ("workgroup_size", C.c_int32), ("force_every", C.c_int32), ("_pad0", C.c_int32),
+        ("tau", C.c_double), ("lambda_", C.c_double), ("smagorinsky", C.c_double), ("u_in", C.c_double),
+        ("u_lid", C.c_double), ("sponge_fraction", C.c_double), ("sponge_tau", C.c_double),
+        ("body_force", C.c_double * 2), ("absorb", C.c_double), ("inlet_layer_fraction", C.c_double),
+    ]
+
+
+class Config3D(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("depth", C.c_int32), ("precision", C.c_int32),
+        ("workgroup_size", C.c_int32), ("force_every", C.c_int32),
+        ("tau", C.c_double), ("lambda_", C.c_double), ("smagorinsky", C.c_double), ("u_in", C.c_double),
+        ("sponge_fraction", C.c_double), ("sponge_tau", C.c_double), ("absorb", C.c_double),
+    ]
+
+
+# Reference config keys (camelCase, src/solver.ts:15-40, src/solver3d.ts:11-31) -> struct fields.
+KEYS2 = {
+    "width": "width", "height": "height", "left": "left", "right": "right", "bottom": "bottom", "top": "top",
+    "tau": "tau", "lambda": "lambda_", "smagorinsky": "smagorinsky", "uIn": "u_in", "parabolic": "parabolic",
+    "uLid": "u_lid", "spongeFraction": "sponge_fraction", "spongeTau": "sponge_tau", "bodyForce": "body_force",
+    "absorb": "absorb", "inletLayerFraction": "inlet_layer_fraction", "workgroupSize": "workgroup_size",
+    "forceEvery": "force_every",
+}
+KEYS3 = {
+    "width": "width", "height": "height", "depth": "depth", "precision": "precision", "tau": "tau",
+    "lambda": "lambda_", "smagorinsky": "smagorinsky", "uIn": "u_in", "spongeFraction": "sponge_fraction",
+    "spongeTau": "sponge_tau", "absorb": "absorb", "workgroupSize": "workgroup_size", "forceEvery": "force_every",
+}
+
+DEFAULTS2 = dict(lambda_=MAGIC_LAMBDA, smagorinsky=0.0, u_in=0.0, parabolic=0, u_lid=0.0, sponge_fraction=0.0,
+                 sponge_tau=1.0, body_force=(0.0, 0.0), absorb=0.0, inlet_layer_fraction=0.0, workgroup_size=128,
+                 force_every=1)
+DEFAULTS3 = dict(lambda_=float("nan"), smagorinsky=0.0, u_in=0.0, sponge_fraction=0.0, sponge_tau=1.0, absorb=0.0,
+                 workgroup_size=128, force_every=1)
+
+
+def make_config2d(cfg: dict) -> Config2D:
+    """SolverConfig with the reference defaults (src/solver.ts:98-112)."""
+    c = Config2D()
+    vals = dict(DEFAULTS2)
+    for k, v in cfg.items():
+        vals[KEYS2.get(k, k)] = v
+    for k, v in vals.items():
+        if k == "body_force":
+            c.body_force[0], c.body_force[1] = float(v[0]), float(v[1])
+        elif k == "parabolic":
+            c.parabolic = 1 if v else 0
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def make_config3d(cfg: dict) -> Config3D:
+    """Solver3DConfig with the reference defaults (src/solver3d.ts:110-119); lambda None or NaN follows
+    trtLambda3(tau)."""
+    c = Config3D()
+    vals = dict(DEFAULTS3)
+    for k, v in cfg.items():
+        vals[KEYS3.get(k, k)] = v
+    for k, v in vals.items():
+        if k == "precision":
+            v = PRECISIONS[v] if isinstance(v, str) else int(v)
+        if k == "lambda_" and v is None:
+            v = float("nan")
+        setattr(c, k, v)
+    return c
+
+
+def trt_lambda3(tau: float) -> float:
+    """trtLambda3 (src/lattice3d.ts:29-31)."""
+    return min(MAGIC_LAMBDA, 50 * (tau - 0.5) ** 2)
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    sig = {
+        "lbm_abi_version": ([], C.c_int),
+        "lbm_last_error": ([], C.c_char_p),
+        "lbm_config2d_defaults": ([C.POINTER(Config2D)], None),
+        "lbm_config3d_defaults": ([C.POINTER(Config3D)], None),
+        "lbm_create_2d": ([C.POINTER(Config2D), C.c_int, C.POINTER(P)], C.c_int),
+        "lbm_create_3d": ([C.POINTER(Config3D), C.c_int, C.POINTER(P)], C.c_int),
+        "lbm_destroy": ([P], None),
+        "lbm_update_2d": ([P, C.POINTER(Config2D)], C.c_int),
+        "lbm_update_3d": ([P, C.POINTER(Config3D)], C.c_int),
+        "lbm_set_inlet_ramp": ([P, C.c_int, C.c_double, C.c_double], C.c_int),
+        "lbm_set_sdf": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_init_field": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_reset_forces": ([P], C.c_int),
+        "lbm_enqueue_steps": ([P, C.c_int64], C.c_int),
+        "lbm_enqueue_macro": ([P], C.c_int),
+        "lbm_run": ([P, C.c_int64, C.c_int64], C.c_int),
+        "lbm_sync": ([P], C.c_int),
+        "lbm_read_macro": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_read_forces_begin": ([P, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_read_forces_wait": ([P, C.c_int64], C.c_int),
+        "lbm_read_forces_end": ([P, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_read_forces": ([P, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_info": ([P, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                      C.POINTER(C.c_int32)], C.c_int),
+        "lbm_read_populations": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_read_flags": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_time_steps": ([P, C.c_int64, C.POINTER(C.c_double)], C.c_int),
+        "lbm_profile_steps": ([P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)],
+                              C.c_int),
+        "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+class LbmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != LBM_OK:
+        raise LbmError(rc, _lib.lbm_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _Base:
+    dim = 2
+    Q = 9
+
+    def __init__(self, handle, cfg: dict):
+        self._h = handle
+        self.cfg = cfg
+        dims = np.zeros(3, dtype=np.int32)
+        n = C.c_int64()
+        step = C.c_int64()
+        q = C.c_int32()
+        prec = C.c_int32()
+        _check(_lib.lbm_info(self._h, _ptr(dims), C.byref(n), C.byref(step), C.byref(q), C.byref(prec)))
+        self.W, self.H, self.D = (int(v) for v in dims)
+        self.N = int(n.value)
+        self.precision = int(prec.value)
+
+    # -- readonly step counter (src/solver.ts:57)
+    @property
+    def step(self) -> int:
+        step = C.c_int64()
+        _check(_lib.lbm_info(self._h, None, None, C.byref(step), None, None))
+        return int(step.value)
+
+    def set_sdf(self, sdf):
+        a = np.ascontiguousarray(sdf, dtype=np.float32)
+        _check(_lib.lbm_set_sdf(self._h, _ptr(a), a.size))
+
+    def reset_forces(self):
+        _check(_lib.lbm_reset_forces(self._h))
+
+    def encode_steps(self, n: int, _pass=None):
+        """encodeSteps: the pass argument of the reference has no equivalent and is ignored."""
+        _check(_lib.lbm_enqueue_steps(self._h, int(n)))
+
+    def encode_macro(self, _pass=None):
+        _check(_lib.lbm_enqueue_macro(self._h))
+
+    def run(self, n: int, chunk: int = 500):
+        _check(_lib.lbm_run(self._h, int(n), int(chunk)))
+
+    def sync(self):
+        _check(_lib.lbm_sync(self._h))
+
+    def set_inlet_ramp(self, kind: str, u_target: float, steps: float):
+        k = {"off": 0, "linear": 1, "smoothstep": 2}[kind]
+        _check(_lib.lbm_set_inlet_ramp(self._h, k, float(u_target), float(steps)))
+
+    def read_macro(self) -> np.ndarray:
+        out = np.empty(self.N * 4, dtype=np.float32)
+        _check(_lib.lbm_read_macro(self._h, _ptr(out), out.size))
+        return out
+
+    def read_forces_raw(self):
+        """(steps int64[n], values float64[n, dim]) since the last read."""
+        steps = np.empty(HISTORY_LEN, dtype=np.int64)
+        vals = np.empty(HISTORY_LEN * self.dim, dtype=np.float64)
+        n = C.c_int64()
+        _check(_lib.lbm_read_forces(self._h, _ptr(steps), _ptr(vals), HISTORY_LEN, C.byref(n)))
+        k = int(n.value)
+        return steps[:k].copy(), vals[: k * self.dim].reshape(k, self.dim).copy()
+
+    def read_forces(self):
+        steps, vals = self.read_forces_raw()
+        T = ForceSample if self.dim == 2 else ForceSample3
+        return [T(int(s), *map(float, v)) for s, v in zip(steps, vals)]
+
+    def read_populations(self) -> np.ndarray:
+        """Current distribution buffer, decoded to float32 (Q, N) of (f - w)."""
+        if self.precision == 0:
+            out = np.empty(self.Q * self.N, dtype=np.float32)
+            _check(_lib.lbm_read_populations(self._h, _ptr(out), out.nbytes))
+            return out.reshape(self.Q, self.N)
+        raw = self.read_populations_raw()
+        return decode(raw, self.precision).reshape(self.Q, self.N)
+
+    def read_populations_raw(self) -> np.ndarray:
+        dt = np.float32 if self.precision == 0 else np.uint16
+        out = np.empty(self.Q * self.N, dtype=dt)
+        _check(_lib.lbm_read_populations(self._h, _ptr(out), out.nbytes))
+        return out
+
+    def read_flags(self) -> np.ndarray:
+        out = np.empty(self.N, dtype=np.uint32)
+        _check(_lib.lbm_read_flags(self._h, _ptr(out), out.size))
+        return out
+
+    def time_steps(self, n: int) -> float:
+        """Milliseconds of n steps between two HIP events on the solver's stream."""
+        ms = C.c_double()
+        _check(_lib.lbm_time_steps(self._h, int(n), C.byref(ms)))
+        return float(ms.value)
+
+    def profile_steps(self, n: int):
+        """(total, bulk-kernel, list-kernel) milliseconds of n steps, events around every kernel."""
+        t, b, l = C.c_double(), C.c_double(), C.c_double()
+        _check(_lib.lbm_profile_steps(self._h, int(n), C.byref(t), C.byref(b), C.byref(l)))
+        return float(t.value), float(b.value), float(l.value)
+
+    def node_counts(self):
+        """(bulk, list, solid) node counts."""
+        b, l, s = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(_lib.lbm_node_counts(self._h, C.byref(b), C.byref(l), C.byref(s)))
+        return int(b.value), int(l.value), int(s.value)
+
+    def destroy(self):
+        if self._h:
+            _lib.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # camelCase names of the reference
+    setSdf = set_sdf
+    resetForces = reset_forces
+    encodeSteps = encode_steps
+    encodeMacro = encode_macro
+    readMacro = read_macro
+    readForces = read_forces
+
+
+def decode(raw: np.ndarray, precision: int) -> np.ndarray:
+    """Storage codec decode (src/shaders/common3d.ts:95): FP16 x 2^-15, BF16 as the top half of a float32."""
+    if precision == 1:
+        return raw.view(np.float16).astype(np.float32) * np.float32(1.0 / 32768.0)
+    if precision == 2:
+        return (raw.astype(np.uint32) << 16).view(np.float32)
+    return raw
+
+
+class Solver(_Base):
+    """D2Q9 solver (src/solver.ts:50-340)."""
+
+    dim = 2
+    Q = 9
+
+    @staticmethod
+    def create(config: dict, device: int = 0) -> "Solver":
+        lib = load_library()
+        c = make_config2d(config)
+        h = C.c_void_p()
+        _check(lib.lbm_create_2d(C.byref(c), int(device), C.byref(h)))
+        full = dict(DEFAULTS2)
+        full.update({KEYS2.get(k, k): v for k, v in config.items()})
+        return Solver(h, full)
+
+    def update(self, changes: Optional[dict] = None, **kw):
+        """update (src/solver.ts:175-178): merge into the config and rewrite the parameters."""
+        ch = dict(changes or {}, **kw)
+        self.cfg.update({KEYS2.get(k, k): v for k, v in ch.items()})
+        _check(_lib.lbm_update_2d(self._h, C.byref(make_config2d(self.cfg))))
+
+    def init_field(self, macro=None):
+        """initField (src/solver.ts:220-234): equilibrium at the (rho, ux, uy, -) field, or rest."""
+        if macro is None:
+            _check(_lib.lbm_init_field(self._h, None, 0))
+        else:
+            m = np.ascontiguousarray(macro, dtype=np.float32)
+            _check(_lib.lbm_init_field(self._h, _ptr(m), m.size))
+
+    initField = init_field
+
+
+class Solver3D(_Base):
+    """D3Q19 solver (src/solver3d.ts:42-319)."""
+
+    dim = 3
+    Q = 19
+
+    @staticmethod
+    def create(config: dict, device: int = 0) -> "Solver3D":
+        lib = load_library()
+        c = make_config3d(config)
+        h = C.c_void_p()
+        _check(lib.lbm_create_3d(C.byref(c), int(device), C.byref(h)))
+        full = dict(DEFAULTS3)
+        full.update({KEYS3.get(k, k): v for k, v in config.items()})
+        return Solver3D(h, full)
+
+    def _push(self):
+        _check(_lib.lbm_update_3d(self._h, C.byref(make_config3d(self.cfg))))
+
+    def set_tau(self, tau: float):
+        """setTau (src/solver3d.ts:182-185)."""
+        self.cfg["tau"] = tau
+        self._push()
+
+    def set_smagorinsky(self, cs: float):
+        """setSmagorinsky (src/solver3d.ts:187-190)."""
+        self.cfg["smagorinsky"] = cs
+        self._push()
+
+    def set_inlet(self, u: float):
+        """setInlet (src/solver3d.ts:192-195)."""
+        self.cfg["u_in"] = u
+        self._push()
+
+    def init_field(self):
+        """initField (src/solver3d.ts:231-242): inflow equilibrium in fluid, zero in solids."""
+        _check(_lib.lbm_init_field(self._h, None, 0))
+
+    setTau = set_tau
+    setSmagorinsky = set_smagorinsky
+    setInlet = set_inlet
+    initField = init_field
+
+
+def bytes_per_cell(q: int, precision: int) -> int:
+    """Algorithmic bytes per cell update (src/bench.ts:24-28): Q loads + Q stores + one flag word."""
+    b = 4 if precision == 0 else 2
+    return q * b * 2 + 4
+
+
+def lambda_for(tau: float, lam: Optional[float]) -> float:
+    return trt_lambda3(tau) if lam is None or (isinstance(lam, float) and math.isnan(lam)) else lam

@@ -1,0 +1,175 @@
+/*
+ * lbm.h - C ABI of the MI355X lattice Boltzmann wind-tunnel solver.
+ *
+ * One opaque handle per tunnel. Every call returns an int status (LBM_OK = 0,
+ * negative on failure); lbm_last_error() gives the message of the last failure
+ * on the calling thread. Plain pointers and sizes only: no torch, no HIP types.
+ *
+ * The handle replaces the reference's WebGPU-backed classes:
+ *   D2Q9  Solver   (src/solver.ts:50-340)
+ *   D3Q19 Solver3D (src/solver3d.ts:42-319)
+ * Each entry point names the method it stands in for. GPU work is enqueued on
+ * one HIP stream per handle and runs in order, as the reference's single
+ * WebGPU queue does; setters are stream-ordered (they affect the steps
+ * enqueued after them), like queue.writeBuffer.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* Status codes. */
+#define LBM_OK 0
+#define LBM_ERR_INVALID (-1)     /* bad argument or configuration */
+#define LBM_ERR_ALLOC (-2)       /* device allocation failed ("Solver3D allocation failed: ...") */
+#define LBM_ERR_DEVICE (-3)      /* HIP runtime error */
+#define LBM_ERR_UNSUPPORTED (-4) /* feature not available (e.g. precision) */
+
+/* Boundary modes (src/lattice.ts:17-28). */
+#define LBM_X_PERIODIC 0
+#define LBM_X_WALL 1
+#define LBM_X_OPEN 2 /* Zou-He inlet on the left, Zou-He pressure outlet on the right */
+#define LBM_Y_PERIODIC 0
+#define LBM_Y_NOSLIP 1
+#define LBM_Y_SLIP 2
+#define LBM_Y_MOVING 3
+
+/* Storage precision of the distribution buffers (src/shaders/common3d.ts:4, plus bf16). */
+#define LBM_FP32 0
+#define LBM_FP16 1 /* f16 of (f - w) * 2^15, clamped to +-1.99 before scaling */
+#define LBM_BF16 2 /* bf16 of (f - w), round to nearest even, no clamp */
+
+/* Length of the on-device force history ring (src/shaders/aux.ts:4, aux3d.ts:5). */
+#define LBM_HISTORY_LEN 8192
+
+/* SolverConfig (src/solver.ts:15-40) with its defaults (src/solver.ts:98-112). */
+typedef struct lbm_config2d {
+  int32_t width, height;
+  int32_t left, right, bottom, top; /* LBM_X_* / LBM_Y_* */
+  int32_t parabolic;                /* bool: parabolic inlet profile */
+  int32_t workgroup_size;           /* threads per block; 0 = autotuned */
+  int32_t force_every;              /* sample the obstacle force every n steps; 0 = off */
+  int32_t _pad0;
+  double tau;
+  double lambda;      /* TRT magic parameter, default 3/16 */
+  double smagorinsky; /* C_s; 0 disables the subgrid model */
+  double u_in;
+  double u_lid;
+  double sponge_fraction;
+  double sponge_tau;
+  double body_force[2];
+  double absorb;
+  double inlet_layer_fraction;
+} lbm_config2d;
+
+/* Solver3DConfig (src/solver3d.ts:11-31) with its defaults (src/solver3d.ts:110-119). */
+typedef struct lbm_config3d {
+  int32_t width, height, depth;
+  int32_t precision;      /* LBM_FP32 / LBM_FP16 / LBM_BF16 */
+  int32_t workgroup_size; /* 0 = autotuned */
+  int32_t force_every;
+  double tau;
+  double lambda; /* NaN = follow tau through min(3/16, 50 (tau - 1/2)^2) (src/lattice3d.ts:29-31) */
+  double smagorinsky;
+  double u_in;
+  double sponge_fraction;
+  double sponge_tau;
+  double absorb;
+} lbm_config3d;
+
+typedef struct lbm_solver lbm_solver;
+
+/* Library-wide. */
+int lbm_abi_version(void);
+const char* lbm_last_error(void);
+/* Fills the reference defaults; the caller then sets width/height/(depth)/tau (and modes in 2D). */
+void lbm_config2d_defaults(lbm_config2d* cfg);
+void lbm_config3d_defaults(lbm_config3d* cfg);
+
+/* Solver.create (src/solver.ts:87-92): allocates, builds flags for an empty SDF (all FAR). */
+int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out);
+/* Solver3D.create (src/solver3d.ts:79-104). On failure nothing stays allocated. */
+int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out);
+/* Solver.destroy / Solver3D.destroy (src/solver.ts:337-339, src/solver3d.ts:317-319). NULL is a no-op. */
+void lbm_destroy(lbm_solver* s);
+
+/* Solver.update (src/solver.ts:175-197): replaces the whole config (width/height must not change). */
+int lbm_update_2d(lbm_solver* s, const lbm_config2d* cfg);
+/* Solver3D.setTau / setSmagorinsky / setInlet (src/solver3d.ts:182-208): replaces the whole config. */
+int lbm_update_3d(lbm_solver* s, const lbm_config3d* cfg);
+
+/* No counterpart in the reference, whose drivers ramp the inlet per chunk through update / setInlet
+ * (src/cases.ts:52-54, src/cases3d.ts:46): evaluate the inlet speed per step as
+ * u_target * r(min(1, step / steps)), r = t (kind 1) or t^2 (3 - 2t) (kind 2); kind 0 turns it off and
+ * restores the configured u_in. The step index is that of the step being taken (0 for the first). */
+int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps);
+
+/* setSdf (src/solver.ts:200-203, src/solver3d.ts:211-214): n = N floats, negative inside solids.
+ * Rebuilds flags, the boundary-cell list (in cell order) and refills cells the body uncovered. */
+int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n);
+
+/* initField (src/solver.ts:220-234, src/solver3d.ts:231-242). 2D: macro = N*4 floats (rho, ux, uy, -)
+ * or NULL for rest; 3D: macro must be NULL (inflow equilibrium). Resets step, parity and forces. */
+int lbm_init_field(lbm_solver* s, const float* macro, size_t n);
+
+/* resetForces (src/solver.ts:236-241, src/solver3d.ts:244-249). */
+int lbm_reset_forces(lbm_solver* s);
+
+/* encodeSteps (src/solver.ts:271-286): enqueue n steps (+ force samples every force_every). */
+int lbm_enqueue_steps(lbm_solver* s, int64_t n);
+/* encodeMacro (src/solver.ts:288-292): enqueue the macro-field kernel for the current buffer. */
+int lbm_enqueue_macro(lbm_solver* s);
+/* run (src/solver.ts:295-307): n steps in chunks, macro after each chunk, then wait for the GPU. */
+int lbm_run(lbm_solver* s, int64_t n, int64_t chunk);
+/* Waits for all enqueued work (queue.onSubmittedWorkDone). */
+int lbm_sync(lbm_solver* s);
+
+/* readMacro (src/solver.ts:309-311, src/solver3d.ts:290-292): N*4 floats, 2D (rho, ux, uy, solid),
+ * 3D (ux, uy, uz, rho). Reads the macro buffer as of the last enqueued macro kernel. */
+int lbm_read_macro(lbm_solver* s, float* out, size_t n);
+
+/* readForces (src/solver.ts:315-335, src/solver3d.ts:295-315), split so that the copy can be waited
+ * for off the caller's thread while the generation check stays on it:
+ *   begin  enqueues a copy of the ring and its counter and returns a ticket (main thread);
+ *   wait   blocks until that copy landed (any thread);
+ *   end    returns the samples produced since the last read, oldest first, as step numbers and
+ *          dim = 2 (2D) or 3 (3D) doubles per sample; a read overtaken by lbm_reset_forces returns 0.
+ * cap is the capacity of steps[] in samples; at most LBM_HISTORY_LEN samples are ever returned. */
+int lbm_read_forces_begin(lbm_solver* s, int64_t* ticket);
+int lbm_read_forces_wait(lbm_solver* s, int64_t ticket);
+int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* values, int64_t cap,
+                        int64_t* count);
+/* begin + wait + end. */
+int lbm_read_forces(lbm_solver* s, int64_t* steps, double* values, int64_t cap, int64_t* count);
+
+/* Readonly fields W, H, D, N, step (src/solver.ts:51-57, src/solver3d.ts:43-50). */
+int lbm_info(const lbm_solver* s, int32_t* dims /* W, H, D */, int64_t* n_cells, int64_t* step,
+             int32_t* lattice_q, int32_t* precision);
+
+/* No counterpart in the reference (its buffers are read only by the renderer): copies the current
+ * distribution buffer, Q*N values in storage format (float / f16 bits / bf16 bits), for parity tests. */
+int lbm_read_populations(lbm_solver* s, void* out, size_t bytes);
+/* No counterpart in the reference: copies the per-cell flag words (N uint32, same bit layout). */
+int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n);
+
+/* Timing, replacing the timestamp queries around one compute pass of src/bench.ts:35-58.
+ * lbm_time_steps enqueues n steps between two HIP events on the handle's stream and returns the
+ * elapsed milliseconds. lbm_profile_steps does the same with events around every kernel of every
+ * step and returns the summed time of the bulk (plain-node) kernels and of the boundary-list
+ * kernels; it perturbs the schedule slightly and is for per-kernel roofline figures. */
+int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
+int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_ms, double* list_ms);
+/* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
+int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_H */

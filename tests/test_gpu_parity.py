@@ -1,0 +1,205 @@
+"""GPU (gfx950 HIP library, through the C ABI) against the oracle, on the same inputs.
+
+Bit-exact where the work is integer (flag words, force slots, node classification); within
+stated tolerances for FP32 arithmetic, whose operation order the GPU compiler may contract into
+FMAs. Each float comparison also prints the error of both the GPU and the FP32 oracle against the
+FP64 oracle, and requires the GPU's to be of the same size.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cells_amd import geometry as g
+from cells_amd.lbm import Solver, Solver3D
+from oracle.oracle import OracleSolver, OracleSolver3D
+from tests.parity_cases import CASES2, case3, taylor_green_macro
+
+pytestmark = pytest.mark.gpu
+
+# Max |f_gpu - f_oracle32| per step count, FP32 storage. Populations are stored as f - w, of size
+# ~1e-2 here; these bounds are ~100-1000 float32 ulps at that size after 100 steps.
+TOL32 = {1: 2e-6, 10: 5e-6, 100: 3e-5}
+# FP16 x 2^15 storage: one quantum at |f - w| ~ 0.03 is 2^-11 / 2^15 * 2^... ~ 1.5e-5; allow a few.
+TOL16 = {1: 8e-5, 10: 2e-4, 100: 6e-4}
+# BF16: 8 significant bits, one quantum at 0.03 is ~1.2e-4.
+TOLBF = {1: 5e-4, 10: 1.5e-3, 100: 4e-3}
+
+
+def _build2(name, real=None):
+    cfg, sdf_fn = CASES2[name]
+    W, H = cfg["width"], cfg["height"]
+    sdf = sdf_fn(W, H) if sdf_fn else None
+    sims = {}
+    sims["gpu"] = Solver.create(dict(cfg))
+    sims["o32"] = OracleSolver(dict(cfg), 32)
+    sims["o64"] = OracleSolver(dict(cfg), 64)
+    for s in sims.values():
+        if sdf is not None:
+            s.set_sdf(sdf)
+        if name == "box":
+            s.init_field(taylor_green_macro(W))
+        else:
+            s.init_field()
+    return sims
+
+
+def _err(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))))
+
+
+@pytest.mark.parametrize("name", list(CASES2))
+def test_d2q9_flags_bit_exact(name):
+    sims = _build2(name)
+    fg = sims["gpu"].read_flags()
+    fo = sims["o32"].read_flags()
+    assert fg.dtype == fo.dtype == np.uint32
+    np.testing.assert_array_equal(fg, fo)
+
+
+@pytest.mark.parametrize("name", list(CASES2))
+def test_d2q9_fields_match_oracle(name):
+    sims = _build2(name)
+    done = 0
+    for n in (1, 10, 100):
+        for s in sims.values():
+            s.run(n - done) if hasattr(s, "sync") else s.steps(n - done)
+        done = n
+        fg = sims["gpu"].read_populations()
+        f32 = sims["o32"].read_populations()
+        f64 = sims["o64"].read_populations()
+        e = _err(fg, f32)
+        eg64, eo64 = _err(fg, f64), _err(f32, f64)
+        print(f"{name} step {n}: |gpu-o32| {e:.3e}  |gpu-o64| {eg64:.3e}  |o32-o64| {eo64:.3e}")
+        assert np.isfinite(fg).all()
+        assert e <= TOL32[n], (name, n, e)
+        assert eg64 <= 3 * eo64 + TOL32[n], (name, n, eg64, eo64)
+
+
+@pytest.mark.parametrize("name", ["tunnel", "channel", "periodic"])
+def test_d2q9_forces_match_oracle(name):
+    sims = _build2(name)
+    for s in sims.values():
+        s.run(300) if hasattr(s, "sync") else s.steps(300)
+    sg, vg = sims["gpu"].read_forces_raw()
+    so, vo = sims["o32"].read_forces_raw()
+    s6, v6 = sims["o64"].read_forces_raw()
+    np.testing.assert_array_equal(sg, so)
+    assert len(sg) == 300 and sg[0] == 1 and sg[-1] == 300
+    scale = np.abs(v6).max()
+    e = np.abs(vg - vo).max() / scale
+    print(f"{name} force: rel |gpu-o32| {e:.3e}, rel |gpu-o64| {np.abs(vg - v6).max() / scale:.3e}, "
+          f"rel |o32-o64| {np.abs(vo - v6).max() / scale:.3e}")
+    assert e < 1e-4
+
+
+def test_d2q9_macro_matches_oracle():
+    sims = _build2("tunnel")
+    sims["gpu"].run(50, 20)
+    sims["o32"].steps(50)
+    mg = sims["gpu"].read_macro().reshape(-1, 4)
+    mo = sims["o32"].macro().reshape(-1, 4)
+    np.testing.assert_array_equal(mg[:, 3], mo[:, 3])  # solid marker
+    assert _err(mg, mo) < 2e-5
+
+
+def test_d2q9_geometry_change_refills():
+    """setSdf after flow has developed: cells the body uncovered are refilled from their neighbours."""
+    name = "tunnel"
+    sims = _build2(name)
+    for s in sims.values():
+        s.run(40) if hasattr(s, "sync") else s.steps(40)
+    cfg = CASES2[name][0]
+    W, H = cfg["width"], cfg["height"]
+    sdf2 = g.empty_sdf(W, H)
+    g.add_circle(sdf2, W, H, 20, 11.5, 3)
+    for s in sims.values():
+        s.set_sdf(sdf2)
+    np.testing.assert_array_equal(sims["gpu"].read_flags(), sims["o32"].read_flags())
+    e = _err(sims["gpu"].read_populations(), sims["o32"].read_populations())
+    assert e < 1e-5
+    for s in sims.values():
+        s.run(20) if hasattr(s, "sync") else s.steps(20)
+    assert _err(sims["gpu"].read_populations(), sims["o32"].read_populations()) < 2e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16"])
+def test_d3q19_fields_match_oracle(precision):
+    cfg, sdf = case3(precision)
+    gpu = Solver3D.create(dict(cfg))
+    o32 = OracleSolver3D(dict(cfg), 32)
+    o64 = OracleSolver3D(dict(cfg), 64)
+    for s in (gpu, o32, o64):
+        s.set_sdf(sdf)
+        s.init_field()
+    np.testing.assert_array_equal(gpu.read_flags(), o32.read_flags())
+    tol = {"fp32": TOL32, "fp16": TOL16, "bf16": TOLBF}[precision]
+    done = 0
+    for n in (1, 10, 100):
+        gpu.run(n - done)
+        o32.steps(n - done)
+        o64.steps(n - done)
+        done = n
+        fg = gpu.read_populations()
+        fo = o32.read_populations()
+        f6 = o64.read_populations()
+        e = _err(fg, fo)
+        print(f"d3q19 {precision} step {n}: |gpu-o32| {e:.3e} |gpu-o64| {_err(fg, f6):.3e} |o32-o64| {_err(fo, f6):.3e}")
+        assert np.isfinite(fg).all()
+        assert e <= tol[n], (precision, n, e)
+    sg, vg = gpu.read_forces_raw()
+    so, vo = o32.read_forces_raw()
+    np.testing.assert_array_equal(sg, so)
+    rel = np.abs(vg - vo).max() / np.abs(vo).max()
+    print(f"d3q19 {precision} force rel err {rel:.3e}")
+    assert rel < {"fp32": 1e-4, "fp16": 2e-3, "bf16": 2e-2}[precision]
+
+
+def test_d3q19_smagorinsky_and_setters():
+    cfg, sdf = case3("fp32", smag=0.12)
+    gpu = Solver3D.create(dict(cfg))
+    o32 = OracleSolver3D(dict(cfg), 32)
+    for s in (gpu, o32):
+        s.set_sdf(sdf)
+        s.init_field()
+    gpu.run(20)
+    o32.steps(20)
+    for s in (gpu, o32):
+        s.set_tau(0.56)
+        s.set_inlet(0.08)
+        s.set_smagorinsky(0.05)
+    gpu.run(20)
+    o32.steps(20)
+    assert _err(gpu.read_populations(), o32.read_populations()) < 2e-5
+
+
+def test_d3q19_fp16_bits_match_oracle_codec():
+    """After init (no arithmetic differences yet) the stored FP16 words equal the oracle's codec."""
+    cfg, sdf = case3("fp16")
+    gpu = Solver3D.create(dict(cfg))
+    o32 = OracleSolver3D(dict(cfg), 32)
+    for s in (gpu, o32):
+        s.set_sdf(sdf)
+        s.init_field()
+    raw_g = gpu.read_populations_raw()
+    raw_o = o32.read_raw().view(np.uint16)
+    mism = np.count_nonzero(raw_g != raw_o)
+    assert mism <= raw_g.size // 1000, mism
+
+
+def test_ring_and_reset_semantics_on_gpu():
+    """readForces across resetForces (src/solver.ts:315-335) with the GPU ring."""
+    cfg, sdf_fn = CASES2["tunnel"]
+    s = Solver.create(dict(cfg, forceEvery=4))
+    s.set_sdf(sdf_fn(cfg["width"], cfg["height"]))
+    s.init_field()
+    s.run(40)
+    f = s.read_forces()
+    assert [x.step for x in f] == list(range(4, 41, 4))
+    assert s.read_forces() == []
+    s.run(8)
+    s.reset_forces()
+    assert s.read_forces() == []
+    s.run(9)
+    f = s.read_forces()
+    assert [x.step for x in f] == [52, 56]
