@@ -1,0 +1,73 @@
+'use strict';
+// Node wrapper on the GPU: Solver / Solver3D through the N-API addon and the C ABI, including the
+// readForces-across-resetForces semantics the reference pins (src/solver.test.ts:38-81,
+// src/solver3d.test.ts:157-195) and its allocation-failure message (src/solver3d.ts:83-86).
+const assert = require('assert');
+const path = require('path');
+const m = require(path.join(__dirname, '..', '..', 'cells_amd', 'node'));
+
+async function main() {
+  // --- 3D tunnel: run, forces, macro
+  const [W, H, D] = [32, 16, 14];
+  const s3 = await m.Solver3D.create(0, { width: W, height: H, depth: D, precision: 'fp32', tau: 0.524, uIn: 0.05,
+    spongeFraction: 0.15, spongeTau: 1, absorb: 0.02, forceEvery: 1 });
+  const sdf = m.emptySdf3(W, H, D);
+  m.addSphere(sdf, W, H, D, 8, 7.5, 6.5, 3);
+  s3.setSdf(sdf);
+  s3.initField();
+  await s3.run(50, 20);
+  assert.strictEqual(s3.step, 50);
+  const f = await s3.readForces();
+  assert.strictEqual(f.length, 50);
+  assert.strictEqual(f[0].step, 1);
+  assert.ok(f[49].fx > 0 && Number.isFinite(f[49].fz));
+  const mac = await s3.readMacro();
+  assert.strictEqual(mac.length, W * H * D * 4);
+  assert.ok(!m.fieldDiverged(mac));
+  console.log(`ok 3d run: C_D-like fx ${f[49].fx.toExponential(4)}`);
+
+  // --- stale read across resetForces (2D and 3D)
+  for (const dim of [2, 3]) {
+    const s = dim === 2
+      ? await m.Solver.create(0, { width: 48, height: 20, left: m.XMode.Open, right: m.XMode.Open, bottom: m.YMode.Slip,
+        top: m.YMode.Slip, tau: 0.6, uIn: 0.04, forceEvery: 4 })
+      : await m.Solver3D.create(0, { width: 24, height: 12, depth: 10, precision: 'fp16', tau: 0.56, uIn: 0.04, forceEvery: 4 });
+    const g = dim === 2 ? m.emptySdf(48, 20) : m.emptySdf3(24, 12, 10);
+    if (dim === 2) m.addCircle(g, 48, 20, 12, 9.5, 3); else m.addSphere(g, 24, 12, 10, 7, 5.5, 4.5, 2);
+    s.setSdf(g);
+    s.initField();
+    await s.run(4000, 1000);
+    const stale = s.readForces();
+    s.initField(); // step = 0 and resetForces
+    const r1 = s.run(40);
+    assert.deepStrictEqual(await stale, []);
+    await r1;
+    await s.run(400);
+    const samples = await s.readForces();
+    assert.strictEqual(samples.length, 110);
+    assert.strictEqual(samples[0].step, 4);
+    assert.strictEqual(samples[109].step, 440);
+    await s.run(400);
+    assert.strictEqual((await s.readForces())[0].step, 444);
+    s.destroy();
+    s.destroy(); // idempotent
+    console.log(`ok ${dim}d force ring semantics`);
+  }
+
+  // --- allocation failure message keeps the reference's prefix
+  let err = null;
+  try {
+    await m.Solver3D.create(0, { width: 4096, height: 4096, depth: 16, precision: 'fp32', tau: 0.6 });
+  } catch (e) {
+    err = e;
+  }
+  assert.ok(err && /^Solver3D allocation failed: /.test(err.message), err && err.message);
+  console.log('ok allocation failure message');
+  s3.destroy();
+  console.log('all passed');
+}
+
+main().catch((e) => {
+  console.log(`FAIL ${e.stack || e}`);
+  process.exit(1);
+});
