@@ -91,6 +91,7 @@ struct lbm_solver {
   uint32_t* counter = nullptr;
   double* ring = nullptr;
   uint32_t* state = nullptr;
+  float4* xtab = nullptr;  // per-x relaxation coefficients of the bulk kernel
 
   int64_t step = 0;
   int parity = 0;
@@ -112,7 +113,8 @@ namespace {
 void free_all(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
-                  (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state})
+                  (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state,
+                  (void*)s->xtab})
     if (p) (void)hipFree(p);
   for (auto& kv : s->tickets) {
     if (kv.second.ring) (void)hipHostFree(kv.second.ring);
@@ -148,6 +150,7 @@ void write_params2(lbm_solver* s) {
   p.forcing = (c.body_force[0] != 0 || c.body_force[1] != 0) ? 1 : 0;
   p.absorb = float(c.absorb);
   p.inletLayer = float(c.inlet_layer_fraction * (s->W - 1));
+  if (s->xtab) launch_xtab2(p, s->xtab, s->stream);
 }
 
 // Solver3D.writeParams (src/solver3d.ts:197-208); lambda defaults to trtLambda3(tau) (src/lattice3d.ts:29-31).
@@ -166,6 +169,7 @@ void write_params3(lbm_solver* s) {
   p.spongeStart = float(c.sponge_fraction > 0 ? (s->W - 1) * (1 - c.sponge_fraction) : 1e9);
   p.tauSponge = float(std::max(c.sponge_tau, c.tau));
   p.absorb = float(c.absorb);
+  if (s->xtab) launch_xtab3(p, s->xtab, s->stream);
 }
 
 // feq(i, 1, (u, 0, 0)) in float, evaluated in the source's operation order without contraction
@@ -271,6 +275,7 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMalloc(&s->counter, 4));
   HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
   HIP_TRY(hipMalloc(&s->state, 4));
+  HIP_TRY(hipMalloc(&s->xtab, sizeof(float4) * size_t(s->W)));
   HIP_TRY(hipMemsetAsync(s->f[0], 0, fbytes, s->stream));
   HIP_TRY(hipMemsetAsync(s->f[1], 0, fbytes, s->stream));
   HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * N, s->stream));
@@ -298,13 +303,14 @@ int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
   void* dst = s->f[s->parity ^ 1];
   if (s->nd == 2) {
     if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
-    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, s->list, s->nlist, s->p2, fs,
-                 s->bulk_block, s->stream, mid);
+    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, s->list, s->nlist, s->p2,
+                 s->xtab, fs, s->bulk_block, s->stream, mid);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];
     inlet_feq3(s->p3.uIn, fin);
-    launch_step3(s->prec, src, dst, s->flags, s->sdf, s->list, s->nlist, s->p3, fin, fs, s->bulk_block, s->stream, mid);
+    launch_step3(s->prec, src, dst, s->flags, s->sdf, s->list, s->nlist, s->p3, fin, s->xtab, fs, s->bulk_block,
+                 s->stream, mid);
   }
   s->parity ^= 1;
   s->step++;
@@ -339,6 +345,7 @@ int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
   if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
   *out = nullptr;
   if (cfg->width < 1 || cfg->height < 1) return fail(LBM_ERR_INVALID, "width and height must be positive");
+  if (cfg->height > 65535) return fail(LBM_ERR_INVALID, "height must be at most 65535 (one grid row per launch row)");
   const uint64_t N = uint64_t(cfg->width) * uint64_t(cfg->height);
   if (9 * N >= (1ull << 32)) return fail(LBM_ERR_INVALID, "grid too large: 9 * N must stay below 2^32");
   const int bb = bulk_block_for(cfg->workgroup_size);
@@ -374,6 +381,8 @@ int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   if (cfg->precision < LBM_FP32 || cfg->precision > LBM_BF16) return fail(LBM_ERR_UNSUPPORTED, "unknown precision");
   if (cfg->width < 3 || cfg->height < 1 || cfg->depth < 1)
     return fail(LBM_ERR_INVALID, "width must be at least 3, height and depth positive");
+  if (cfg->height > 65535 || cfg->depth > 65535)
+    return fail(LBM_ERR_INVALID, "height and depth must be at most 65535 (one grid row per launch row)");
   const uint64_t N = uint64_t(cfg->width) * uint64_t(cfg->height) * uint64_t(cfg->depth);
   const size_t elem = cfg->precision == LBM_FP32 ? 4 : 2;
   if (19 * N >= (1ull << 32)) {

@@ -11,8 +11,11 @@ namespace lbm {
 // One time step: the bulk kernel (plain fluid and plain inlet/outlet nodes, one thread per cell)
 // followed by the boundary-list kernel (every other non-solid node, plus the force sample).
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf,
-                  const uint32_t* list, uint32_t nlist, const Params3& p, const float* feq_in,
+                  const uint32_t* list, uint32_t nlist, const Params3& p, const float* feq_in, const float4* xtab,
                   const ForceSink& fs, int bulk_block, hipStream_t st, hipEvent_t mid = nullptr);
+// Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
+void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
+uint32_t bulk_block_x(int W, int requested);
 void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);
 void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st);
 void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
@@ -21,8 +24,9 @@ uint32_t list_grid3(uint32_t nlist);
 
 // ---- D2Q9 (FP32 storage) ----
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                  uint32_t nlist, const Params2& p, const ForceSink& fs, int bulk_block, hipStream_t st,
-                  hipEvent_t mid = nullptr);
+                  uint32_t nlist, const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block,
+                  hipStream_t st, hipEvent_t mid = nullptr);
+void launch_xtab2(const Params2& p, float4* tab, hipStream_t st);
 void launch_macro2(const float* f, const uint32_t* flags, float4* mac, const Params2& p, hipStream_t st);
 void launch_init2(float* fa, float* fb, const float4* mac, const Params2& p, hipStream_t st);
 void launch_flags2(const float* sdf, uint32_t* flags, uint8_t* touches, float* fa, float* fb, const float4* mac,

@@ -39,9 +39,34 @@ __device__ __forceinline__ void zou_he_outlet(float (&f)[9]) {
   f[6] = f[8] - 0.5f * (f[2] - f[4]) - 0.16666666666666666f * u;
 }
 
+// Relaxation coefficients that depend only on x: omega_base (sponge ramp of tau), omega_minus for
+// the no-Smagorinsky case, the absorbing-layer strength sigma (outlet sponge and inlet layer) and
+// 1 / omega_base (src/shaders/step.ts:215-223, 48-63).
+struct Relax2 {
+  float omBase, omM, sigma, t0;
+};
+
+__device__ __forceinline__ Relax2 relax2(int x, const Params2& p) {
+  Relax2 r;
+  float omegaBase = 1.0f / p.tau;
+  float sOut = 0.0f;
+  const float wm1 = float(p.W - 1);
+  if (p.spongeStart < wm1) {
+    sOut = smoothstep(p.spongeStart, wm1, float(x));
+    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
+  }
+  float sIn = 0.0f;
+  if (p.inletLayer > 0.0f) sIn = 1.0f - smoothstep(0.0f, p.inletLayer, float(x));
+  r.omBase = omegaBase;
+  r.sigma = p.absorb * fmaxf(sOut * sOut, sIn * sIn);
+  r.t0 = 1.0f / omegaBase;
+  r.omM = 1.0f / (0.5f + p.lambda / (r.t0 - 0.5f));
+  return r;
+}
+
 // Moments, TRT collision with sponge, inlet layer, Smagorinsky, absorbing layers and Guo forcing:
 // src/shaders/step.ts:22-103, 211-226.
-__device__ __forceinline__ void collide2(const float (&f)[9], int x, int y, const Params2& p, float (&out)[9]) {
+__device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, int y, const Params2& p, float (&out)[9]) {
   const float drho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
   const float rho = 1.0f + drho;
   const float invRho = 1.0f / rho;
@@ -65,16 +90,8 @@ __device__ __forceinline__ void collide2(const float (&f)[9], int x, int y, cons
     eqS[a] = W2[a] * (drho + rho * (0.5f * cu[a] * cu[a] - usq));
     eqA[a] = W2[a] * rho * cu[a];
   }
-  float omegaBase = 1.0f / p.tau;
-  float sOut = 0.0f;
-  const float wm1 = float(p.W - 1);
-  if (p.spongeStart < wm1) {
-    sOut = smoothstep(p.spongeStart, wm1, float(x));
-    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
-  }
-  float sIn = 0.0f;
-  if (p.inletLayer > 0.0f) sIn = 1.0f - smoothstep(0.0f, p.inletLayer, float(x));
-  const float sigma = p.absorb * fmaxf(sOut * sOut, sIn * sIn);
+  const float omegaBase = r.omBase;
+  const float sigma = r.sigma;
   float omP = omegaBase;
   if (p.smagC2 > 0.0f) {
     float n[9];
@@ -88,11 +105,11 @@ __device__ __forceinline__ void collide2(const float (&f)[9], int x, int y, cons
     const float pyy = n[2] + n[4] + n[5] + n[6] + n[7] + n[8];
     const float pxy = n[5] - n[6] + n[7] - n[8];
     const float pi = sqrtf(pxx * pxx + pyy * pyy + 2.0f * pxy * pxy);
-    const float t0 = 1.0f / omegaBase;
+    const float t0 = r.t0;
     const float tEff = 0.5f * (t0 + sqrtf(t0 * t0 + 25.455844122715714f * p.smagC2 * pi * invRho));
     omP = 1.0f / tEff;
   }
-  const float omM = 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f));
+  const float omM = p.smagC2 > 0.0f ? 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f)) : r.omM;
   float s0 = 0.0f;
   float sS[9], sA[9];
 #pragma unroll
@@ -132,33 +149,53 @@ __device__ __forceinline__ void collide2(const float (&f)[9], int x, int y, cons
 }
 
 // ============================================================================ bulk kernel
+// Grid (ceil(W / blockDim.x), H): x and y come from the launch coordinates; per direction the source
+// address is a wave-uniform base (src + i*N - delta_i) plus the lane's byte offset.
 __global__ __launch_bounds__(1024) void step2_bulk(const float* __restrict__ src, float* __restrict__ dst,
-                                                   const uint32_t* __restrict__ flags, Params2 p) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= p.N) return;
+                                                   const uint32_t* __restrict__ flags,
+                                                   const float4* __restrict__ xtab, Params2 p) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= uint32_t(p.W)) return;
+  const int y = int(blockIdx.y);
+  const uint32_t idx = x + uint32_t(p.W) * blockIdx.y;
   const uint32_t fl = flags[idx];
   const bool inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
   const bool outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
   if (fl != 0u && !inlet && !outlet) return;
-  const uint32_t N = p.N;
-  const int y = int(idx / uint32_t(p.W));
-  const int x = int(idx - uint32_t(y) * uint32_t(p.W));
+  const size_t N = p.N;
+  const uint32_t off = idx * 4u;
+  const char* const sb = reinterpret_cast<const char*>(src);
   float f[9];
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const int delta = CY2[i] * p.W + CX2[i];
-    uint32_t g = uint32_t(i) * N + idx - uint32_t(delta);
+    const ptrdiff_t delta = ptrdiff_t(CY2[i]) * p.W + CX2[i];
+    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * 4;
+    uint32_t o = off;
     // The populations entering from off the grid are placeholders (step.ts:173-174) that Zou-He
-    // overwrites; read the node's own opposite population so the address stays on the grid.
-    if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) g = uint32_t(OPP2[i]) * N + idx;
-    f[i] = src[g];
+    // overwrites; read the node's own population instead so the address stays on the grid.
+    if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) o = off + uint32_t(delta) * 4u;
+    f[i] = *reinterpret_cast<const float*>(base + o);
   }
   if (inlet) zou_he_inlet(f, y, p);
   if (outlet) zou_he_outlet(f);
+  const float4 t = xtab[x];
+  Relax2 r;
+  r.omBase = t.x;
+  r.omM = t.y;
+  r.sigma = t.z;
+  r.t0 = t.w;
   float out[9];
-  collide2(f, x, y, p, out);
+  collide2(f, r, y, p, out);
+  char* const d = reinterpret_cast<char*>(dst);
 #pragma unroll
-  for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
+  for (int i = 0; i < 9; i++) *reinterpret_cast<float*>(d + i * N * 4 + off) = out[i];
+}
+
+__global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restrict__ tab) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= p.W) return;
+  const Relax2 r = relax2(x, p);
+  tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
 }
 
 // ============================================================================ boundary-list kernel
@@ -249,7 +286,7 @@ __global__ __launch_bounds__(256) void step2_list(const float* __restrict__ src,
     if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
     else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
     float out[9];
-    collide2(f, x, y, p, out);
+    collide2(f, relax2(x, p), y, p, out);
 #pragma unroll
     for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
   }
@@ -373,10 +410,16 @@ static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) /
 uint32_t list_grid2(uint32_t nlist) { return nlist ? blocks_for(nlist, 256) : 1u; }
 
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                  uint32_t nlist, const Params2& p, const ForceSink& fs, int bb, hipStream_t st, hipEvent_t mid) {
-  step2_bulk<<<blocks_for(p.N, bb), bb, 0, st>>>(src, dst, flags, p);
+                  uint32_t nlist, const Params2& p, const float4* xtab, const ForceSink& fs, int bb, hipStream_t st,
+                  hipEvent_t mid) {
+  const uint32_t bx = bulk_block_x(p.W, bb);
+  const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H));
+  step2_bulk<<<grid, bx, 0, st>>>(src, dst, flags, xtab, p);
   if (mid) (void)hipEventRecord(mid, st);
   if (nlist || fs.sample) step2_list<<<list_grid2(nlist), 256, 0, st>>>(src, dst, flags, sdf, list, nlist, p, fs);
+}
+void launch_xtab2(const Params2& p, float4* tab, hipStream_t st) {
+  xtab2_kernel<<<blocks_for(uint32_t(p.W), 256), 256, 0, st>>>(p, tab);
 }
 void launch_macro2(const float* f, const uint32_t* flags, float4* mac, const Params2& p, hipStream_t st) {
   macro2_kernel<<<blocks_for(p.N, 256), 256, 0, st>>>(f, flags, mac, p);

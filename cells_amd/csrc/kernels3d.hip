@@ -54,9 +54,32 @@ __device__ __forceinline__ Mom3 moments3(const float (&f)[19]) {
   return m;
 }
 
+// Relaxation coefficients that depend only on x (the sponge ramps tau toward tauSponge over the
+// last fraction of the tunnel, src/shaders/step3d.ts:213-219): omega_base, omega_minus for the
+// no-Smagorinsky case, the absorbing-layer strength sigma, and 1 / omega_base.
+struct Relax {
+  float omBase, omM, sigma, t0;
+};
+
+__device__ __forceinline__ Relax relax3(float x, const Params3& p) {
+  Relax r;
+  float omegaBase = 1.0f / p.tau;
+  float sOut = 0.0f;
+  const float wm1 = float(p.W - 1);
+  if (p.spongeStart < wm1) {
+    sOut = smoothstep(p.spongeStart, wm1, x);
+    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
+  }
+  r.omBase = omegaBase;
+  r.sigma = p.absorb * sOut * sOut;
+  r.t0 = 1.0f / omegaBase;
+  r.omM = 1.0f / (0.5f + p.lambda / (r.t0 - 0.5f));
+  return r;
+}
+
 // TRT collision with sponge, Smagorinsky and the outlet absorbing layer: src/shaders/step3d.ts:47-102,
 // 210-222. Writes the post-collision populations to out.
-__device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, float x, const Params3& p,
+__device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, const Relax& r, const Params3& p,
                                          float (&out)[19]) {
   float cu[19], eqS[19], eqA[19];
 #pragma unroll
@@ -72,14 +95,8 @@ __device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, fl
     eqS[a] = W3[a] * (m.drho + m.rho * (0.5f * cu[a] * cu[a] - m.usq));
     eqA[a] = W3[a] * m.rho * cu[a];
   }
-  float omegaBase = 1.0f / p.tau;
-  float sOut = 0.0f;
-  const float wm1 = float(p.W - 1);
-  if (p.spongeStart < wm1) {
-    sOut = smoothstep(p.spongeStart, wm1, x);
-    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
-  }
-  const float sigma = p.absorb * sOut * sOut;
+  const float omegaBase = r.omBase;
+  const float sigma = r.sigma;
   float omP = omegaBase;
   if (p.smagC2 > 0.0f) {
     float n[19];
@@ -103,11 +120,11 @@ __device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, fl
       if (CY3[i] * CZ3[i] < 0) pyz = pyz - n[i];
     }
     const float pi = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-    const float t0 = 1.0f / omegaBase;
+    const float t0 = r.t0;
     const float tEff = 0.5f * (t0 + sqrtf(t0 * t0 + 25.455844122715714f * p.smagC2 * pi * m.invRho));
     omP = 1.0f / tEff;
   }
-  const float omM = 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f));
+  const float omM = p.smagC2 > 0.0f ? 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f)) : r.omM;
   float s0 = 0.0f;
   float sS[19], sA[19];
 #pragma unroll
@@ -141,47 +158,67 @@ struct FeqIn {
 };
 
 // ============================================================================ bulk kernel
+// Grid (ceil(W / blockDim.x), H, D): one block row of threads per x row, so x, y, z come from the
+// launch coordinates. Per direction the source address is a wave-uniform base (src + i*N - delta_i,
+// kept in SGPRs) plus the lane's byte offset, shared by all 19 loads; stores likewise.
 template <class S>
 __global__ __launch_bounds__(1024) void step3_bulk(const typename S::elem* __restrict__ src,
                                                    typename S::elem* __restrict__ dst,
-                                                   const uint32_t* __restrict__ flags, Params3 p, FeqIn fin) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= p.N) return;
+                                                   const uint32_t* __restrict__ flags,
+                                                   const float4* __restrict__ xtab, Params3 p, FeqIn fin) {
+  using E = typename S::elem;
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= uint32_t(p.W)) return;
+  const uint32_t idx = x + uint32_t(p.W) * (blockIdx.y + uint32_t(p.H) * blockIdx.z);
   const uint32_t fl = flags[idx];
   const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3);
   const bool outlet = fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
   if (fl != 0u && !inlet && !outlet) return;  // handled by the boundary list (or solid)
-  const uint32_t N = p.N;
+  const size_t N = p.N;
+  const uint32_t off = idx * uint32_t(sizeof(E));
+  char* const d = reinterpret_cast<char*>(dst);
   if (inlet) {
     // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
+    for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, fin.v[i]);
     return;
   }
-  const uint32_t z = idx / p.WH;
-  const uint32_t rem = idx - z * p.WH;
-  const uint32_t y = rem / uint32_t(p.W);
-  const uint32_t x = rem - y * uint32_t(p.W);
+  const char* const sb = reinterpret_cast<const char*>(src);
   float f[19];
 #pragma unroll
   for (int i = 0; i < 19; i++) {
-    const int delta = CZ3[i] * int(p.WH) + CY3[i] * p.W + CX3[i];
-    uint32_t k = idx - uint32_t(delta);
-    // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z).
-    if (CX3[i] < 0 && outlet) k = idx - 1u;
-    f[i] = S::load(src + size_t(i) * N, k);
+    const ptrdiff_t delta = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W + CX3[i];
+    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * ptrdiff_t(sizeof(E));
+    uint32_t o = off;
+    // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z),
+    // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements.
+    if (CX3[i] < 0 && outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
+    f[i] = S::load_at(base, o);
   }
   const Mom3 m = moments3(f);
   if (outlet) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+    for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz));
     return;
   }
+  const float4 t = xtab[x];
+  Relax r;
+  r.omBase = t.x;
+  r.omM = t.y;
+  r.sigma = t.z;
+  r.t0 = t.w;
   float out[19];
-  collide3(f, m, float(x), p, out);
+  collide3(f, m, r, p, out);
 #pragma unroll
-  for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+  for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, out[i]);
+}
+
+__global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restrict__ tab) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= p.W) return;
+  const Relax r = relax3(float(x), p);
+  tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
 }
 
 // ============================================================================ boundary-list kernel
@@ -297,7 +334,7 @@ __global__ __launch_bounds__(256) void step3_list(const typename S::elem* __rest
         for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
       } else {
         float out[19];
-        collide3(f, m, float(x), p, out);
+        collide3(f, m, relax3(float(x), p), p, out);
 #pragma unroll
         for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
       }
@@ -422,14 +459,26 @@ __global__ void reset_state_kernel(uint32_t* state) { state[0] = 0u; }
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 uint32_t list_grid3(uint32_t nlist) { return nlist ? blocks_for(nlist, 256) : 1u; }
 
+// Threads per block along x: the widest multiple of 64 not above the requested size that divides W
+// (no idle lanes), else one block of ceil(W / 64) waves per row.
+uint32_t bulk_block_x(int W, int requested) {
+  const int req = requested > 0 ? requested : 256;
+  for (int b = (req / 64) * 64; b >= 64; b -= 64)
+    if (W % b == 0) return uint32_t(b);
+  const int whole = ((W + 63) / 64) * 64;
+  return uint32_t(whole <= 1024 ? whole : 256);
+}
+
 template <class S>
 static void step3_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                    uint32_t nlist, const Params3& p, const float* feq_in, const ForceSink& fs, int bb,
-                    hipStream_t st, hipEvent_t mid) {
+                    uint32_t nlist, const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs,
+                    int bb, hipStream_t st, hipEvent_t mid) {
   using E = typename S::elem;
   FeqIn fin;
   for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
-  step3_bulk<S><<<blocks_for(p.N, bb), bb, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, p, fin);
+  const uint32_t bx = bulk_block_x(p.W, bb);
+  const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H), uint32_t(p.D));
+  step3_bulk<S><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin);
   if (mid) (void)hipEventRecord(mid, st);
   if (nlist || fs.sample)
     step3_list<S><<<list_grid3(nlist), 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, sdf, list,
@@ -437,13 +486,17 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const flo
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                  uint32_t nlist, const Params3& p, const float* feq_in, const ForceSink& fs, int bulk_block,
-                  hipStream_t st, hipEvent_t mid) {
+                  uint32_t nlist, const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs,
+                  int bulk_block, hipStream_t st, hipEvent_t mid) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
-    default: step3_t<StoreF32>(src, dst, flags, sdf, list, nlist, p, feq_in, fs, bulk_block, st, mid); break;
+    case 1: step3_t<StoreF16>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    default: step3_t<StoreF32>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
   }
+}
+
+void launch_xtab3(const Params3& p, float4* tab, hipStream_t st) {
+  xtab3_kernel<<<blocks_for(uint32_t(p.W), 256), 256, 0, st>>>(p, tab);
 }
 
 template <class S>

@@ -56,31 +56,45 @@ __device__ __forceinline__ float smoothstep(float lo, float hi, float x) {
 __device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 
 // ---------------------------------------------------------------- storage codecs
-struct StoreF32 {
+// decode / encode one stored element; load(p, k) and store(p, k, v) address element k of p, while
+// load_at(base, off) and store_at(base, off, v) take a 32-bit byte offset from a wave-uniform base,
+// which lets the compiler use the SGPR-base + VGPR-offset addressing of global loads and stores.
+template <class Codec>
+struct StoreOps {
+  using elem = typename Codec::elem;
+  __device__ static __forceinline__ float load(const elem* p, uint32_t k) { return Codec::dec(p[k]); }
+  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) { p[k] = Codec::enc(v); }
+  __device__ static __forceinline__ float load_at(const char* base, uint32_t off) {
+    return Codec::dec(*reinterpret_cast<const elem*>(base + off));
+  }
+  __device__ static __forceinline__ void store_at(char* base, uint32_t off, float v) {
+    *reinterpret_cast<elem*>(base + off) = Codec::enc(v);
+  }
+};
+struct CodecF32 {
   using elem = float;
-  __device__ static __forceinline__ float load(const elem* p, uint32_t k) { return p[k]; }
-  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) { p[k] = v; }
+  __device__ static __forceinline__ float dec(elem e) { return e; }
+  __device__ static __forceinline__ elem enc(float v) { return v; }
 };
-struct StoreF16 {
+struct CodecF16 {
   using elem = _Float16;
-  __device__ static __forceinline__ float load(const elem* p, uint32_t k) {
-    return float(p[k]) * (1.0f / 32768.0f);
-  }
-  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) {
-    p[k] = _Float16(wclamp(v, -1.99f, 1.99f) * 32768.0f);  // v_cvt_f16_f32: round to nearest even
-  }
+  __device__ static __forceinline__ float dec(elem e) { return float(e) * (1.0f / 32768.0f); }
+  // v_cvt_f16_f32 rounds to nearest even
+  __device__ static __forceinline__ elem enc(float v) { return _Float16(wclamp(v, -1.99f, 1.99f) * 32768.0f); }
 };
-struct StoreBF16 {
+struct CodecBF16 {
   using elem = uint16_t;
-  __device__ static __forceinline__ float load(const elem* p, uint32_t k) {
-    return __uint_as_float(uint32_t(p[k]) << 16);
-  }
-  __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) {
-    uint32_t x = __float_as_uint(v);
-    uint32_t r = ((x & 0x7fffffffu) > 0x7f800000u) ? ((x >> 16) | 0x40u) : ((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
-    p[k] = uint16_t(r);
+  __device__ static __forceinline__ float dec(elem e) { return __uint_as_float(uint32_t(e) << 16); }
+  __device__ static __forceinline__ elem enc(float v) {
+    const uint32_t x = __float_as_uint(v);
+    const uint32_t r =
+        ((x & 0x7fffffffu) > 0x7f800000u) ? ((x >> 16) | 0x40u) : ((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+    return uint16_t(r);
   }
 };
+struct StoreF32 : StoreOps<CodecF32> {};
+struct StoreF16 : StoreOps<CodecF16> {};
+struct StoreBF16 : StoreOps<CodecBF16> {};
 
 // ---------------------------------------------------------------- parameter blocks
 // Values of the reference's Params uniform (src/shaders/common.ts:55-63) after writeParams
