@@ -20,10 +20,11 @@
 
 #include "../../include/lbm.h"
 #include "kernels.hpp"
+#include "solver_internal.hpp"
 
 using namespace lbm;
 
-namespace {
+namespace lbm {
 
 thread_local std::string g_err;
 
@@ -32,83 +33,11 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                                     \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess) {                                                               \
-      return fail(LBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-    }                                                                                     \
-  } while (0)
-
-constexpr double MAGIC_LAMBDA = 3.0 / 16.0;
-
-// Inlet ramp evaluated per step on the host (an extension; the reference's drivers ramp per chunk,
-// src/cases.ts:52-54): kind 0 off, 1 linear, 2 smoothstep t^2 (3 - 2t).
-struct Ramp {
-  int kind = 0;
-  double u_target = 0, steps = 0;
-  double at(int64_t k) const {
-    double r = steps > 0 ? std::min(1.0, double(k) / steps) : 1.0;
-    if (kind == 2) r = r * r * (3 - 2 * r);
-    return u_target * r;
-  }
-};
-
-struct Ticket {
-  double* ring = nullptr;    // pinned, HISTORY_LEN * dim
-  uint32_t* state = nullptr; // pinned
-  hipEvent_t ev = nullptr;
-  int64_t generation = 0;
-};
-
-}  // namespace
-
-struct lbm_solver {
-  int nd = 2;  // 2 or 3
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int W = 1, H = 1, D = 1;
-  uint32_t N = 0;
-  int Q = 9;
-  int prec = LBM_FP32;
-  size_t elem = 4;
-  lbm_config2d cfg2{};
-  lbm_config3d cfg3{};
-  Params2 p2{};
-  Params3 p3{};
-  Ramp ramp;
-  int bulk_block = 256;
-
-  void* f[2] = {nullptr, nullptr};
-  uint32_t* flags = nullptr;
-  uint8_t* touches = nullptr;
-  float* sdf = nullptr;
-  float4* mac = nullptr;
-  uint32_t* list = nullptr;
-  uint32_t nlist = 0;
-  int64_t n_bulk = 0, n_solid = 0;
-  double* partials = nullptr;
-  uint32_t* counter = nullptr;
-  double* ring = nullptr;
-  uint32_t* state = nullptr;
-  float4* xtab = nullptr;  // per-x relaxation coefficients of the bulk kernel
-
-  int64_t step = 0;
-  int parity = 0;
-
-  // readForces bookkeeping (src/solver.ts:57-61, 236-241, 315-335)
-  int64_t samples_read = 0;
-  std::deque<int64_t> force_steps;
-  int64_t generation = 0;
-  std::mutex mu;
-  std::map<int64_t, Ticket> tickets;
-  int64_t next_ticket = 1;
-
-  int dim() const { return nd == 2 ? 2 : 3; }
-  int force_every() const { return nd == 2 ? cfg2.force_every : cfg3.force_every; }
-};
+}  // namespace lbm
 
 namespace {
+
+constexpr double MAGIC_LAMBDA = 3.0 / 16.0;
 
 void free_all(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -122,7 +51,7 @@ void free_all(lbm_solver* s) {
     if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
   }
   s->tickets.clear();
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->stream && s->owns_stream) (void)hipStreamDestroy(s->stream);
   s->stream = nullptr;
 }
 
@@ -189,7 +118,7 @@ void inlet_feq3(float u, float* out) {
 
 // rebuildFlags (src/solver.ts:205-217, src/solver3d.ts:216-228): macro on the current buffer, flags
 // and refill on the GPU, then the boundary list (and 2D force slots) from the flags, in cell order.
-int rebuild_flags(lbm_solver* s) {
+int rebuild_flags_impl(lbm_solver* s) {
   const uint32_t N = s->N;
   if (s->nd == 2) {
     launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
@@ -261,7 +190,10 @@ int rebuild_flags(lbm_solver* s) {
 int alloc_common(lbm_solver* s) {
   const size_t N = s->N;
   HIP_TRY(hipSetDevice(s->device));
-  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (!s->stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->owns_stream = true;
+  }
   const size_t fbytes = size_t(s->Q) * N * s->elem;
   HIP_TRY(hipMalloc(&s->f[0], fbytes));
   HIP_TRY(hipMalloc(&s->f[1], fbytes));
@@ -320,6 +252,23 @@ int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
 
 }  // namespace
 
+namespace lbm {
+
+int rebuild_flags(lbm_solver* s) { return rebuild_flags_impl(s); }
+
+// Advances the host bookkeeping of a tunnel by n steps that a batch launch performed.
+void record_steps(lbm_solver* s, int64_t n) {
+  const int every = s->force_every();
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (int64_t k = 0; k < n; k++) {
+    s->step++;
+    if (every > 0 && s->step % every == 0) s->force_steps.push_back(s->step);
+  }
+  s->parity ^= int(n & 1);
+}
+
+}  // namespace lbm
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -342,6 +291,12 @@ void lbm_config3d_defaults(lbm_config3d* c) {
 }
 
 int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
+  return lbm::create_2d_on(cfg, device, nullptr, out);
+}
+
+}  // extern "C"
+
+int lbm::create_2d_on(const lbm_config2d* cfg, int device, hipStream_t stream, lbm_solver** out) {
   if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
   *out = nullptr;
   if (cfg->width < 1 || cfg->height < 1) return fail(LBM_ERR_INVALID, "width and height must be positive");
@@ -353,6 +308,10 @@ int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
   lbm_solver* s = new lbm_solver();
   s->nd = 2;
   s->device = device;
+  if (stream) {
+    s->stream = stream;
+    s->owns_stream = false;
+  }
   s->W = cfg->width;
   s->H = cfg->height;
   s->N = uint32_t(N);
@@ -363,7 +322,7 @@ int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
   int rc = alloc_common(s);
   if (rc == LBM_OK) {
     write_params2(s);
-    rc = rebuild_flags(s);
+    rc = rebuild_flags_impl(s);
   }
   if (rc != LBM_OK) {
     std::string msg = g_err;
@@ -374,6 +333,8 @@ int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
   *out = s;
   return LBM_OK;
 }
+
+extern "C" {
 
 int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
@@ -408,7 +369,7 @@ int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   int rc = alloc_common(s);
   if (rc == LBM_OK) {
     write_params3(s);
-    rc = rebuild_flags(s);
+    rc = rebuild_flags_impl(s);
   }
   if (rc != LBM_OK) {
     std::string msg = g_err;
@@ -464,7 +425,7 @@ int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps) {
 int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n) {
   if (!s || !sdf || n != s->N) return fail(LBM_ERR_INVALID, "sdf must hold N floats");
   HIP_TRY(hipMemcpyAsync(s->sdf, sdf, 4 * size_t(s->N), hipMemcpyHostToDevice, s->stream));
-  return rebuild_flags(s);
+  return rebuild_flags_impl(s);
 }
 
 int lbm_init_field(lbm_solver* s, const float* macro, size_t n) {

@@ -5,148 +5,11 @@
 // nodes and plain Zou-He inlet / outlet nodes with coalesced loads and stores; every other
 // non-solid node (walls, periodic seams, body links) runs the reference's pullSlow rules from a
 // list, with the momentum-exchange force reduced in FP64 in the same launch.
+#include "d2q9.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
 
 namespace lbm {
-
-// src/shaders/aux.ts:15-18
-__device__ __forceinline__ float feq2(int i, float rho, float ux, float uy) {
-  const float cu = 3.0f * (float(CX2[i]) * ux + float(CY2[i]) * uy);
-  return W2[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (ux * ux + uy * uy)));
-}
-
-// src/shaders/common.ts:84-90
-__device__ __forceinline__ float inlet_velocity(int y, const Params2& p) {
-  if (p.parabolic == 0) return p.uIn;
-  const float h = float(p.H);
-  const float s = float(y) + 0.5f;
-  return 4.0f * p.uIn * s * (h - s) / (h * h);
-}
-
-// src/shaders/step.ts:197-209: Zou-He reconstruction of the populations entering from outside.
-__device__ __forceinline__ void zou_he_inlet(float (&f)[9], int y, const Params2& p) {
-  const float u = inlet_velocity(y, p);
-  const float r = (1.0f + f[0] + f[2] + f[4] + 2.0f * (f[3] + f[6] + f[7])) / (1.0f - u);
-  f[1] = f[3] + 0.6666666666666666f * r * u;
-  f[5] = f[7] - 0.5f * (f[2] - f[4]) + 0.16666666666666666f * r * u;
-  f[8] = f[6] + 0.5f * (f[2] - f[4]) + 0.16666666666666666f * r * u;
-}
-__device__ __forceinline__ void zou_he_outlet(float (&f)[9]) {
-  const float u = f[0] + f[2] + f[4] + 2.0f * (f[1] + f[5] + f[8]);
-  f[3] = f[1] - 0.6666666666666666f * u;
-  f[7] = f[5] + 0.5f * (f[2] - f[4]) - 0.16666666666666666f * u;
-  f[6] = f[8] - 0.5f * (f[2] - f[4]) - 0.16666666666666666f * u;
-}
-
-// Relaxation coefficients that depend only on x: omega_base (sponge ramp of tau), omega_minus for
-// the no-Smagorinsky case, the absorbing-layer strength sigma (outlet sponge and inlet layer) and
-// 1 / omega_base (src/shaders/step.ts:215-223, 48-63).
-struct Relax2 {
-  float omBase, omM, sigma, t0;
-};
-
-__device__ __forceinline__ Relax2 relax2(int x, const Params2& p) {
-  Relax2 r;
-  float omegaBase = 1.0f / p.tau;
-  float sOut = 0.0f;
-  const float wm1 = float(p.W - 1);
-  if (p.spongeStart < wm1) {
-    sOut = smoothstep(p.spongeStart, wm1, float(x));
-    omegaBase = 1.0f / wmix(p.tau, p.tauSponge, sOut);
-  }
-  float sIn = 0.0f;
-  if (p.inletLayer > 0.0f) sIn = 1.0f - smoothstep(0.0f, p.inletLayer, float(x));
-  r.omBase = omegaBase;
-  r.sigma = p.absorb * fmaxf(sOut * sOut, sIn * sIn);
-  r.t0 = 1.0f / omegaBase;
-  r.omM = 1.0f / (0.5f + p.lambda / (r.t0 - 0.5f));
-  return r;
-}
-
-// Moments, TRT collision with sponge, inlet layer, Smagorinsky, absorbing layers and Guo forcing:
-// src/shaders/step.ts:22-103, 211-226.
-__device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, int y, const Params2& p, float (&out)[9]) {
-  const float drho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
-  const float rho = 1.0f + drho;
-  const float invRho = 1.0f / rho;
-  float ux = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) * invRho;
-  float uy = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]) * invRho;
-  if (p.forcing != 0) {
-    ux += 0.5f * p.fx * invRho;
-    uy += 0.5f * p.fy * invRho;
-  }
-  const float usq = 1.5f * (ux * ux + uy * uy);
-  float cu[9], eqS[9], eqA[9];
-  cu[1] = 3.0f * (ux);
-  cu[2] = 3.0f * (uy);
-  cu[5] = 3.0f * (ux + uy);
-  cu[6] = 3.0f * (-ux + uy);
-  constexpr int PA[4] = {1, 2, 5, 6};
-  constexpr int PB[4] = {3, 4, 7, 8};
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int a = PA[k];
-    eqS[a] = W2[a] * (drho + rho * (0.5f * cu[a] * cu[a] - usq));
-    eqA[a] = W2[a] * rho * cu[a];
-  }
-  const float omegaBase = r.omBase;
-  const float sigma = r.sigma;
-  float omP = omegaBase;
-  if (p.smagC2 > 0.0f) {
-    float n[9];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int a = PA[k], b = PB[k];
-      n[a] = f[a] - (eqS[a] + eqA[a]);
-      n[b] = f[b] - (eqS[a] - eqA[a]);
-    }
-    const float pxx = n[1] + n[3] + n[5] + n[6] + n[7] + n[8];
-    const float pyy = n[2] + n[4] + n[5] + n[6] + n[7] + n[8];
-    const float pxy = n[5] - n[6] + n[7] - n[8];
-    const float pi = sqrtf(pxx * pxx + pyy * pyy + 2.0f * pxy * pxy);
-    const float t0 = r.t0;
-    const float tEff = 0.5f * (t0 + sqrtf(t0 * t0 + 25.455844122715714f * p.smagC2 * pi * invRho));
-    omP = 1.0f / tEff;
-  }
-  const float omM = p.smagC2 > 0.0f ? 1.0f / (0.5f + p.lambda / (1.0f / omP - 0.5f)) : r.omM;
-  float s0 = 0.0f;
-  float sS[9], sA[9];
-#pragma unroll
-  for (int k = 0; k < 4; k++) sS[PA[k]] = sA[PA[k]] = 0.0f;
-  if (sigma > 0.0f) {
-    const float ut = inlet_velocity(y, p);
-    const float dq = 1.5f * rho * (ut * ut - ux * ux - uy * uy);
-    s0 += sigma * W2[0] * -dq;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int a = PA[k];
-      const float cut = 3.0f * (float(CX2[a]) * ut);
-      sS[a] += sigma * W2[a] * (0.5f * rho * (cut * cut - cu[a] * cu[a]) - dq);
-      sA[a] += sigma * W2[a] * rho * (cut - cu[a]);
-    }
-  }
-  if (p.forcing != 0) {
-    const float uF = ux * p.fx + uy * p.fy;
-    s0 += (1.0f - 0.5f * omP) * W2[0] * (-3.0f * uF);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int a = PA[k];
-      const float cF = (float(CX2[a]) * p.fx + float(CY2[a]) * p.fy);
-      sS[a] += (1.0f - 0.5f * omP) * W2[a] * (-3.0f * uF + 9.0f * (cu[a] / 3.0f) * cF);
-      sA[a] += (1.0f - 0.5f * omM) * W2[a] * 3.0f * cF;
-    }
-  }
-  out[0] = f[0] - omP * (f[0] - W2[0] * (drho - rho * usq)) + s0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    const int a = PA[k], b = PB[k];
-    const float sym = omP * (0.5f * (f[a] + f[b]) - eqS[a]);
-    const float asym = omM * (0.5f * (f[a] - f[b]) - eqA[a]);
-    out[a] = f[a] - sym - asym + sS[a] + sA[a];
-    out[b] = f[b] - sym + asym + sS[a] - sA[a];
-  }
-}
 
 // ============================================================================ bulk kernel
 // Grid (ceil(W / blockDim.x), H): x and y come from the launch coordinates; per direction the source
@@ -156,39 +19,7 @@ __global__ __launch_bounds__(1024) void step2_bulk(const float* __restrict__ src
                                                    const float4* __restrict__ xtab, Params2 p) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  const int y = int(blockIdx.y);
-  const uint32_t idx = x + uint32_t(p.W) * blockIdx.y;
-  const uint32_t fl = flags[idx];
-  const bool inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
-  const bool outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
-  if (fl != 0u && !inlet && !outlet) return;
-  const size_t N = p.N;
-  const uint32_t off = idx * 4u;
-  const char* const sb = reinterpret_cast<const char*>(src);
-  float f[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) {
-    const ptrdiff_t delta = ptrdiff_t(CY2[i]) * p.W + CX2[i];
-    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * 4;
-    uint32_t o = off;
-    // The populations entering from off the grid are placeholders (step.ts:173-174) that Zou-He
-    // overwrites; read the node's own population instead so the address stays on the grid.
-    if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) o = off + uint32_t(delta) * 4u;
-    f[i] = *reinterpret_cast<const float*>(base + o);
-  }
-  if (inlet) zou_he_inlet(f, y, p);
-  if (outlet) zou_he_outlet(f);
-  const float4 t = xtab[x];
-  Relax2 r;
-  r.omBase = t.x;
-  r.omM = t.y;
-  r.sigma = t.z;
-  r.t0 = t.w;
-  float out[9];
-  collide2(f, r, y, p, out);
-  char* const d = reinterpret_cast<char*>(dst);
-#pragma unroll
-  for (int i = 0; i < 9; i++) *reinterpret_cast<float*>(d + i * N * 4 + off) = out[i];
+  bulk_cell2(src, dst, flags, xtab, p, x, int(blockIdx.y), x + uint32_t(p.W) * blockIdx.y);
 }
 
 __global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restrict__ tab) {
@@ -198,98 +29,13 @@ __global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restric
   tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
 }
 
-// ============================================================================ boundary-list kernel
-// src/shaders/step.ts:119-175
-template <int i>
-__device__ __forceinline__ float pull_slow2(const float* __restrict__ src, const uint32_t* __restrict__ flags,
-                                            const float* __restrict__ sdf, int x, int y, uint32_t idx, uint32_t fl,
-                                            const Params2& p, float (&force)[2]) {
-  constexpr int j = OPP2[i];
-  const uint32_t N = p.N;
-  const int W = p.W, H = p.H;
-  int sx = x - CX2[i];
-  int sy = y - CY2[i];
-  if ((fl & (1u << i)) == 0u) return src[size_t(i) * N + uint32_t(sy * W + sx)];
-  auto wrap = [&](int& qx, int& qy) {
-    if (qx < 0 && p.left == 0) qx += W;
-    if (qx >= W && p.right == 0) qx -= W;
-    if (qy < 0 && p.bottom == 0) qy += H;
-    if (qy >= H && p.top == 0) qy -= H;
-  };
-  wrap(sx, sy);
-  const bool inX = sx >= 0 && sx < W;
-  const bool inY = sy >= 0 && sy < H;
-  if (inX && inY) {
-    const uint32_t s = uint32_t(sy * W + sx);
-    if ((flags[s] & FLAG_SOLID2) == 0u) return src[size_t(i) * N + s];
-    const float fj = src[size_t(j) * N + idx];
-    const float dF = sdf[idx];
-    const float dS = sdf[s];
-    const float q = wclamp(dF / fmaxf(dF - dS, 1e-6f), 1e-3f, 1.0f);
-    float fi = fj;
-    if (q < 0.5f) {
-      int nx = x + CX2[i], ny = y + CY2[i];
-      wrap(nx, ny);
-      if (nx >= 0 && nx < W && ny >= 0 && ny < H) {
-        const uint32_t n = uint32_t(ny * W + nx);
-        if ((flags[n] & FLAG_SOLID2) == 0u) fi = 2.0f * q * fj + (1.0f - 2.0f * q) * src[size_t(j) * N + n];
-      }
-    } else {
-      fi = fj / (2.0f * q) + (2.0f * q - 1.0f) / (2.0f * q) * src[size_t(i) * N + idx];
-    }
-    force[0] += float(CX2[j]) * (fj + fi);
-    force[1] += float(CY2[j]) * (fj + fi);
-    return fi;
-  }
-  if (!inY) {
-    const int mode = sy < 0 ? p.bottom : p.top;
-    if (mode == 2) {  // slip: mirror from the same row
-      const int rx = sx < 0 ? 0 : (sx > W - 1 ? W - 1 : sx);
-      return src[size_t(MIRROR_Y2[i]) * N + uint32_t(y) * uint32_t(W) + uint32_t(rx)];
-    }
-    if (mode == 3) return src[size_t(j) * N + idx] + 6.0f * W2[i] * float(CX2[i]) * p.uLid;  // moving wall
-    return src[size_t(j) * N + idx];
-  }
-  return src[size_t(j) * N + idx];  // walls, and open-boundary placeholders
-}
-
-template <int i>
-struct PullAll2 {
-  __device__ static __forceinline__ void run(const float* src, const uint32_t* flags, const float* sdf, int x, int y,
-                                             uint32_t idx, uint32_t fl, const Params2& p, float (&f)[9],
-                                             float (&force)[2]) {
-    f[i] = pull_slow2<i>(src, flags, sdf, x, y, idx, fl, p, force);
-    PullAll2<i + 1>::run(src, flags, sdf, x, y, idx, fl, p, f, force);
-  }
-};
-template <>
-struct PullAll2<9> {
-  __device__ static __forceinline__ void run(const float*, const uint32_t*, const float*, int, int, uint32_t, uint32_t,
-                                             const Params2&, float (&)[9], float (&)[2]) {}
-};
-
 __global__ __launch_bounds__(256) void step2_list(const float* __restrict__ src, float* __restrict__ dst,
                                                   const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
                                                   const uint32_t* __restrict__ list, uint32_t nlist, Params2 p,
                                                   ForceSink fs) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   float force[2] = {0.0f, 0.0f};
-  if (k < nlist) {
-    const uint32_t idx = list[k];
-    const uint32_t fl = flags[idx];
-    const uint32_t N = p.N;
-    const int y = int(idx / uint32_t(p.W));
-    const int x = int(idx - uint32_t(y) * uint32_t(p.W));
-    float f[9];
-    f[0] = src[idx];
-    PullAll2<1>::run(src, flags, sdf, x, y, idx, fl, p, f, force);
-    if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
-    else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
-    float out[9];
-    collide2(f, relax2(x, p), y, p, out);
-#pragma unroll
-    for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
-  }
+  if (k < nlist) list_cell2(src, dst, flags, sdf, list[k], p, force);
   if (fs.sample) force_sample<2>(force, fs);
 }
 

@@ -39,9 +39,11 @@ __device__ __forceinline__ void block_sum(double (&v)[DIM], double (*lds)[16]) {
   __syncthreads();
 }
 
-// Every thread of every block of the grid must call this (no early return before it).
+// Every thread of every block of the group must call this (no early return before it). A group is
+// the whole grid (single tunnel) or the contiguous list tiles of one tunnel (batch): block is this
+// block's index inside the group, nblocks the group's size.
 template <int DIM>
-__device__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
+__device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uint32_t block, uint32_t nblocks) {
   __shared__ double lds[DIM][16];
   __shared__ int is_last;
   double v[DIM];
@@ -50,10 +52,10 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
   block_sum<DIM>(v, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) fs.partials[size_t(blockIdx.x) * DIM + d] = v[d];
+    for (int d = 0; d < DIM; d++) fs.partials[size_t(block) * DIM + d] = v[d];
     __threadfence();
     const uint32_t prev = atomicAdd(fs.counter, 1u);
-    is_last = (prev == gridDim.x - 1);
+    is_last = (prev == nblocks - 1);
   }
   __syncthreads();
   if (!is_last) return;
@@ -61,7 +63,7 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
   double acc[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) acc[d] = 0.0;
-  for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+  for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x)
 #pragma unroll
     for (int d = 0; d < DIM; d++)
       acc[d] += __hip_atomic_load(fs.partials + size_t(k) * DIM + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -74,6 +76,11 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
     *fs.state = st + 1;
     atomicExch(fs.counter, 0u);
   }
+}
+
+template <int DIM>
+__device__ __forceinline__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
+  force_sample<DIM>(force, fs, blockIdx.x, gridDim.x);
 }
 
 }  // namespace lbm

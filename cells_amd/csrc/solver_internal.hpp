@@ -1,0 +1,101 @@
+// Internal definition of a tunnel solver handle, shared by the C ABI (capi.hip) and the batched
+// multi-tunnel driver (batch2d.hip). Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "../../include/lbm.h"
+#include "kernels.hpp"
+
+namespace lbm {
+
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      return ::lbm::fail(LBM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    }                                                                                     \
+  } while (0)
+
+// Inlet ramp evaluated per step (an extension; the reference's drivers ramp per chunk,
+// src/cases.ts:52-54): kind 0 off, 1 linear, 2 smoothstep t^2 (3 - 2t). Evaluated in double, on the
+// host for single tunnels and identically on the device for batches.
+struct Ramp {
+  int kind = 0;
+  double u_target = 0, steps = 0;
+  double at(int64_t k) const {
+    double r = steps > 0 ? std::min(1.0, double(k) / steps) : 1.0;
+    if (kind == 2) r = r * r * (3 - 2 * r);
+    return u_target * r;
+  }
+};
+
+struct Ticket {
+  double* ring = nullptr;     // pinned, HISTORY_LEN * dim
+  uint32_t* state = nullptr;  // pinned
+  hipEvent_t ev = nullptr;
+  int64_t generation = 0;
+};
+
+}  // namespace lbm
+
+struct lbm_solver {
+  int nd = 2;  // 2 or 3
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool owns_stream = true;
+  int W = 1, H = 1, D = 1;
+  uint32_t N = 0;
+  int Q = 9;
+  int prec = LBM_FP32;
+  size_t elem = 4;
+  lbm_config2d cfg2{};
+  lbm_config3d cfg3{};
+  lbm::Params2 p2{};
+  lbm::Params3 p3{};
+  lbm::Ramp ramp;
+  int bulk_block = 256;
+
+  void* f[2] = {nullptr, nullptr};
+  uint32_t* flags = nullptr;
+  uint8_t* touches = nullptr;
+  float* sdf = nullptr;
+  float4* mac = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t nlist = 0;
+  int64_t n_bulk = 0, n_solid = 0;
+  double* partials = nullptr;
+  uint32_t* counter = nullptr;
+  double* ring = nullptr;
+  uint32_t* state = nullptr;
+  float4* xtab = nullptr;  // per-x relaxation coefficients of the bulk kernel
+
+  int64_t step = 0;
+  int parity = 0;
+
+  // readForces bookkeeping (src/solver.ts:57-61, 236-241, 315-335)
+  int64_t samples_read = 0;
+  std::deque<int64_t> force_steps;
+  int64_t generation = 0;
+  std::mutex mu;
+  std::map<int64_t, lbm::Ticket> tickets;
+  int64_t next_ticket = 1;
+
+  int dim() const { return nd == 2 ? 2 : 3; }
+  int force_every() const { return nd == 2 ? cfg2.force_every : cfg3.force_every; }
+};
+
+namespace lbm {
+// Helpers of capi.hip used by the batch driver.
+int create_2d_on(const lbm_config2d* cfg, int device, hipStream_t stream, lbm_solver** out);
+int rebuild_flags(lbm_solver* s);
+void record_steps(lbm_solver* s, int64_t n);
+}  // namespace lbm

@@ -150,6 +150,15 @@ def load_library(path: str = LIB_PATH):
         "lbm_profile_steps": ([P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)],
                               C.c_int),
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "lbm_batch_create_2d": ([C.POINTER(Config2D), C.c_int32, C.c_int, C.POINTER(P)], C.c_int),
+        "lbm_batch_destroy": ([P], None),
+        "lbm_batch_size": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
+        "lbm_batch_tunnel": ([P, C.c_int32], P),
+        "lbm_batch_set_inlet_ramp": ([P, C.c_int, C.c_double], C.c_int),
+        "lbm_batch_reset_clock": ([P], C.c_int),
+        "lbm_batch_enqueue_steps": ([P, C.c_int64], C.c_int),
+        "lbm_batch_sync": ([P], C.c_int),
+        "lbm_batch_time_steps": ([P, C.c_int64, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -178,8 +187,9 @@ class _Base:
     dim = 2
     Q = 9
 
-    def __init__(self, handle, cfg: dict):
+    def __init__(self, handle, cfg: dict, owned: bool = True):
         self._h = handle
+        self._owned = owned
         self.cfg = cfg
         dims = np.zeros(3, dtype=np.int32)
         n = C.c_int64()
@@ -281,7 +291,8 @@ class _Base:
 
     def destroy(self):
         if self._h:
-            _lib.lbm_destroy(self._h)
+            if self._owned:
+                _lib.lbm_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -383,6 +394,70 @@ class Solver3D(_Base):
     setSmagorinsky = set_smagorinsky
     setInlet = set_inlet
     initField = init_field
+
+
+class Batch:
+    """Batched D2Q9 tunnels advanced by one launch per step (include/lbm.h, lbm_batch_*)."""
+
+    @staticmethod
+    def create(configs, device: int = 0) -> "Batch":
+        lib = load_library()
+        arr = (Config2D * len(configs))(*[make_config2d(c) for c in configs])
+        h = C.c_void_p()
+        _check(lib.lbm_batch_create_2d(arr, len(configs), int(device), C.byref(h)))
+        b = Batch()
+        b._h = h
+        b.tunnels = []
+        for i, c in enumerate(configs):
+            th = lib.lbm_batch_tunnel(h, i)
+            full = dict(DEFAULTS2)
+            full.update({KEYS2.get(k, k): v for k, v in c.items()})
+            b.tunnels.append(Solver(th, full, owned=False))
+        return b
+
+    def size(self):
+        n = C.c_int32()
+        cells = C.c_int64()
+        _check(_lib.lbm_batch_size(self._h, C.byref(n), C.byref(cells)))
+        return int(n.value), int(cells.value)
+
+    def tunnel(self, i: int) -> "Solver":
+        _lib.lbm_batch_tunnel(self._h, int(i))  # marks the batch tables for refresh
+        return self.tunnels[i]
+
+    def set_inlet_ramp(self, kind: str, steps: float):
+        _check(_lib.lbm_batch_set_inlet_ramp(self._h, {"off": 0, "linear": 1, "smoothstep": 2}[kind], float(steps)))
+
+    def reset_clock(self):
+        _check(_lib.lbm_batch_reset_clock(self._h))
+
+    def enqueue_steps(self, n: int):
+        _check(_lib.lbm_batch_enqueue_steps(self._h, int(n)))
+
+    def run(self, n: int):
+        self.enqueue_steps(n)
+        self.sync()
+
+    def sync(self):
+        _check(_lib.lbm_batch_sync(self._h))
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        _check(_lib.lbm_batch_time_steps(self._h, int(n), C.byref(ms)))
+        return float(ms.value)
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            for t in self.tunnels:
+                t._h = None
+            _lib.lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def bytes_per_cell(q: int, precision: int) -> int:

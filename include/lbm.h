@@ -169,6 +169,26 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_m
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 
+/* ---- Batched D2Q9 tunnels ----
+ * No counterpart in the reference (one tunnel per page); the nearest analogue is encoding many steps
+ * into one pass (src/solver.ts:271-286). n tunnels of any sizes and parameters advance together, one
+ * kernel launch per time step for all of them, replayed from hipGraphs. Each tunnel is an ordinary
+ * 2D solver handle (lbm_batch_tunnel) sharing the batch's stream: use lbm_set_sdf, lbm_init_field,
+ * lbm_update_2d, lbm_enqueue_macro, lbm_read_macro and lbm_read_forces on it as on a lone solver, but
+ * step it only through the batch. After lbm_init_field on the tunnels call lbm_batch_reset_clock. */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create_2d(const lbm_config2d* cfgs, int32_t n, int device, lbm_batch** out);
+void lbm_batch_destroy(lbm_batch* b);
+int lbm_batch_size(const lbm_batch* b, int32_t* n_tunnels, int64_t* total_cells);
+/* Borrowed handle of tunnel i (owned by the batch: never lbm_destroy it). */
+lbm_solver* lbm_batch_tunnel(lbm_batch* b, int32_t i);
+/* Per-step inlet ramp for every tunnel toward its own configured u_in (kinds as lbm_set_inlet_ramp). */
+int lbm_batch_set_inlet_ramp(lbm_batch* b, int kind, double steps);
+int lbm_batch_reset_clock(lbm_batch* b);
+int lbm_batch_enqueue_steps(lbm_batch* b, int64_t n);
+int lbm_batch_sync(lbm_batch* b);
+int lbm_batch_time_steps(lbm_batch* b, int64_t n, double* elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif
