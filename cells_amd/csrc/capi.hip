@@ -666,6 +666,83 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_m
   return LBM_OK;
 }
 
+// Autotuner for the bulk kernel's block width: each distinct width the launcher can use for this W
+// is timed over trial_steps steps (after one warm-up step) on a copy of the solver's own state, the
+// fastest is kept, and the populations, step, parity and force ring are restored. Results are cached
+// per (lattice, precision, W, H, D) for the life of the process; the list kernel keeps 256 threads,
+// which fixes the order of the FP64 force summation.
+int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
+  if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  static std::mutex cache_mu;
+  static std::map<std::string, int> cache;
+  const std::string key = std::to_string(s->nd) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
+                          std::to_string(s->H) + "x" + std::to_string(s->D);
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      s->bulk_block = it->second;
+      if (chosen) *chosen = it->second;
+      return LBM_OK;
+    }
+  }
+  const size_t fbytes = size_t(s->Q) * s->N * s->elem;
+  void* save[2] = {nullptr, nullptr};
+  HIP_TRY(hipMalloc(&save[0], fbytes));
+  HIP_TRY(hipMalloc(&save[1], fbytes));
+  HIP_TRY(hipMemcpyAsync(save[0], s->f[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(save[1], s->f[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  const int64_t step0 = s->step;
+  const int parity0 = s->parity;
+  const int every = s->force_every();
+  const size_t nsteps = s->force_steps.size();
+  // trials run without force samples so the ring is untouched
+  if (s->nd == 2) s->cfg2.force_every = 0;
+  else s->cfg3.force_every = 0;
+  std::vector<int> tried;
+  int best = s->bulk_block;
+  double best_ms = 1e300;
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  for (int req = 64; req <= 1024; req += 64) {
+    const int bx = int(bulk_block_x(s->W, req));
+    if (std::find(tried.begin(), tried.end(), bx) != tried.end()) continue;
+    tried.push_back(bx);
+    s->bulk_block = bx;
+    enqueue_one(s);
+    HIP_TRY(hipEventRecord(a, s->stream));
+    for (int k = 0; k < trial_steps; k++) enqueue_one(s);
+    HIP_TRY(hipEventRecord(b, s->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = bx;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (s->nd == 2) s->cfg2.force_every = every;
+  else s->cfg3.force_every = every;
+  HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  (void)hipFree(save[0]);
+  (void)hipFree(save[1]);
+  s->step = step0;
+  s->parity = parity0;
+  s->force_steps.resize(nsteps);
+  s->bulk_block = best;
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    cache[key] = best;
+  }
+  if (chosen) *chosen = best;
+  return LBM_OK;
+}
+
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (bulk) *bulk = s->n_bulk;

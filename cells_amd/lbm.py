@@ -150,6 +150,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_profile_steps": ([P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)],
                               C.c_int),
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+        "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "lbm_batch_create_2d": ([C.POINTER(Config2D), C.c_int32, C.c_int, C.POINTER(P)], C.c_int),
         "lbm_batch_destroy": ([P], None),
         "lbm_batch_size": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
@@ -282,6 +283,12 @@ class _Base:
         t, b, l = C.c_double(), C.c_double(), C.c_double()
         _check(_lib.lbm_profile_steps(self._h, int(n), C.byref(t), C.byref(b), C.byref(l)))
         return float(t.value), float(b.value), float(l.value)
+
+    def autotune(self, trial_steps: int = 50) -> int:
+        """Picks the fastest bulk block width for this grid (state is restored); returns it."""
+        c = C.c_int32()
+        _check(_lib.lbm_autotune(self._h, int(trial_steps), C.byref(c)))
+        return int(c.value)
 
     def node_counts(self):
         """(bulk, list, solid) node counts."""

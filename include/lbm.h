@@ -166,6 +166,11 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n);
  * kernels; it perturbs the schedule slightly and is for per-kernel roofline figures. */
 int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
 int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_ms, double* list_ms);
+/* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
+ * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid on a copy of the
+ * solver's state, keeps the fastest (cached per lattice, precision and grid size for the process),
+ * and restores populations, step and force ring. chosen (may be NULL) receives the width. */
+int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 
