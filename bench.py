@@ -53,6 +53,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     s.set_sdf(tunnel_sdf(W, H, D))
     s.init_field()
     s.set_inlet_ramp("smoothstep", 0.1, 3000)
+    block = s.autotune(40)
     if warmup:
         s.time_steps(warmup)
     s.sync()
@@ -66,6 +67,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     out["roofline_frac"] = out["algorithmic_gbs"] / HBM_PEAK_GBS
     nb, nl, ns = s.node_counts()
     out["nodes"] = dict(bulk=nb, list=nl, solid=ns)
+    out["bulk_block"] = block
     if profile:
         tot, tb, tl = s.profile_steps(steps)
         b = 2 if s.precision else 4
@@ -115,6 +117,27 @@ def cpu_baseline(W, H, D, budget_s=12.0):
                        f"{dt:.1f} s on {threads} threads of {cpu}")
 
 
+def sweep_bench(steps):
+    """The 64-tunnel NACA0012 sweep: one batch vs the same tunnels as sequential plain runs."""
+    import numpy as np
+
+    from cells_amd import sweep
+
+    cases = sweep.sweep_cases(64, seed=0)
+    tb, rb = sweep.run_batched(cases, steps)
+    ts, rs = sweep.run_sequential(cases, steps)
+    worst = 0.0
+    for c, (sb, vb), (ss, vs) in zip(cases, rb, rs):
+        cd_b, cl_b = sweep.coefficients(sb, vb, c["chord"], min(2000, steps // 2))
+        cd_s, cl_s = sweep.coefficients(ss, vs, c["chord"], min(2000, steps // 2))
+        worst = max(worst, abs(cd_b - cd_s) / max(abs(cd_s), 1e-12), abs(cl_b - cl_s) / max(abs(cl_s), 1e-3))
+    bm, sm = sweep.mlups(cases, steps, tb), sweep.mlups(cases, steps, ts)
+    return dict(tunnels=len(cases), cells=sum(c["config"]["width"] * c["config"]["height"] for c in cases),
+                steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
+                batched_s=tb, sequential_s=ts, mean_cd_cl_max_rel_diff=worst,
+                finite=bool(all(np.isfinite(v).all() for _, v in rb)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +145,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="FP32 192x96x96 only (for profiling)")
+    ap.add_argument("--sweep-steps", type=int, default=10000, help="steps of the 64-tunnel sweep (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
@@ -137,6 +161,11 @@ def main():
         for prec in ("fp32", "fp16"):
             variants[f"{prec}_256x128x128"] = run_gpu(256, 128, 128, prec, max(1, args.steps // 2),
                                                       args.warmup // 2)
+    from cells_amd.lbm import copy_roofline
+
+    copy = {"fp32_192x96x96_gbs": copy_roofline(W * H * D, 19, 4), "fp16_192x96x96_gbs": copy_roofline(W * H * D, 19, 2),
+            "fp32_256x128x128_gbs": copy_roofline(256 * 128 * 128, 19, 4)}
+    sweep_res = sweep_bench(args.sweep_steps) if (args.sweep_steps > 0 and not args.quick) else None
     cpu = None if args.no_cpu else cpu_baseline(W, H, D)
 
     tb = main_run["bulk_kernel_us"] * 1e-6
@@ -166,9 +195,12 @@ def main():
         "cpu_baseline": cpu,
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "bulk_kernel_us", "list_kernel_us", "bulk_achieved_gbs",
-                                               "finite") if kk in v}
+                                               "bulk_block", "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],
+        "bulk_block": main_run["bulk_block"],
+        "copy_roofline": copy,
+        "sweep": sweep_res,
     }
     print(json.dumps(line))
 

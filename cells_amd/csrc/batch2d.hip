@@ -5,10 +5,10 @@
 // single-tunnel solver (capi.hip) created on the batch's stream, so setSdf / initField / readMacro
 // / readForces keep their per-tunnel semantics; only the stepping is shared. One launch covers every
 // tunnel through a flat block table: blocks [0, n_bulk) are bulk tiles of 256 consecutive cells of
-// one tunnel, the rest are boundary-list tiles of 256 list nodes of one tunnel. Both tile kinds run
-// the same per-cell code as the single-tunnel kernels (d2q9.hpp), and the list tiles of each tunnel
-// reduce that tunnel's momentum-exchange force into its own FP64 ring, so a batched tunnel computes
-// exactly what it would alone. Per-tunnel parameters, the inlet ramp and the force cadence are read
+// one tunnel, then the link tiles and the body tiles of the tunnels' boundary lists, 256 nodes each.
+// Every tile kind runs the same per-cell code as the single-tunnel kernels (d2q9.hpp), and the body
+// tiles of each tunnel reduce that tunnel's momentum-exchange force into its own FP64 ring in the
+// same block order, so a batched tunnel computes exactly what it would alone. Per-tunnel parameters, the inlet ramp and the force cadence are read
 // on the device from a descriptor table and a device step counter, so a captured graph stays valid
 // while the host advances time.
 #include <hip/hip_runtime.h>
@@ -31,12 +31,14 @@ struct TunnelDesc {
   const float* sdf;
   const uint32_t* list;
   const float4* xtab;
+  const uint32_t* gtab;
+  const uint32_t* gmov;
   double* partials;
   uint32_t* counter;
   double* ring;
   uint32_t* state;
-  uint32_t nlist;
-  uint32_t bulk_block0, list_block0, n_list_blocks;
+  uint32_t n_link, n_body;
+  uint32_t bulk_block0, link_block0, body_block0, n_body_blocks;
   // y = floor(idx / W) by multiplication: Granlund & Montgomery, "Division by invariant integers
   // using multiplication" (PLDI 1994), round-up variant: q = (mulhi(n, m) + n) >> l, exact for all
   // 32-bit n, with l = ceil(log2 W) and m = floor(2^32 (2^l - W) / W) + 1.
@@ -50,9 +52,12 @@ __device__ __forceinline__ uint32_t div_w(uint32_t n, uint32_t m, uint32_t l) {
   return uint32_t((uint64_t(__umulhi(n, m)) + n) >> l);
 }
 
+// Block table layout: bulk tiles of every tunnel, then link tiles of every tunnel, then body tiles
+// of every tunnel (each tunnel has at least one body tile, which writes its force samples).
 __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict__ T,
                                                    const uint16_t* __restrict__ block_tunnel, uint32_t n_bulk_blocks,
-                                                   int parity, const uint64_t* __restrict__ base_step, uint32_t k) {
+                                                   uint32_t link_end, int parity, const uint64_t* __restrict__ base_step,
+                                                   uint32_t k) {
   const uint32_t b = blockIdx.x;
   const TunnelDesc& d = T[block_tunnel[b]];
   Params2 p = d.p;
@@ -73,13 +78,18 @@ __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict_
     bulk_cell2(src, dst, d.flags, d.xtab, p, x, int(y), idx);
     return;
   }
-  const uint32_t tile = b - d.list_block0;
+  if (b < link_end) {
+    const uint32_t kl = (b - d.link_block0) * 256u + threadIdx.x;
+    if (kl < d.n_link) link_cell2(src, dst, d.flags, d.xtab, d.gtab, d.gmov, d.n_link, kl, d.list[kl], p);
+    return;
+  }
+  const uint32_t tile = b - d.body_block0;
   const uint32_t kk = tile * 256u + threadIdx.x;
   float force[2] = {0.0f, 0.0f};
-  if (kk < d.nlist) list_cell2(src, dst, d.flags, d.sdf, d.list[kk], p, force);
+  if (kk < d.n_body) list_cell2(src, dst, d.flags, d.sdf, d.list[d.n_link + kk], p, force);
   if (d.force_every > 0 && (step + 1) % uint64_t(d.force_every) == 0) {
     ForceSink fs{d.partials, d.counter, d.ring, d.state, 1};
-    force_sample<2>(force, fs, tile, d.n_list_blocks);
+    force_sample<2>(force, fs, tile, d.n_body_blocks);
   }
 }
 
@@ -97,7 +107,7 @@ struct lbm_batch {
   uint16_t* d_block_tunnel = nullptr;
   size_t block_cap = 0;
   uint64_t* d_step = nullptr;
-  uint32_t n_bulk_blocks = 0, n_blocks = 0;
+  uint32_t n_bulk_blocks = 0, link_end = 0, n_blocks = 0;
   int64_t step = 0;
   int parity = 0;
   int ramp_kind = 0;
@@ -140,11 +150,14 @@ int refresh(lbm_batch* b) {
     d.sdf = s->sdf;
     d.list = s->list;
     d.xtab = s->xtab;
+    d.gtab = s->gtab;
+    d.gmov = s->gmov;
     d.partials = s->partials;
     d.counter = s->counter;
     d.ring = s->ring;
     d.state = s->state;
-    d.nlist = s->nlist;
+    d.n_link = s->n_link;
+    d.n_body = s->n_body;
     magic_for(uint32_t(s->W), &d.magic, &d.shift);
     d.force_every = s->cfg2.force_every;
     d.ramp_kind = b->ramp_kind;
@@ -153,20 +166,30 @@ int refresh(lbm_batch* b) {
     d.bulk_block0 = blocks;
     blocks += (s->N + 255) / 256;
   }
+  if (blocks != b->n_bulk_blocks) drop_graphs(b);
   b->n_bulk_blocks = blocks;
   for (size_t i = 0; i < n; i++) {
     TunnelDesc& d = b->desc[i];
-    d.list_block0 = blocks;
-    d.n_list_blocks = list_grid2(d.nlist);
-    blocks += d.n_list_blocks;
+    d.link_block0 = blocks;
+    blocks += link_blocks(d.n_link);
+  }
+  if (blocks != b->link_end) drop_graphs(b);
+  b->link_end = blocks;
+  for (size_t i = 0; i < n; i++) {
+    TunnelDesc& d = b->desc[i];
+    d.body_block0 = blocks;
+    d.n_body_blocks = body_blocks(d.n_body, true);
+    blocks += d.n_body_blocks;
   }
   if (blocks > 0x7fffffffu) return fail(LBM_ERR_INVALID, "batch too large for one launch");
+  if (blocks != b->n_blocks) drop_graphs(b);  // the captured launches hold the old grid and table bounds
   b->n_blocks = blocks;
   b->block_tunnel.assign(blocks, 0);
   for (size_t i = 0; i < n; i++) {
     const TunnelDesc& d = b->desc[i];
     for (uint32_t k = d.bulk_block0; k < d.bulk_block0 + (b->t[i]->N + 255) / 256; k++) b->block_tunnel[k] = uint16_t(i);
-    for (uint32_t k = d.list_block0; k < d.list_block0 + d.n_list_blocks; k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.link_block0; k < d.link_block0 + link_blocks(d.n_link); k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.body_block0; k < d.body_block0 + d.n_body_blocks; k++) b->block_tunnel[k] = uint16_t(i);
   }
   if (blocks > b->block_cap) {
     if (b->d_block_tunnel) HIP_TRY(hipFree(b->d_block_tunnel));
@@ -183,7 +206,8 @@ int refresh(lbm_batch* b) {
 }
 
 void launch_step(lbm_batch* b, int parity, uint32_t k) {
-  batch2_step<<<b->n_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_block_tunnel, b->n_bulk_blocks, parity, b->d_step, k);
+  batch2_step<<<b->n_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_block_tunnel, b->n_bulk_blocks, b->link_end, parity,
+                                                  b->d_step, k);
 }
 
 // Graph of GRAPH_STEPS steps starting at the given parity, reading the step from d_step + k.

@@ -43,7 +43,7 @@ void free_all(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
                   (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state,
-                  (void*)s->xtab})
+                  (void*)s->xtab, (void*)s->gtab, (void*)s->gmov})
     if (p) (void)hipFree(p);
   for (auto& kv : s->tickets) {
     if (kv.second.ring) (void)hipHostFree(kv.second.ring);
@@ -116,6 +116,166 @@ void inlet_feq3(float u, float* out) {
   }
 }
 
+// Host resolution of one pull of a boundary node into a single gather index (population * N + cell),
+// following the rule order of pullSlow3 (src/shaders/step3d.ts:119-168). Returns false when the pull
+// is a body link (Bouzidi interpolation, or a slip face whose mirror node is solid), which stays on
+// the dynamic path because it also contributes force.
+bool resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g) {
+  const int W = s->W, H = s->H, D = s->D;
+  const uint32_t N = s->N;
+  const uint32_t WH = uint32_t(W) * uint32_t(H);
+  const int z = int(idx / WH), y = int((idx % WH) / uint32_t(W)), x = int(idx % uint32_t(W));
+  const int j = opp3(i);
+  const int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
+  auto at = [&](int ax, int ay, int az) { return uint32_t(ax) + uint32_t(W) * (uint32_t(ay) + uint32_t(H) * uint32_t(az)); };
+  const uint32_t f = fl[idx];
+  if ((f & (1u << i)) == 0u) {
+    *g = uint32_t(i) * N + at(sx, sy, sz);
+    return true;
+  }
+  const bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
+  const bool outlet = (f & FLAG_OUTLET3) != 0u;
+  if (outX && outlet) {
+    *g = uint32_t(i) * N + at(W - 2, y, z);
+  } else if (outY && outZ) {
+    *g = uint32_t(j) * N + idx;
+  } else if (outY || outZ) {
+    const uint32_t rs = outY ? at(sx, y, sz) : at(sx, sy, z);
+    if (fl[rs] & FLAG_SOLID3) return false;
+    *g = uint32_t(outY ? mirror_y3(i) : mirror_z3(i)) * N + rs;
+  } else if (outX) {
+    *g = uint32_t(j) * N + idx;
+  } else {
+    const uint32_t sc = at(sx, sy, sz);
+    if (fl[sc] & FLAG_SOLID3) return false;
+    *g = uint32_t(i) * N + sc;
+  }
+  return true;
+}
+
+// The same for pullSlow (src/shaders/step.ts:119-175): periodic wrap, Bouzidi (a body link), slip
+// mirror in the same row, moving lid (a gather plus the additive term flagged in *moving),
+// bounce-back and the open-boundary placeholders.
+bool resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, bool* moving) {
+  const int W = s->W, H = s->H;
+  const uint32_t N = s->N;
+  const Params2& p = s->p2;
+  const int y = int(idx / uint32_t(W)), x = int(idx % uint32_t(W));
+  const int j = OPP2[i];
+  int sx = x - CX2[i], sy = y - CY2[i];
+  *moving = false;
+  if ((fl[idx] & (1u << i)) == 0u) {
+    *g = uint32_t(i) * N + uint32_t(sy * W + sx);
+    return true;
+  }
+  if (sx < 0 && p.left == 0) sx += W;
+  if (sx >= W && p.right == 0) sx -= W;
+  if (sy < 0 && p.bottom == 0) sy += H;
+  if (sy >= H && p.top == 0) sy -= H;
+  const bool inX = sx >= 0 && sx < W, inY = sy >= 0 && sy < H;
+  if (inX && inY) {
+    const uint32_t sc = uint32_t(sy * W + sx);
+    if (fl[sc] & FLAG_SOLID2) return false;
+    *g = uint32_t(i) * N + sc;
+  } else if (!inY) {
+    const int mode = sy < 0 ? p.bottom : p.top;
+    if (mode == 2) {
+      const int rx = std::min(std::max(sx, 0), W - 1);
+      *g = uint32_t(MIRROR_Y2[i]) * N + uint32_t(y) * uint32_t(W) + uint32_t(rx);
+    } else {
+      *g = uint32_t(j) * N + idx;
+      *moving = mode == 3;
+    }
+  } else {
+    *g = uint32_t(j) * N + idx;
+  }
+  return true;
+}
+
+// The boundary list and its gather tables from the host copy of the flag words.
+int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
+  const uint32_t N = s->N;
+  const bool d2 = s->nd == 2;
+  const uint32_t solid = d2 ? FLAG_SOLID2 : FLAG_SOLID3;
+  const uint32_t inlet_flag = d2 ? FLAG_INLET2 : FLAG_INLET3;
+  const uint32_t outlet_flag = d2 ? FLAG_OUTLET2 : FLAG_OUTLET3;
+  const uint32_t plain_in = d2 ? (FLAG_INLET2 | INLET_DIRS2) : (FLAG_INLET3 | INLET_DIRS3);
+  const uint32_t plain_out = d2 ? (FLAG_OUTLET2 | OUTLET_DIRS2) : (FLAG_OUTLET3 | OUTLET_DIRS3);
+  // Boundary list: nodes the bulk kernel skips. Link nodes (every pull resolves to one gather: faces,
+  // edges, walls, seams, open columns) come first, grouped inlet / outlet / other; body nodes (a
+  // Bouzidi link, or a 3D slip face mirroring a solid node) follow. All groups are in cell order.
+  const int Q = s->Q;
+  std::vector<uint32_t> link[3], body;
+  std::vector<uint32_t> gt;  // gather table, direction-major: gt[(i - 1) * n_link + k]
+  std::vector<uint32_t> gm;                  // 2D moving-lid mask per link node
+  s->n_bulk = 0;
+  s->n_solid = 0;
+  for (uint32_t k = 0; k < N; k++) {
+    const uint32_t v = fl[k];
+    if (v & solid) {
+      s->n_solid++;
+      continue;
+    }
+    if (v == 0 || v == plain_in || v == plain_out) {
+      s->n_bulk++;
+      continue;
+    }
+    const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
+    bool is_body = false;
+    if (!(s->nd == 3 && kind == 0)) {
+      for (int i = 1; i < Q && !is_body; i++) {
+        uint32_t gi = 0;
+        bool mov = false;
+        is_body = s->nd == 2 ? !resolve2(s, fl.data(), k, i, &gi, &mov) : !resolve3(s, fl.data(), k, i, &gi);
+      }
+    }
+    if (is_body) body.push_back(k);
+    else link[kind].push_back(k);
+  }
+  std::vector<uint32_t> lst;
+  for (auto& l : link) lst.insert(lst.end(), l.begin(), l.end());
+  const uint32_t n_link = uint32_t(lst.size());
+  lst.insert(lst.end(), body.begin(), body.end());
+  gt.assign(size_t(Q - 1) * n_link, 0u);
+  if (s->nd == 2) gm.assign(n_link, 0u);
+  for (uint32_t k = 0; k < n_link; k++) {
+    const uint32_t idx = lst[k];
+    if (s->nd == 3 && (fl[idx] & inlet_flag)) {
+      for (int i = 1; i < Q; i++) gt[size_t(i - 1) * n_link + k] = idx;  // the inlet takes no pulls
+      continue;
+    }
+    for (int i = 1; i < Q; i++) {
+      uint32_t gi = 0;
+      bool mov = false;
+      if (s->nd == 2) {
+        resolve2(s, fl.data(), idx, i, &gi, &mov);
+        if (mov) gm[k] |= 1u << i;
+      } else {
+        resolve3(s, fl.data(), idx, i, &gi);
+      }
+      gt[size_t(i - 1) * n_link + k] = gi;
+    }
+  }
+  s->nlist = uint32_t(lst.size());
+  s->n_link = n_link;
+  s->n_body = s->nlist - n_link;
+  if (s->gtab) HIP_TRY(hipFree(s->gtab));
+  if (s->gmov) HIP_TRY(hipFree(s->gmov));
+  s->gtab = nullptr;
+  s->gmov = nullptr;
+  if (n_link) {
+    HIP_TRY(hipMalloc(&s->gtab, sizeof(uint32_t) * gt.size()));
+    HIP_TRY(hipMemcpyAsync(s->gtab, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
+    if (s->nd == 2) {
+      HIP_TRY(hipMalloc(&s->gmov, sizeof(uint32_t) * n_link));
+      HIP_TRY(hipMemcpyAsync(s->gmov, gm.data(), sizeof(uint32_t) * n_link, hipMemcpyHostToDevice, s->stream));
+    }
+  }
+  if (s->nlist) HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * lst.size(), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return LBM_OK;
+}
+
 // rebuildFlags (src/solver.ts:205-217, src/solver3d.ts:216-228): macro on the current buffer, flags
 // and refill on the GPU, then the boundary list (and 2D force slots) from the flags, in cell order.
 int rebuild_flags_impl(lbm_solver* s) {
@@ -138,13 +298,7 @@ int rebuild_flags_impl(lbm_solver* s) {
   }
   HIP_TRY(hipStreamSynchronize(s->stream));
 
-  uint32_t solid, inlet_flag, outlet_flag, plain_in, plain_out;
   if (s->nd == 2) {
-    solid = FLAG_SOLID2;
-    inlet_flag = FLAG_INLET2;
-    outlet_flag = FLAG_OUTLET2;
-    plain_in = FLAG_INLET2 | INLET_DIRS2;
-    plain_out = FLAG_OUTLET2 | OUTLET_DIRS2;
     // Force slots in cell order (the reference's atomicAdd order is arbitrary, src/shaders/aux.ts:63-66).
     uint32_t slot = 0;
     bool changed = false;
@@ -157,34 +311,8 @@ int rebuild_flags_impl(lbm_solver* s) {
       }
     }
     if (changed) HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s->stream));
-  } else {
-    solid = FLAG_SOLID3;
-    inlet_flag = FLAG_INLET3;
-    outlet_flag = FLAG_OUTLET3;
-    plain_in = FLAG_INLET3 | INLET_DIRS3;
-    plain_out = FLAG_OUTLET3 | OUTLET_DIRS3;
   }
-  // Boundary list: nodes the bulk kernel skips, grouped as inlet, outlet, other (uniform waves).
-  std::vector<uint32_t> lst;
-  s->n_bulk = 0;
-  s->n_solid = 0;
-  for (uint32_t k = 0; k < N; k++) {
-    const uint32_t v = fl[k];
-    if (v & solid) s->n_solid++;
-    else if (v == 0 || v == plain_in || v == plain_out) s->n_bulk++;
-  }
-  for (int pass = 0; pass < 3; pass++) {
-    for (uint32_t k = 0; k < N; k++) {
-      const uint32_t v = fl[k];
-      if (v == 0 || (v & solid) || v == plain_in || v == plain_out) continue;
-      const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
-      if (kind == pass) lst.push_back(k);
-    }
-  }
-  s->nlist = uint32_t(lst.size());
-  if (s->nlist) HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * lst.size(), hipMemcpyHostToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  return LBM_OK;
+  return build_links(s, fl);
 }
 
 int alloc_common(lbm_solver* s) {
@@ -235,14 +363,15 @@ int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
   void* dst = s->f[s->parity ^ 1];
   if (s->nd == 2) {
     if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
-    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, s->list, s->nlist, s->p2,
-                 s->xtab, fs, s->bulk_block, s->stream, mid);
+    const ListTables lt{s->list, s->gtab, s->gmov, s->n_link, s->n_body};
+    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, lt, s->p2, s->xtab, fs,
+                 s->bulk_block, s->stream, mid);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];
     inlet_feq3(s->p3.uIn, fin);
-    launch_step3(s->prec, src, dst, s->flags, s->sdf, s->list, s->nlist, s->p3, fin, s->xtab, fs, s->bulk_block,
-                 s->stream, mid);
+    const ListTables lt{s->list, s->gtab, s->gmov, s->n_link, s->n_body};
+    launch_step3(s->prec, src, dst, s->flags, s->sdf, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, mid);
   }
   s->parity ^= 1;
   s->step++;
@@ -392,9 +521,19 @@ int lbm_update_2d(lbm_solver* s, const lbm_config2d* cfg) {
   if (cfg->width != s->W || cfg->height != s->H) return fail(LBM_ERR_INVALID, "width and height cannot change");
   const int bb = bulk_block_for(cfg->workgroup_size);
   if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
+  const bool modes_changed = cfg->left != s->cfg2.left || cfg->right != s->cfg2.right ||
+                             cfg->bottom != s->cfg2.bottom || cfg->top != s->cfg2.top;
   s->cfg2 = *cfg;
   s->bulk_block = bb;
   write_params2(s);
+  if (modes_changed) {
+    // The reference reads the modes in pullSlow every step with the flags it has; re-resolve the
+    // gather tables against the current flags so the same rules apply.
+    std::vector<uint32_t> fl(s->N);
+    HIP_TRY(hipMemcpyAsync(fl.data(), s->flags, sizeof(uint32_t) * s->N, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return build_links(s, fl);
+  }
   return LBM_OK;
 }
 

@@ -254,7 +254,41 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   for (int i = 0; i < 9; i++) *reinterpret_cast<float*>(d + i * N * 4 + off) = out[i];
 }
 
-// One node of the boundary list: the reference's slow path with every pullSlow rule, Zou-He and
+// One link node of the boundary list: every pull is a single gather through the precomputed table
+// (k is the node's position in the list), plus the moving-lid term where the table says so.
+__device__ __forceinline__ void link_cell2(const float* __restrict__ src, float* __restrict__ dst,
+                                           const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
+                                           const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ gmov,
+                                           uint32_t n_link, uint32_t k, uint32_t idx, const Params2& p) {
+  const uint32_t fl = flags[idx];
+  const uint32_t N = p.N;
+  const int y = int(idx / uint32_t(p.W));
+  const int x = int(idx - uint32_t(y) * uint32_t(p.W));
+  float f[9];
+  f[0] = src[idx];
+#pragma unroll
+  for (int i = 1; i < 9; i++) f[i] = src[gtab[size_t(i - 1) * n_link + k]];
+  const uint32_t mv = gmov[k];
+  if (mv) {
+#pragma unroll
+    for (int i = 1; i < 9; i++)
+      if (mv & (1u << i)) f[i] = f[i] + 6.0f * W2[i] * float(CX2[i]) * p.uLid;
+  }
+  if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
+  else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
+  const float4 t = xtab[x];
+  Relax2 r;
+  r.omBase = t.x;
+  r.omM = t.y;
+  r.sigma = t.z;
+  r.t0 = t.w;
+  float out[9];
+  collide2(f, r, y, p, out);
+#pragma unroll
+  for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
+}
+
+// One body node of the boundary list: the reference's slow path with every pullSlow rule, Zou-He and
 // the collision; accumulates the node's momentum-exchange force.
 __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float* __restrict__ sdf,

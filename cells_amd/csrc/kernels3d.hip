@@ -304,18 +304,55 @@ struct PullAll3<S, 19> {
                                              uint32_t, uint32_t, bool, const Params3&, float (&)[19], float (&)[3]) {}
 };
 
+// Boundary-list kernel. Blocks [0, nlb) take the link nodes: their pulls are single gathers through
+// the precomputed table (the reference's pullSlow3 rule for that node and direction, resolved once
+// per geometry), so all 19 loads are independent. The remaining blocks take the body nodes, which
+// run pullSlow3 itself (Bouzidi interpolation, slip through a solid) and carry the force sample.
 template <class S>
 __global__ __launch_bounds__(256) void step3_list(const typename S::elem* __restrict__ src,
                                                   typename S::elem* __restrict__ dst,
                                                   const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                                  const uint32_t* __restrict__ list, uint32_t nlist, Params3 p,
-                                                  FeqIn fin, ForceSink fs) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  float force[3] = {0.0f, 0.0f, 0.0f};
-  if (k < nlist) {
-    const uint32_t idx = list[k];
+                                                  ListTables lt, uint32_t nlb, Params3 p, FeqIn fin,
+                                                  const float4* __restrict__ xtab, ForceSink fs) {
+  const uint32_t N = p.N;
+  if (blockIdx.x < nlb) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= lt.n_link) return;
+    const uint32_t idx = lt.list[k];
     const uint32_t fl = flags[idx];
-    const uint32_t N = p.N;
+    if (fl & FLAG_INLET3) {
+#pragma unroll
+      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
+      return;
+    }
+    float f[19];
+    f[0] = S::load(src, idx);
+#pragma unroll
+    for (int i = 1; i < 19; i++) f[i] = S::load(src, lt.gtab[size_t(i - 1) * lt.n_link + k]);
+    const Mom3 m = moments3(f);
+    if (fl & FLAG_OUTLET3) {
+#pragma unroll
+      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+      return;
+    }
+    const uint32_t x = idx % uint32_t(p.W);
+    const float4 t = xtab[x];
+    Relax r;
+    r.omBase = t.x;
+    r.omM = t.y;
+    r.sigma = t.z;
+    r.t0 = t.w;
+    float out[19];
+    collide3(f, m, r, p, out);
+#pragma unroll
+    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+    return;
+  }
+  const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
+  float force[3] = {0.0f, 0.0f, 0.0f};
+  if (kb < lt.n_body) {
+    const uint32_t idx = lt.list[lt.n_link + kb];
+    const uint32_t fl = flags[idx];
     if (fl & FLAG_INLET3) {
 #pragma unroll
       for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
@@ -340,7 +377,7 @@ __global__ __launch_bounds__(256) void step3_list(const typename S::elem* __rest
       }
     }
   }
-  if (fs.sample) force_sample<3>(force, fs);
+  if (fs.sample) force_sample<3>(force, fs, blockIdx.x - nlb, gridDim.x - nlb);
 }
 
 // ============================================================================ aux kernels
@@ -457,7 +494,8 @@ __global__ void reset_state_kernel(uint32_t* state) { state[0] = 0u; }
 
 // ============================================================================ launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
-uint32_t list_grid3(uint32_t nlist) { return nlist ? blocks_for(nlist, 256) : 1u; }
+uint32_t link_blocks(uint32_t n_link) { return blocks_for(n_link, 256); }
+uint32_t body_blocks(uint32_t n_body, bool sample) { return n_body ? blocks_for(n_body, 256) : (sample ? 1u : 0u); }
 
 // Threads per block along x: the widest multiple of 64 not above the requested size that divides W
 // (no idle lanes), else one block of ceil(W / 64) waves per row.
@@ -470,9 +508,9 @@ uint32_t bulk_block_x(int W, int requested) {
 }
 
 template <class S>
-static void step3_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                    uint32_t nlist, const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs,
-                    int bb, hipStream_t st, hipEvent_t mid) {
+static void step3_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+                    const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bb,
+                    hipStream_t st, hipEvent_t mid) {
   using E = typename S::elem;
   FeqIn fin;
   for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
@@ -480,18 +518,20 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const flo
   const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H), uint32_t(p.D));
   step3_bulk<S><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin);
   if (mid) (void)hipEventRecord(mid, st);
-  if (nlist || fs.sample)
-    step3_list<S><<<list_grid3(nlist), 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, sdf, list,
-                                                      nlist, p, fin, fs);
+  const uint32_t nlb = link_blocks(lt.n_link);
+  const uint32_t nbb = body_blocks(lt.n_body, fs.sample != 0);
+  if (nlb + nbb)
+    step3_list<S><<<nlb + nbb, 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, sdf, lt, nlb, p,
+                                              fin, xtab, fs);
 }
 
-void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const uint32_t* list,
-                  uint32_t nlist, const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs,
-                  int bulk_block, hipStream_t st, hipEvent_t mid) {
+void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+                  const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
+                  hipStream_t st, hipEvent_t mid) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    default: step3_t<StoreF32>(src, dst, flags, sdf, list, nlist, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 1: step3_t<StoreF16>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    default: step3_t<StoreF32>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
   }
 }
 

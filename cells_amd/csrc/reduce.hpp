@@ -1,11 +1,16 @@
 // Deterministic FP64 force reduction used by the boundary-list step kernels.
 //
 // Replaces the reference's single-workgroup FP32 reduce pass (src/shaders/aux.ts:144-172,
-// src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every block of the list kernel
-// sums its cells' momentum-exchange forces in double with wave64 butterfly shuffles and LDS, writes
-// one partial, and the last block to finish (completion counter) sums the partials in block order
-// and appends the sample to the history ring. The summation order is fixed, so a sample is
+// src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every block of the group sums its
+// cells' momentum-exchange forces in double with wave64 butterfly shuffles and LDS, publishes one
+// partial, and the last block to finish (completion counter) sums the partials in block order and
+// appends the sample to the history ring. The summation order is fixed, so a sample is
 // bit-reproducible run to run.
+//
+// Publication uses device-scope (agent) atomic stores and loads, which are coherent across the
+// XCDs' separate L2 caches, and a wait for the stores' completion before the counter increment.
+// A device-scope release fence would also be correct but writes back every dirty L2 line of the
+// issuing XCD, i.e. much of what the step just stored, once per block.
 #pragma once
 #include "lbm_device.hpp"
 
@@ -52,14 +57,14 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uin
   block_sum<DIM>(v, lds);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int d = 0; d < DIM; d++) fs.partials[size_t(block) * DIM + d] = v[d];
-    __threadfence();
-    const uint32_t prev = atomicAdd(fs.counter, 1u);
+    for (int d = 0; d < DIM; d++)
+      __hip_atomic_store(fs.partials + size_t(block) * DIM + d, v[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the partial stores have completed before the counter moves
+    const uint32_t prev = __hip_atomic_fetch_add(fs.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == nblocks - 1);
   }
   __syncthreads();
   if (!is_last) return;
-  __threadfence();
   double acc[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) acc[d] = 0.0;
@@ -69,12 +74,12 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uin
       acc[d] += __hip_atomic_load(fs.partials + size_t(k) * DIM + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   block_sum<DIM>(acc, lds);
   if (threadIdx.x == 0) {
-    const uint32_t st = *fs.state;
+    const uint32_t st = __hip_atomic_load(fs.state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t slot = st % HISTORY_LEN;
 #pragma unroll
     for (int d = 0; d < DIM; d++) fs.ring[size_t(slot) * DIM + d] = acc[d];
-    *fs.state = st + 1;
-    atomicExch(fs.counter, 0u);
+    __hip_atomic_store(fs.state, st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(fs.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

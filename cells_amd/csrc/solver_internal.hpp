@@ -71,6 +71,9 @@ struct lbm_solver {
   float4* mac = nullptr;
   uint32_t* list = nullptr;
   uint32_t nlist = 0;
+  uint32_t n_link = 0, n_body = 0;  // the list holds n_link gather nodes, then n_body body-link nodes
+  uint32_t* gtab = nullptr;         // gather table of the link nodes, direction-major
+  uint32_t* gmov = nullptr;         // 2D: moving-lid mask of the link nodes
   int64_t n_bulk = 0, n_solid = 0;
   double* partials = nullptr;
   uint32_t* counter = nullptr;

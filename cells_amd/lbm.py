@@ -151,6 +151,7 @@ def load_library(path: str = LIB_PATH):
                               C.c_int),
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "lbm_batch_create_2d": ([C.POINTER(Config2D), C.c_int32, C.c_int, C.POINTER(P)], C.c_int),
         "lbm_batch_destroy": ([P], None),
         "lbm_batch_size": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
@@ -465,6 +466,14 @@ class Batch:
             self.destroy()
         except Exception:
             pass
+
+
+def copy_roofline(n_cells: int, q: int = 19, elem_bytes: int = 4, iters: int = 200, device: int = 0) -> float:
+    """GB/s of the copy-roofline kernel: q populations + one flag word read, q populations written."""
+    lib = load_library()
+    ms = C.c_double()
+    _check(lib.lbm_copy_roofline(int(device), int(n_cells), int(q), int(elem_bytes), int(iters), C.byref(ms)))
+    return n_cells * (q * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
 
 
 def bytes_per_cell(q: int, precision: int) -> int:

@@ -171,6 +171,10 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_m
  * solver's state, keeps the fastest (cached per lattice, precision and grid size for the process),
  * and restores populations, step and force ring. chosen (may be NULL) receives the width. */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
+/* Copy roofline (src/bench.ts:61-95): iters launches of a kernel that reads q structure-of-arrays
+ * populations of elem_bytes each plus a u32 flag word per cell and writes the q populations, with
+ * no arithmetic; returns their total milliseconds. The memory-only ceiling of a step. */
+int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t iters, double* ms_total);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 
