@@ -131,11 +131,17 @@ def sweep_bench(steps):
         cd_b, cl_b = sweep.coefficients(sb, vb, c["chord"], min(2000, steps // 2))
         cd_s, cl_s = sweep.coefficients(ss, vs, c["chord"], min(2000, steps // 2))
         worst = max(worst, abs(cd_b - cd_s) / max(abs(cd_s), 1e-12), abs(cl_b - cl_s) / max(abs(cl_s), 1e-3))
+    bad = []
+    for i, (c, (sb, vb)) in enumerate(zip(cases, rb)):
+        ok = np.isfinite(vb).all(axis=1)
+        if not ok.all():
+            bad.append(dict(tunnel=i, nx=c["config"]["width"], alpha_deg=round(c["alpha_deg"], 3),
+                            re=round(c["re"], 1), tau=c["config"]["tau"], first_bad_step=int(sb[~ok][0])))
     bm, sm = sweep.mlups(cases, steps, tb), sweep.mlups(cases, steps, ts)
     return dict(tunnels=len(cases), cells=sum(c["config"]["width"] * c["config"]["height"] for c in cases),
                 steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
                 batched_s=tb, sequential_s=ts, mean_cd_cl_max_rel_diff=worst,
-                finite=bool(all(np.isfinite(v).all() for _, v in rb)))
+                finite=not bad, nonfinite_tunnels=bad)
 
 
 def main():

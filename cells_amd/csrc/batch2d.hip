@@ -28,16 +28,12 @@ struct TunnelDesc {
   Params2 p;
   float* f[2];
   const uint32_t* flags;
-  const float* sdf;
-  const uint32_t* list;
   const float4* xtab;
-  const uint32_t* gtab;
-  const uint32_t* gmov;
+  ListTables lt;
   double* partials;
   uint32_t* counter;
   double* ring;
   uint32_t* state;
-  uint32_t n_link, n_body;
   uint32_t bulk_block0, link_block0, body_block0, n_body_blocks;
   // y = floor(idx / W) by multiplication: Granlund & Montgomery, "Division by invariant integers
   // using multiplication" (PLDI 1994), round-up variant: q = (mulhi(n, m) + n) >> l, exact for all
@@ -80,13 +76,14 @@ __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict_
   }
   if (b < link_end) {
     const uint32_t kl = (b - d.link_block0) * 256u + threadIdx.x;
-    if (kl < d.n_link) link_cell2(src, dst, d.flags, d.xtab, d.gtab, d.gmov, d.n_link, kl, d.list[kl], p);
+    float none[2] = {0.0f, 0.0f};
+    if (kl < d.lt.n_link) list_cell2<false>(src, dst, d.flags, d.xtab, d.lt, kl, p, none);
     return;
   }
   const uint32_t tile = b - d.body_block0;
   const uint32_t kk = tile * 256u + threadIdx.x;
   float force[2] = {0.0f, 0.0f};
-  if (kk < d.n_body) list_cell2(src, dst, d.flags, d.sdf, d.list[d.n_link + kk], p, force);
+  if (kk < d.lt.n_body) list_cell2<true>(src, dst, d.flags, d.xtab, d.lt, d.lt.n_link + kk, p, force);
   if (d.force_every > 0 && (step + 1) % uint64_t(d.force_every) == 0) {
     ForceSink fs{d.partials, d.counter, d.ring, d.state, 1};
     force_sample<2>(force, fs, tile, d.n_body_blocks);
@@ -147,17 +144,12 @@ int refresh(lbm_batch* b) {
     d.f[0] = static_cast<float*>(s->f[0]);
     d.f[1] = static_cast<float*>(s->f[1]);
     d.flags = s->flags;
-    d.sdf = s->sdf;
-    d.list = s->list;
     d.xtab = s->xtab;
-    d.gtab = s->gtab;
-    d.gmov = s->gmov;
+    d.lt = ListTables{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
     d.partials = s->partials;
     d.counter = s->counter;
     d.ring = s->ring;
     d.state = s->state;
-    d.n_link = s->n_link;
-    d.n_body = s->n_body;
     magic_for(uint32_t(s->W), &d.magic, &d.shift);
     d.force_every = s->cfg2.force_every;
     d.ramp_kind = b->ramp_kind;
@@ -171,14 +163,14 @@ int refresh(lbm_batch* b) {
   for (size_t i = 0; i < n; i++) {
     TunnelDesc& d = b->desc[i];
     d.link_block0 = blocks;
-    blocks += link_blocks(d.n_link);
+    blocks += link_blocks(d.lt.n_link);
   }
   if (blocks != b->link_end) drop_graphs(b);
   b->link_end = blocks;
   for (size_t i = 0; i < n; i++) {
     TunnelDesc& d = b->desc[i];
     d.body_block0 = blocks;
-    d.n_body_blocks = body_blocks(d.n_body, true);
+    d.n_body_blocks = body_blocks(d.lt.n_body, true);
     blocks += d.n_body_blocks;
   }
   if (blocks > 0x7fffffffu) return fail(LBM_ERR_INVALID, "batch too large for one launch");
@@ -188,7 +180,7 @@ int refresh(lbm_batch* b) {
   for (size_t i = 0; i < n; i++) {
     const TunnelDesc& d = b->desc[i];
     for (uint32_t k = d.bulk_block0; k < d.bulk_block0 + (b->t[i]->N + 255) / 256; k++) b->block_tunnel[k] = uint16_t(i);
-    for (uint32_t k = d.link_block0; k < d.link_block0 + link_blocks(d.n_link); k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.link_block0; k < d.link_block0 + link_blocks(d.lt.n_link); k++) b->block_tunnel[k] = uint16_t(i);
     for (uint32_t k = d.body_block0; k < d.body_block0 + d.n_body_blocks; k++) b->block_tunnel[k] = uint16_t(i);
   }
   if (blocks > b->block_cap) {

@@ -43,7 +43,8 @@ void free_all(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
                   (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state,
-                  (void*)s->xtab, (void*)s->gtab, (void*)s->gmov})
+                  (void*)s->xtab, (void*)s->gtab, (void*)s->bmask, (void*)s->xmask, (void*)s->lbase,
+                  (void*)s->links})
     if (p) (void)hipFree(p);
   for (auto& kv : s->tickets) {
     if (kv.second.ring) (void)hipHostFree(kv.second.ring);
@@ -116,11 +117,19 @@ void inlet_feq3(float u, float* out) {
   }
 }
 
-// Host resolution of one pull of a boundary node into a single gather index (population * N + cell),
-// following the rule order of pullSlow3 (src/shaders/step3d.ts:119-168). Returns false when the pull
-// is a body link (Bouzidi interpolation, or a slip face whose mirror node is solid), which stays on
-// the dynamic path because it also contributes force.
-bool resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g) {
+// Host resolution of one pull of a boundary node, following the rule order of pullSlow3
+// (src/shaders/step3d.ts:119-168) and pullSlow (src/shaders/step.ts:119-175). Every pull becomes a
+// gather index (population * N + cell); a Bouzidi pull also gets its record (q and the second
+// operand), a 3D slip face whose mirror node is solid is marked (bounce-back carrying 2 f_j of
+// force), and a 2D moving lid is marked (adds its lid term).
+enum PullKind { PULL_GATHER = 0, PULL_BOUZIDI = 1, PULL_MARKED = 2 };
+
+// q as the kernels computed it, in float: clamp(dF / max(dF - dS, 1e-6), 1e-3, 1).
+float link_q(float dF, float dS) {
+  return std::min(std::max(dF / std::max(dF - dS, 1e-6f), 1e-3f), 1.0f);
+}
+
+int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H, D = s->D;
   const uint32_t N = s->N;
   const uint32_t WH = uint32_t(W) * uint32_t(H);
@@ -131,7 +140,7 @@ bool resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint
   const uint32_t f = fl[idx];
   if ((f & (1u << i)) == 0u) {
     *g = uint32_t(i) * N + at(sx, sy, sz);
-    return true;
+    return PULL_GATHER;
   }
   const bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
   const bool outlet = (f & FLAG_OUTLET3) != 0u;
@@ -141,55 +150,82 @@ bool resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint
     *g = uint32_t(j) * N + idx;
   } else if (outY || outZ) {
     const uint32_t rs = outY ? at(sx, y, sz) : at(sx, sy, z);
-    if (fl[rs] & FLAG_SOLID3) return false;
+    if (fl[rs] & FLAG_SOLID3) {
+      *g = uint32_t(j) * N + idx;
+      return PULL_MARKED;
+    }
     *g = uint32_t(outY ? mirror_y3(i) : mirror_z3(i)) * N + rs;
   } else if (outX) {
     *g = uint32_t(j) * N + idx;
   } else {
     const uint32_t sc = at(sx, sy, sz);
-    if (fl[sc] & FLAG_SOLID3) return false;
-    *g = uint32_t(i) * N + sc;
+    if (!(fl[sc] & FLAG_SOLID3)) {
+      *g = uint32_t(i) * N + sc;
+      return PULL_GATHER;
+    }
+    *g = uint32_t(j) * N + idx;
+    rec->q = link_q(s->sdf_host[idx], s->sdf_host[sc]);
+    if (rec->q < 0.5f) {
+      const int nx = x + CX3[i], ny = y + CY3[i], nz = z + CZ3[i];
+      const bool ok = nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D && !(fl[at(nx, ny, nz)] & FLAG_SOLID3);
+      rec->gb = ok ? uint32_t(j) * N + at(nx, ny, nz) : NO_OPERAND;
+    } else {
+      rec->gb = uint32_t(i) * N + idx;
+    }
+    return PULL_BOUZIDI;
   }
-  return true;
+  return PULL_GATHER;
 }
 
-// The same for pullSlow (src/shaders/step.ts:119-175): periodic wrap, Bouzidi (a body link), slip
-// mirror in the same row, moving lid (a gather plus the additive term flagged in *moving),
-// bounce-back and the open-boundary placeholders.
-bool resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, bool* moving) {
+int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H;
   const uint32_t N = s->N;
   const Params2& p = s->p2;
   const int y = int(idx / uint32_t(W)), x = int(idx % uint32_t(W));
   const int j = OPP2[i];
   int sx = x - CX2[i], sy = y - CY2[i];
-  *moving = false;
   if ((fl[idx] & (1u << i)) == 0u) {
     *g = uint32_t(i) * N + uint32_t(sy * W + sx);
-    return true;
+    return PULL_GATHER;
   }
-  if (sx < 0 && p.left == 0) sx += W;
-  if (sx >= W && p.right == 0) sx -= W;
-  if (sy < 0 && p.bottom == 0) sy += H;
-  if (sy >= H && p.top == 0) sy -= H;
+  auto wrap = [&](int& qx, int& qy) {
+    if (qx < 0 && p.left == 0) qx += W;
+    if (qx >= W && p.right == 0) qx -= W;
+    if (qy < 0 && p.bottom == 0) qy += H;
+    if (qy >= H && p.top == 0) qy -= H;
+  };
+  wrap(sx, sy);
   const bool inX = sx >= 0 && sx < W, inY = sy >= 0 && sy < H;
   if (inX && inY) {
     const uint32_t sc = uint32_t(sy * W + sx);
-    if (fl[sc] & FLAG_SOLID2) return false;
-    *g = uint32_t(i) * N + sc;
-  } else if (!inY) {
+    if (!(fl[sc] & FLAG_SOLID2)) {
+      *g = uint32_t(i) * N + sc;
+      return PULL_GATHER;
+    }
+    *g = uint32_t(j) * N + idx;
+    rec->q = link_q(s->sdf_host[idx], s->sdf_host[sc]);
+    if (rec->q < 0.5f) {
+      int nx = x + CX2[i], ny = y + CY2[i];
+      wrap(nx, ny);
+      const bool ok = nx >= 0 && nx < W && ny >= 0 && ny < H && !(fl[uint32_t(ny * W + nx)] & FLAG_SOLID2);
+      rec->gb = ok ? uint32_t(j) * N + uint32_t(ny * W + nx) : NO_OPERAND;
+    } else {
+      rec->gb = uint32_t(i) * N + idx;
+    }
+    return PULL_BOUZIDI;
+  }
+  if (!inY) {
     const int mode = sy < 0 ? p.bottom : p.top;
     if (mode == 2) {
       const int rx = std::min(std::max(sx, 0), W - 1);
       *g = uint32_t(MIRROR_Y2[i]) * N + uint32_t(y) * uint32_t(W) + uint32_t(rx);
-    } else {
-      *g = uint32_t(j) * N + idx;
-      *moving = mode == 3;
+      return PULL_GATHER;
     }
-  } else {
     *g = uint32_t(j) * N + idx;
+    return mode == 3 ? PULL_MARKED : PULL_GATHER;
   }
-  return true;
+  *g = uint32_t(j) * N + idx;
+  return PULL_GATHER;
 }
 
 // The boundary list and its gather tables from the host copy of the flag words.
@@ -201,15 +237,16 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
   const uint32_t outlet_flag = d2 ? FLAG_OUTLET2 : FLAG_OUTLET3;
   const uint32_t plain_in = d2 ? (FLAG_INLET2 | INLET_DIRS2) : (FLAG_INLET3 | INLET_DIRS3);
   const uint32_t plain_out = d2 ? (FLAG_OUTLET2 | OUTLET_DIRS2) : (FLAG_OUTLET3 | OUTLET_DIRS3);
-  // Boundary list: nodes the bulk kernel skips. Link nodes (every pull resolves to one gather: faces,
-  // edges, walls, seams, open columns) come first, grouped inlet / outlet / other; body nodes (a
-  // Bouzidi link, or a 3D slip face mirroring a solid node) follow. All groups are in cell order.
+  // Boundary list: every non-solid node the bulk kernel skips. Nodes without body links come first,
+  // grouped inlet / outlet / other, then nodes with a body link (Bouzidi, or in 3D a slip face whose
+  // mirror node is solid); cell order inside each group. Every pull is resolved here once.
   const int Q = s->Q;
-  std::vector<uint32_t> link[3], body;
-  std::vector<uint32_t> gt;  // gather table, direction-major: gt[(i - 1) * n_link + k]
-  std::vector<uint32_t> gm;                  // 2D moving-lid mask per link node
+  std::vector<uint32_t> grp[3], body;
   s->n_bulk = 0;
   s->n_solid = 0;
+  auto resolve = [&](uint32_t idx, int i, uint32_t* g, BzLink* rec) {
+    return d2 ? resolve2(s, fl.data(), idx, i, g, rec) : resolve3(s, fl.data(), idx, i, g, rec);
+  };
   for (uint32_t k = 0; k < N; k++) {
     const uint32_t v = fl[k];
     if (v & solid) {
@@ -221,60 +258,77 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
       continue;
     }
     const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
-    bool is_body = false;
-    if (!(s->nd == 3 && kind == 0)) {
-      for (int i = 1; i < Q && !is_body; i++) {
-        uint32_t gi = 0;
-        bool mov = false;
-        is_body = s->nd == 2 ? !resolve2(s, fl.data(), k, i, &gi, &mov) : !resolve3(s, fl.data(), k, i, &gi);
+    bool has_body = false;
+    if (!(!d2 && kind == 0)) {  // the 3D inlet takes no pulls
+      for (int i = 1; i < Q && !has_body; i++) {
+        uint32_t gi;
+        BzLink rec;
+        const int pk = resolve(k, i, &gi, &rec);
+        has_body = pk == PULL_BOUZIDI || (!d2 && pk == PULL_MARKED);
       }
     }
-    if (is_body) body.push_back(k);
-    else link[kind].push_back(k);
+    if (has_body) body.push_back(k);
+    else grp[kind].push_back(k);
   }
   std::vector<uint32_t> lst;
-  for (auto& l : link) lst.insert(lst.end(), l.begin(), l.end());
+  for (auto& l : grp) lst.insert(lst.end(), l.begin(), l.end());
   const uint32_t n_link = uint32_t(lst.size());
   lst.insert(lst.end(), body.begin(), body.end());
-  gt.assign(size_t(Q - 1) * n_link, 0u);
-  if (s->nd == 2) gm.assign(n_link, 0u);
-  for (uint32_t k = 0; k < n_link; k++) {
+  const uint32_t nl = uint32_t(lst.size());
+  std::vector<uint32_t> gt(size_t(Q - 1) * nl, 0u), bm(nl, 0u), xm(nl, 0u), lb(nl, 0u);
+  std::vector<BzLink> recs;
+  for (uint32_t k = 0; k < nl; k++) {
     const uint32_t idx = lst[k];
-    if (s->nd == 3 && (fl[idx] & inlet_flag)) {
-      for (int i = 1; i < Q; i++) gt[size_t(i - 1) * n_link + k] = idx;  // the inlet takes no pulls
+    lb[k] = uint32_t(recs.size());
+    if (!d2 && (fl[idx] & inlet_flag)) {
+      for (int i = 1; i < Q; i++) gt[size_t(i - 1) * nl + k] = idx;  // never read
       continue;
     }
     for (int i = 1; i < Q; i++) {
       uint32_t gi = 0;
-      bool mov = false;
-      if (s->nd == 2) {
-        resolve2(s, fl.data(), idx, i, &gi, &mov);
-        if (mov) gm[k] |= 1u << i;
-      } else {
-        resolve3(s, fl.data(), idx, i, &gi);
+      BzLink rec{0.0f, NO_OPERAND};
+      const int pk = resolve(idx, i, &gi, &rec);
+      gt[size_t(i - 1) * nl + k] = gi;
+      if (pk == PULL_BOUZIDI) {
+        bm[k] |= 1u << i;
+        recs.push_back(rec);
+      } else if (pk == PULL_MARKED) {
+        xm[k] |= 1u << i;
       }
-      gt[size_t(i - 1) * n_link + k] = gi;
     }
   }
-  s->nlist = uint32_t(lst.size());
+  // Every index must address a population of the grid before anything is launched with it.
+  const uint64_t lim = uint64_t(Q) * N;
+  for (uint32_t v : gt)
+    if (v >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: gather index out of range");
+  for (const BzLink& r : recs)
+    if (r.gb != NO_OPERAND && r.gb >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: operand out of range");
+  s->nlist = nl;
   s->n_link = n_link;
-  s->n_body = s->nlist - n_link;
-  if (s->gtab) HIP_TRY(hipFree(s->gtab));
-  if (s->gmov) HIP_TRY(hipFree(s->gmov));
-  s->gtab = nullptr;
-  s->gmov = nullptr;
-  if (n_link) {
-    HIP_TRY(hipMalloc(&s->gtab, sizeof(uint32_t) * gt.size()));
-    HIP_TRY(hipMemcpyAsync(s->gtab, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
-    if (s->nd == 2) {
-      HIP_TRY(hipMalloc(&s->gmov, sizeof(uint32_t) * n_link));
-      HIP_TRY(hipMemcpyAsync(s->gmov, gm.data(), sizeof(uint32_t) * n_link, hipMemcpyHostToDevice, s->stream));
-    }
+  s->n_body = nl - n_link;
+  s->n_links = uint32_t(recs.size());
+  for (void** pp : {(void**)&s->gtab, (void**)&s->bmask, (void**)&s->xmask, (void**)&s->lbase, (void**)&s->links}) {
+    if (*pp) HIP_TRY(hipFree(*pp));
+    *pp = nullptr;
   }
-  if (s->nlist) HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * lst.size(), hipMemcpyHostToDevice, s->stream));
+  if (nl) {
+    HIP_TRY(hipMalloc(&s->gtab, sizeof(uint32_t) * gt.size()));
+    HIP_TRY(hipMalloc(&s->bmask, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&s->xmask, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&s->lbase, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&s->links, sizeof(BzLink) * std::max<size_t>(1, recs.size())));
+    HIP_TRY(hipMemcpyAsync(s->gtab, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->bmask, bm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->xmask, xm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->lbase, lb.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    if (!recs.empty())
+      HIP_TRY(hipMemcpyAsync(s->links, recs.data(), sizeof(BzLink) * recs.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+  }
   HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
+
 
 // rebuildFlags (src/solver.ts:205-217, src/solver3d.ts:216-228): macro on the current buffer, flags
 // and refill on the GPU, then the boundary list (and 2D force slots) from the flags, in cell order.
@@ -315,6 +369,7 @@ int rebuild_flags_impl(lbm_solver* s) {
   return build_links(s, fl);
 }
 
+
 int alloc_common(lbm_solver* s) {
   const size_t N = s->N;
   HIP_TRY(hipSetDevice(s->device));
@@ -344,6 +399,7 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMemsetAsync(s->state, 0, 4, s->stream));
   HIP_TRY(hipMemsetAsync(s->ring, 0, sizeof(double) * 3 * HISTORY_LEN, s->stream));
   std::vector<float> far(N, 1e6f);  // src/solver.ts:133, src/solver3d.ts:142 (FAR)
+  s->sdf_host = far;
   HIP_TRY(hipMemcpyAsync(s->sdf, far.data(), 4 * N, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
@@ -363,15 +419,15 @@ int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
   void* dst = s->f[s->parity ^ 1];
   if (s->nd == 2) {
     if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
-    const ListTables lt{s->list, s->gtab, s->gmov, s->n_link, s->n_body};
-    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, s->sdf, lt, s->p2, s->xtab, fs,
+    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, lt, s->p2, s->xtab, fs,
                  s->bulk_block, s->stream, mid);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];
     inlet_feq3(s->p3.uIn, fin);
-    const ListTables lt{s->list, s->gtab, s->gmov, s->n_link, s->n_body};
-    launch_step3(s->prec, src, dst, s->flags, s->sdf, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, mid);
+    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, mid);
   }
   s->parity ^= 1;
   s->step++;
@@ -563,6 +619,7 @@ int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps) {
 
 int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n) {
   if (!s || !sdf || n != s->N) return fail(LBM_ERR_INVALID, "sdf must hold N floats");
+  s->sdf_host.assign(sdf, sdf + s->N);
   HIP_TRY(hipMemcpyAsync(s->sdf, sdf, 4 * size_t(s->N), hipMemcpyHostToDevice, s->stream));
   return rebuild_flags_impl(s);
 }

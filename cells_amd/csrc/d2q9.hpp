@@ -3,7 +3,7 @@
 // the TRT collision and the per-direction boundary pulls. Restates src/shaders/step.ts:9-229 and
 // src/shaders/common.ts:84-102.
 #pragma once
-#include "lbm_device.hpp"
+#include "kernels.hpp"
 
 namespace lbm {
 
@@ -145,76 +145,6 @@ __device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, i
   }
 }
 
-// ============================================================================ boundary-list kernel
-// src/shaders/step.ts:119-175
-template <int i>
-__device__ __forceinline__ float pull_slow2(const float* __restrict__ src, const uint32_t* __restrict__ flags,
-                                            const float* __restrict__ sdf, int x, int y, uint32_t idx, uint32_t fl,
-                                            const Params2& p, float (&force)[2]) {
-  constexpr int j = OPP2[i];
-  const uint32_t N = p.N;
-  const int W = p.W, H = p.H;
-  int sx = x - CX2[i];
-  int sy = y - CY2[i];
-  if ((fl & (1u << i)) == 0u) return src[size_t(i) * N + uint32_t(sy * W + sx)];
-  auto wrap = [&](int& qx, int& qy) {
-    if (qx < 0 && p.left == 0) qx += W;
-    if (qx >= W && p.right == 0) qx -= W;
-    if (qy < 0 && p.bottom == 0) qy += H;
-    if (qy >= H && p.top == 0) qy -= H;
-  };
-  wrap(sx, sy);
-  const bool inX = sx >= 0 && sx < W;
-  const bool inY = sy >= 0 && sy < H;
-  if (inX && inY) {
-    const uint32_t s = uint32_t(sy * W + sx);
-    if ((flags[s] & FLAG_SOLID2) == 0u) return src[size_t(i) * N + s];
-    const float fj = src[size_t(j) * N + idx];
-    const float dF = sdf[idx];
-    const float dS = sdf[s];
-    const float q = wclamp(dF / fmaxf(dF - dS, 1e-6f), 1e-3f, 1.0f);
-    float fi = fj;
-    if (q < 0.5f) {
-      int nx = x + CX2[i], ny = y + CY2[i];
-      wrap(nx, ny);
-      if (nx >= 0 && nx < W && ny >= 0 && ny < H) {
-        const uint32_t n = uint32_t(ny * W + nx);
-        if ((flags[n] & FLAG_SOLID2) == 0u) fi = 2.0f * q * fj + (1.0f - 2.0f * q) * src[size_t(j) * N + n];
-      }
-    } else {
-      fi = fj / (2.0f * q) + (2.0f * q - 1.0f) / (2.0f * q) * src[size_t(i) * N + idx];
-    }
-    force[0] += float(CX2[j]) * (fj + fi);
-    force[1] += float(CY2[j]) * (fj + fi);
-    return fi;
-  }
-  if (!inY) {
-    const int mode = sy < 0 ? p.bottom : p.top;
-    if (mode == 2) {  // slip: mirror from the same row
-      const int rx = sx < 0 ? 0 : (sx > W - 1 ? W - 1 : sx);
-      return src[size_t(MIRROR_Y2[i]) * N + uint32_t(y) * uint32_t(W) + uint32_t(rx)];
-    }
-    if (mode == 3) return src[size_t(j) * N + idx] + 6.0f * W2[i] * float(CX2[i]) * p.uLid;  // moving wall
-    return src[size_t(j) * N + idx];
-  }
-  return src[size_t(j) * N + idx];  // walls, and open-boundary placeholders
-}
-
-template <int i>
-struct PullAll2 {
-  __device__ static __forceinline__ void run(const float* src, const uint32_t* flags, const float* sdf, int x, int y,
-                                             uint32_t idx, uint32_t fl, const Params2& p, float (&f)[9],
-                                             float (&force)[2]) {
-    f[i] = pull_slow2<i>(src, flags, sdf, x, y, idx, fl, p, force);
-    PullAll2<i + 1>::run(src, flags, sdf, x, y, idx, fl, p, f, force);
-  }
-};
-template <>
-struct PullAll2<9> {
-  __device__ static __forceinline__ void run(const float*, const uint32_t*, const float*, int, int, uint32_t, uint32_t,
-                                             const Params2&, float (&)[9], float (&)[2]) {}
-};
-
 // One plain fluid / plain Zou-He node of the bulk pass (src/shaders/step.ts:177-227 for flags that
 // are 0 or only the off-grid x bits of an open column). Per direction the source address is a
 // wave-uniform base (src + i*N - delta_i) plus the lane's byte offset.
@@ -254,21 +184,62 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   for (int i = 0; i < 9; i++) *reinterpret_cast<float*>(d + i * N * 4 + off) = out[i];
 }
 
-// One link node of the boundary list: every pull is a single gather through the precomputed table
-// (k is the node's position in the list), plus the moving-lid term where the table says so.
-__device__ __forceinline__ void link_cell2(const float* __restrict__ src, float* __restrict__ dst,
+// One node of the boundary list (k is its position in the list). Every pull is a gather through the
+// table the host resolved from pullSlow's rules (src/shaders/step.ts:119-175), so the 8 loads are
+// independent; a moving-lid pull adds 6 w_i c_ix uLid (step.ts:167). Nodes with body links (BODY)
+// then apply their Bouzidi links (step.ts:138-160): the gathered f_j becomes the interpolation
+// against the record's second operand, and c_j (f_j + f_i) goes into the force. Zou-He and the
+// collision follow as in the bulk pass.
+template <bool BODY>
+__device__ __forceinline__ void list_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
-                                           const uint32_t* __restrict__ gtab, const uint32_t* __restrict__ gmov,
-                                           uint32_t n_link, uint32_t k, uint32_t idx, const Params2& p) {
-  const uint32_t fl = flags[idx];
+                                           const ListTables& lt, uint32_t k, const Params2& p, float (&force)[2]) {
   const uint32_t N = p.N;
+  const uint32_t nl = lt.n_link + lt.n_body;
+  const uint32_t idx = lt.list[k];
+  const uint32_t fl = flags[idx];
   const int y = int(idx / uint32_t(p.W));
   const int x = int(idx - uint32_t(y) * uint32_t(p.W));
   float f[9];
   f[0] = src[idx];
 #pragma unroll
-  for (int i = 1; i < 9; i++) f[i] = src[gtab[size_t(i - 1) * n_link + k]];
-  const uint32_t mv = gmov[k];
+  for (int i = 1; i < 9; i++) f[i] = src[lt.gtab[size_t(i - 1) * nl + k]];
+  if (BODY) {
+    const uint32_t bm = lt.bmask[k];
+    const BzLink* __restrict__ rec = lt.links + lt.lbase[k];
+    float q[9], b[9];
+    uint32_t gb[9];
+#pragma unroll
+    for (int i = 1; i < 9; i++) {
+      q[i] = 1.0f;
+      gb[i] = NO_OPERAND;
+      if (bm & (1u << i)) {
+        const BzLink r = rec[__popc(bm & ((1u << i) - 1u))];
+        q[i] = r.q;
+        gb[i] = r.gb;
+      }
+    }
+#pragma unroll
+    for (int i = 1; i < 9; i++) b[i] = gb[i] != NO_OPERAND ? src[gb[i]] : 0.0f;
+#pragma unroll
+    for (int i = 1; i < 9; i++) {
+      if (bm & (1u << i)) {
+        const int j = OPP2[i];
+        const float fj = f[i];
+        const float qi = q[i];
+        float fi = fj;
+        if (qi < 0.5f) {
+          if (gb[i] != NO_OPERAND) fi = 2.0f * qi * fj + (1.0f - 2.0f * qi) * b[i];
+        } else {
+          fi = fj / (2.0f * qi) + (2.0f * qi - 1.0f) / (2.0f * qi) * b[i];
+        }
+        force[0] += float(CX2[j]) * (fj + fi);
+        force[1] += float(CY2[j]) * (fj + fi);
+        f[i] = fi;
+      }
+    }
+  }
+  const uint32_t mv = lt.xmask[k];
   if (mv) {
 #pragma unroll
     for (int i = 1; i < 9; i++)
@@ -284,26 +255,6 @@ __device__ __forceinline__ void link_cell2(const float* __restrict__ src, float*
   r.t0 = t.w;
   float out[9];
   collide2(f, r, y, p, out);
-#pragma unroll
-  for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
-}
-
-// One body node of the boundary list: the reference's slow path with every pullSlow rule, Zou-He and
-// the collision; accumulates the node's momentum-exchange force.
-__device__ __forceinline__ void list_cell2(const float* __restrict__ src, float* __restrict__ dst,
-                                           const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                           uint32_t idx, const Params2& p, float (&force)[2]) {
-  const uint32_t fl = flags[idx];
-  const uint32_t N = p.N;
-  const int y = int(idx / uint32_t(p.W));
-  const int x = int(idx - uint32_t(y) * uint32_t(p.W));
-  float f[9];
-  f[0] = src[idx];
-  PullAll2<1>::run(src, flags, sdf, x, y, idx, fl, p, f, force);
-  if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
-  else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
-  float out[9];
-  collide2(f, relax2(x, p), y, p, out);
 #pragma unroll
   for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
 }

@@ -7,21 +7,37 @@
 
 namespace lbm {
 
-// Boundary-list tables of one tunnel: list[0, n_link) are link nodes whose pulls are single gathers
-// gtab[(i - 1) * n_link + k] (plus, in 2D, a moving-lid term for the directions set in gmov[k]);
-// list[n_link, n_link + n_body) are body nodes, which run the dynamic pull rules and carry force.
+// A Bouzidi link of the boundary list: its fluid fraction q (src/shaders/step.ts:140-142, fixed per
+// geometry) and the gather index of the interpolation's second operand: f_j at x + c_i for q < 1/2
+// (NO_OPERAND when that node is off the grid or solid, which falls back to f_j), f_i at x for q >= 1/2.
+struct BzLink {
+  float q;
+  uint32_t gb;
+};
+constexpr uint32_t NO_OPERAND = 0xffffffffu;
+
+// Boundary-list tables of one tunnel. list[0, n_link) are nodes without body links, list[n_link,
+// n_link + n_body) nodes with at least one (they carry the force). For node k and direction i:
+//   gtab[(i - 1) * (n_link + n_body) + k]  gather index of the pull (population * N + cell), which for
+//                                          a Bouzidi or slip-through-solid direction is f_j at the node;
+//   bmask[k] bit i                         Bouzidi link, record links[lbase[k] + rank of i in bmask[k]];
+//   xmask[k] bit i                         3D: slip face mirroring a solid node (bounce-back, force 2 f_j);
+//                                          2D: moving lid (adds 6 w_i c_ix uLid).
 struct ListTables {
   const uint32_t* list;
   const uint32_t* gtab;
-  const uint32_t* gmov;
+  const uint32_t* bmask;
+  const uint32_t* xmask;
+  const uint32_t* lbase;
+  const BzLink* links;
   uint32_t n_link, n_body;
 };
 
 // ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 ----
 // One time step: the bulk kernel (plain fluid and plain inlet/outlet nodes, one thread per cell)
 // followed by the boundary-list kernel (every other non-solid node, plus the force sample).
-void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf,
-                  const ListTables& lt, const Params3& p, const float* feq_in, const float4* xtab,
+void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
+                  const Params3& p, const float* feq_in, const float4* xtab,
                   const ForceSink& fs, int bulk_block, hipStream_t st, hipEvent_t mid = nullptr);
 // Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
 void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
@@ -30,13 +46,13 @@ void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, 
 void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st);
 void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
                    const Params3& p, hipStream_t st);
-// Blocks of 256 for the link nodes and for the body nodes (at least one body block when a force
+// Blocks of 256 for the nodes without and with body links (at least one body block when a force
 // sample is due, so the sample is always written).
 uint32_t link_blocks(uint32_t n_link);
 uint32_t body_blocks(uint32_t n_body, bool sample);
 
 // ---- D2Q9 (FP32 storage) ----
-void launch_step2(const float* src, float* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
                   const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block, hipStream_t st,
                   hipEvent_t mid = nullptr);
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st);

@@ -29,20 +29,19 @@ __global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restric
   tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
 }
 
-// Blocks [0, nlb) take the link nodes (table gathers); the rest take the body nodes (dynamic
-// pullSlow with Bouzidi links) and carry the force sample.
+// Blocks [0, nlb) take the nodes without body links; the rest take the nodes with body links and
+// carry the force sample.
 __global__ __launch_bounds__(256) void step2_list(const float* __restrict__ src, float* __restrict__ dst,
-                                                  const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                                  ListTables lt, uint32_t nlb, Params2 p,
-                                                  const float4* __restrict__ xtab, ForceSink fs) {
+                                                  const uint32_t* __restrict__ flags, ListTables lt, uint32_t nlb,
+                                                  Params2 p, const float4* __restrict__ xtab, ForceSink fs) {
+  float force[2] = {0.0f, 0.0f};
   if (blockIdx.x < nlb) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < lt.n_link) link_cell2(src, dst, flags, xtab, lt.gtab, lt.gmov, lt.n_link, k, lt.list[k], p);
+    if (k < lt.n_link) list_cell2<false>(src, dst, flags, xtab, lt, k, p, force);
     return;
   }
   const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
-  float force[2] = {0.0f, 0.0f};
-  if (kb < lt.n_body) list_cell2(src, dst, flags, sdf, lt.list[lt.n_link + kb], p, force);
+  if (kb < lt.n_body) list_cell2<true>(src, dst, flags, xtab, lt, lt.n_link + kb, p, force);
   if (fs.sample) force_sample<2>(force, fs, blockIdx.x - nlb, gridDim.x - nlb);
 }
 
@@ -161,7 +160,7 @@ __global__ __launch_bounds__(256) void flags2_kernel(const float* __restrict__ s
 // ============================================================================ launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
-void launch_step2(const float* src, float* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
                   const Params2& p, const float4* xtab, const ForceSink& fs, int bb, hipStream_t st, hipEvent_t mid) {
   const uint32_t bx = bulk_block_x(p.W, bb);
   const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H));
@@ -169,7 +168,7 @@ void launch_step2(const float* src, float* dst, const uint32_t* flags, const flo
   if (mid) (void)hipEventRecord(mid, st);
   const uint32_t nlb = link_blocks(lt.n_link);
   const uint32_t nbb = body_blocks(lt.n_body, fs.sample != 0);
-  if (nlb + nbb) step2_list<<<nlb + nbb, 256, 0, st>>>(src, dst, flags, sdf, lt, nlb, p, xtab, fs);
+  if (nlb + nbb) step2_list<<<nlb + nbb, 256, 0, st>>>(src, dst, flags, lt, nlb, p, xtab, fs);
 }
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st) {
   xtab2_kernel<<<blocks_for(uint32_t(p.W), 256), 256, 0, st>>>(p, tab);

@@ -8,9 +8,10 @@
 //     in the first and last wave of each x row, so every wave streams fully coalesced loads and
 //     stores. Other nodes (solid, faces, body links) are skipped.
 //   * step3_list: one thread per node of a list built once per geometry change (every non-solid
-//     node the bulk kernel skips, grouped by kind, in cell order within a kind). It runs the
-//     reference's per-direction boundary rules (pullSlow3), and fuses the momentum-exchange force
-//     reduction in FP64 (reduce.hpp).
+//     node the bulk kernel skips, grouped by kind, in cell order within a kind). The reference's
+//     per-direction boundary rules (pullSlow3) are resolved on the host into gather indices and
+//     Bouzidi records, so the kernel issues independent loads only; it fuses the momentum-exchange
+//     force reduction in FP64 (reduce.hpp).
 // Storage is structure-of-arrays f[i * N + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16;
 // arithmetic is FP32 throughout, as in the reference.
 #include "kernels.hpp"
@@ -222,161 +223,105 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
 }
 
 // ============================================================================ boundary-list kernel
-template <class S>
-__device__ __forceinline__ bool solid3(const uint32_t* flags, uint32_t s) {
-  return (flags[s] & FLAG_SOLID3) != 0u;
-}
-
-// src/shaders/step3d.ts:119-168
-template <class S, int i>
-__device__ __forceinline__ float pull_slow3(const typename S::elem* __restrict__ src, const uint32_t* __restrict__ flags,
-                                            const float* __restrict__ sdf, int x, int y, int z, uint32_t idx,
-                                            uint32_t fl, bool outlet, const Params3& p, float (&force)[3]) {
-  constexpr int j = opp3(i);
+// One node of the boundary list (k is its position in the list). Every pull is a gather through the
+// table the host resolved from pullSlow3's rules (src/shaders/step3d.ts:119-168), so the 19 loads are
+// independent. Nodes with body links (BODY) then apply them: a Bouzidi link (step3d.ts:147-166)
+// replaces the gathered f_j with the interpolation against its record's second operand and adds
+// c_j (f_j + f_i) to the force; a slip face mirroring a solid node (step3d.ts:139-146) keeps f_j and
+// adds c_j 2 f_j.
+template <class S, bool BODY>
+__device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+                                           const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
+                                           const Params3& p, const FeqIn& fin, const float4* __restrict__ xtab,
+                                           float (&force)[3]) {
   const uint32_t N = p.N;
-  const int W = p.W, H = p.H, D = p.D;
-  const int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
-  auto at = [&](int ax, int ay, int az) { return uint32_t(ax + W * (ay + H * az)); };
-  const typename S::elem* srci = src + size_t(i) * N;
-  const typename S::elem* srcj = src + size_t(j) * N;
-  if ((fl & (1u << i)) == 0u) return S::load(srci, at(sx, sy, sz));
-  const bool outX = sx < 0 || sx >= W;
-  const bool outY = sy < 0 || sy >= H;
-  const bool outZ = sz < 0 || sz >= D;
-  if (outX && outlet) return S::load(srci, at(W - 2, y, z));
-  if (outY && outZ) return S::load(srcj, idx);
-  if (outY || outZ) {
-    int rx = sx, ry = sy, rz = sz;
-    int mdir;
-    if (outY) {
-      ry = y;
-      mdir = mirror_y3(i);
-    } else {
-      rz = z;
-      mdir = mirror_z3(i);
-    }
-    const uint32_t rs = at(rx, ry, rz);
-    if (solid3<S>(flags, rs)) {
-      const float fj = S::load(srcj, idx);
-      force[0] += float(CX3[j]) * (2.0f * fj);
-      force[1] += float(CY3[j]) * (2.0f * fj);
-      force[2] += float(CZ3[j]) * (2.0f * fj);
-      return fj;
-    }
-    return S::load(src + size_t(mdir) * N, rs);
-  }
-  if (outX) return S::load(srcj, idx);
-  const uint32_t s = at(sx, sy, sz);
-  if (!solid3<S>(flags, s)) return S::load(srci, s);
-  // Bouzidi linear interpolated bounce-back with q from the SDF.
-  const float fj = S::load(srcj, idx);
-  const float dF = sdf[idx];
-  const float dS = sdf[s];
-  const float q = wclamp(dF / fmaxf(dF - dS, 1e-6f), 1e-3f, 1.0f);
-  float fi = fj;
-  if (q < 0.5f) {
-    const int nx = x + CX3[i], ny = y + CY3[i], nz = z + CZ3[i];
-    if (nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D) {
-      const uint32_t n = at(nx, ny, nz);
-      if (!solid3<S>(flags, n)) fi = 2.0f * q * fj + (1.0f - 2.0f * q) * S::load(srcj, n);
-    }
-  } else {
-    fi = fj / (2.0f * q) + (2.0f * q - 1.0f) / (2.0f * q) * S::load(srci, idx);
-  }
-  force[0] += float(CX3[j]) * (fj + fi);
-  force[1] += float(CY3[j]) * (fj + fi);
-  force[2] += float(CZ3[j]) * (fj + fi);
-  return fi;
-}
-
-template <class S, int i>
-struct PullAll3 {
-  __device__ static __forceinline__ void run(const typename S::elem* src, const uint32_t* flags, const float* sdf, int x,
-                                             int y, int z, uint32_t idx, uint32_t fl, bool outlet, const Params3& p,
-                                             float (&f)[19], float (&force)[3]) {
-    f[i] = pull_slow3<S, i>(src, flags, sdf, x, y, z, idx, fl, outlet, p, force);
-    PullAll3<S, i + 1>::run(src, flags, sdf, x, y, z, idx, fl, outlet, p, f, force);
-  }
-};
-template <class S>
-struct PullAll3<S, 19> {
-  __device__ static __forceinline__ void run(const typename S::elem*, const uint32_t*, const float*, int, int, int,
-                                             uint32_t, uint32_t, bool, const Params3&, float (&)[19], float (&)[3]) {}
-};
-
-// Boundary-list kernel. Blocks [0, nlb) take the link nodes: their pulls are single gathers through
-// the precomputed table (the reference's pullSlow3 rule for that node and direction, resolved once
-// per geometry), so all 19 loads are independent. The remaining blocks take the body nodes, which
-// run pullSlow3 itself (Bouzidi interpolation, slip through a solid) and carry the force sample.
-template <class S>
-__global__ __launch_bounds__(256) void step3_list(const typename S::elem* __restrict__ src,
-                                                  typename S::elem* __restrict__ dst,
-                                                  const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                                  ListTables lt, uint32_t nlb, Params3 p, FeqIn fin,
-                                                  const float4* __restrict__ xtab, ForceSink fs) {
-  const uint32_t N = p.N;
-  if (blockIdx.x < nlb) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= lt.n_link) return;
-    const uint32_t idx = lt.list[k];
-    const uint32_t fl = flags[idx];
-    if (fl & FLAG_INLET3) {
+  const uint32_t nl = lt.n_link + lt.n_body;
+  const uint32_t idx = lt.list[k];
+  const uint32_t fl = flags[idx];
+  if (fl & FLAG_INLET3) {
 #pragma unroll
-      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
-      return;
-    }
-    float f[19];
-    f[0] = S::load(src, idx);
-#pragma unroll
-    for (int i = 1; i < 19; i++) f[i] = S::load(src, lt.gtab[size_t(i - 1) * lt.n_link + k]);
-    const Mom3 m = moments3(f);
-    if (fl & FLAG_OUTLET3) {
-#pragma unroll
-      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
-      return;
-    }
-    const uint32_t x = idx % uint32_t(p.W);
-    const float4 t = xtab[x];
-    Relax r;
-    r.omBase = t.x;
-    r.omM = t.y;
-    r.sigma = t.z;
-    r.t0 = t.w;
-    float out[19];
-    collide3(f, m, r, p, out);
-#pragma unroll
-    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
     return;
   }
-  const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
-  float force[3] = {0.0f, 0.0f, 0.0f};
-  if (kb < lt.n_body) {
-    const uint32_t idx = lt.list[lt.n_link + kb];
-    const uint32_t fl = flags[idx];
-    if (fl & FLAG_INLET3) {
+  float f[19];
+  f[0] = S::load(src, idx);
 #pragma unroll
-      for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
-    } else {
-      const bool outlet = (fl & FLAG_OUTLET3) != 0u;
-      const int z = int(idx / p.WH);
-      const uint32_t rem = idx - uint32_t(z) * p.WH;
-      const int y = int(rem / uint32_t(p.W));
-      const int x = int(rem - uint32_t(y) * uint32_t(p.W));
-      float f[19];
-      f[0] = S::load(src, idx);
-      PullAll3<S, 1>::run(src, flags, sdf, x, y, z, idx, fl, outlet, p, f, force);
-      const Mom3 m = moments3(f);
-      if (outlet) {
+  for (int i = 1; i < 19; i++) f[i] = S::load(src, lt.gtab[size_t(i - 1) * nl + k]);
+  if (BODY) {
+    const uint32_t bm = lt.bmask[k];
+    const uint32_t xm = lt.xmask[k];
+    const BzLink* __restrict__ rec = lt.links + lt.lbase[k];
+    float q[19], b[19];
+    uint32_t gb[19];
 #pragma unroll
-        for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
-      } else {
-        float out[19];
-        collide3(f, m, relax3(float(x), p), p, out);
+    for (int i = 1; i < 19; i++) {
+      q[i] = 1.0f;
+      gb[i] = NO_OPERAND;
+      if (bm & (1u << i)) {
+        const BzLink r = rec[__popc(bm & ((1u << i) - 1u))];
+        q[i] = r.q;
+        gb[i] = r.gb;
+      }
+    }
 #pragma unroll
-        for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+    for (int i = 1; i < 19; i++) b[i] = gb[i] != NO_OPERAND ? S::load(src, gb[i]) : 0.0f;
+#pragma unroll
+    for (int i = 1; i < 19; i++) {
+      const int j = opp3(i);
+      const float fj = f[i];
+      if (bm & (1u << i)) {
+        const float qi = q[i];
+        float fi = fj;
+        if (qi < 0.5f) {
+          if (gb[i] != NO_OPERAND) fi = 2.0f * qi * fj + (1.0f - 2.0f * qi) * b[i];
+        } else {
+          fi = fj / (2.0f * qi) + (2.0f * qi - 1.0f) / (2.0f * qi) * b[i];
+        }
+        force[0] += float(CX3[j]) * (fj + fi);
+        force[1] += float(CY3[j]) * (fj + fi);
+        force[2] += float(CZ3[j]) * (fj + fi);
+        f[i] = fi;
+      } else if (xm & (1u << i)) {
+        force[0] += float(CX3[j]) * (2.0f * fj);
+        force[1] += float(CY3[j]) * (2.0f * fj);
+        force[2] += float(CZ3[j]) * (2.0f * fj);
       }
     }
   }
+  const Mom3 m = moments3(f);
+  if (fl & FLAG_OUTLET3) {
+#pragma unroll
+    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+    return;
+  }
+  const float4 t = xtab[idx % uint32_t(p.W)];
+  Relax r;
+  r.omBase = t.x;
+  r.omM = t.y;
+  r.sigma = t.z;
+  r.t0 = t.w;
+  float out[19];
+  collide3(f, m, r, p, out);
+#pragma unroll
+  for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+}
+
+// Boundary-list kernel. Blocks [0, nlb) take the nodes without body links; the remaining blocks take
+// the nodes with body links and carry the force sample (an FP64 reduction over those blocks).
+template <class S>
+__global__ __launch_bounds__(256) void step3_list(const typename S::elem* __restrict__ src,
+                                                  typename S::elem* __restrict__ dst,
+                                                  const uint32_t* __restrict__ flags, ListTables lt, uint32_t nlb,
+                                                  Params3 p, FeqIn fin, const float4* __restrict__ xtab,
+                                                  ForceSink fs) {
+  float force[3] = {0.0f, 0.0f, 0.0f};
+  if (blockIdx.x < nlb) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < lt.n_link) list_node3<S, false>(src, dst, flags, lt, k, p, fin, xtab, force);
+    return;
+  }
+  const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
+  if (kb < lt.n_body) list_node3<S, true>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, force);
   if (fs.sample) force_sample<3>(force, fs, blockIdx.x - nlb, gridDim.x - nlb);
 }
 
@@ -508,7 +453,7 @@ uint32_t bulk_block_x(int W, int requested) {
 }
 
 template <class S>
-static void step3_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                     const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bb,
                     hipStream_t st, hipEvent_t mid) {
   using E = typename S::elem;
@@ -521,17 +466,17 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const flo
   const uint32_t nlb = link_blocks(lt.n_link);
   const uint32_t nbb = body_blocks(lt.n_body, fs.sample != 0);
   if (nlb + nbb)
-    step3_list<S><<<nlb + nbb, 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, sdf, lt, nlb, p,
-                                              fin, xtab, fs);
+    step3_list<S><<<nlb + nbb, 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, lt, nlb, p, fin,
+                                              xtab, fs);
 }
 
-void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const ListTables& lt,
+void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
                   hipStream_t st, hipEvent_t mid) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    default: step3_t<StoreF32>(src, dst, flags, sdf, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
   }
 }
 

@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/lbm.h"
 #include "kernels.hpp"
@@ -71,9 +72,14 @@ struct lbm_solver {
   float4* mac = nullptr;
   uint32_t* list = nullptr;
   uint32_t nlist = 0;
-  uint32_t n_link = 0, n_body = 0;  // the list holds n_link gather nodes, then n_body body-link nodes
-  uint32_t* gtab = nullptr;         // gather table of the link nodes, direction-major
-  uint32_t* gmov = nullptr;         // 2D: moving-lid mask of the link nodes
+  uint32_t n_link = 0, n_body = 0;  // the list holds n_link nodes without body links, then n_body with
+  uint32_t* gtab = nullptr;         // gather table of every list node, direction-major
+  uint32_t* bmask = nullptr;        // Bouzidi directions per list node
+  uint32_t* xmask = nullptr;        // slip-through-solid (3D) / moving-lid (2D) directions per list node
+  uint32_t* lbase = nullptr;        // first Bouzidi record of each list node
+  lbm::BzLink* links = nullptr;     // Bouzidi records
+  uint32_t n_links = 0;
+  std::vector<float> sdf_host;      // the SDF as last set (q is computed from it on the host)
   int64_t n_bulk = 0, n_solid = 0;
   double* partials = nullptr;
   uint32_t* counter = nullptr;
