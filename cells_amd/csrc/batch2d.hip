@@ -71,19 +71,19 @@ __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict_
     if (idx >= p.N) return;
     const uint32_t y = div_w(idx, d.magic, d.shift);
     const uint32_t x = idx - y * uint32_t(p.W);
-    bulk_cell2(src, dst, d.flags, d.xtab, p, x, int(y), idx);
+    bulk_cell2<false>(src, dst, d.flags, d.xtab, p, x, int(y), idx, Record{nullptr, nullptr});
     return;
   }
   if (b < link_end) {
     const uint32_t kl = (b - d.link_block0) * 256u + threadIdx.x;
     float none[2] = {0.0f, 0.0f};
-    if (kl < d.lt.n_link) list_cell2<false>(src, dst, d.flags, d.xtab, d.lt, kl, p, none);
+    if (kl < d.lt.n_link) list_cell2<false, false>(src, dst, d.flags, d.xtab, d.lt, kl, p, Record{nullptr, nullptr}, none);
     return;
   }
   const uint32_t tile = b - d.body_block0;
   const uint32_t kk = tile * 256u + threadIdx.x;
   float force[2] = {0.0f, 0.0f};
-  if (kk < d.lt.n_body) list_cell2<true>(src, dst, d.flags, d.xtab, d.lt, d.lt.n_link + kk, p, force);
+  if (kk < d.lt.n_body) list_cell2<true, false>(src, dst, d.flags, d.xtab, d.lt, d.lt.n_link + kk, p, Record{nullptr, nullptr}, force);
   if (d.force_every > 0 && (step + 1) % uint64_t(d.force_every) == 0) {
     ForceSink fs{d.partials, d.counter, d.ring, d.state, 1};
     force_sample<2>(force, fs, tile, d.n_body_blocks);
@@ -163,14 +163,14 @@ int refresh(lbm_batch* b) {
   for (size_t i = 0; i < n; i++) {
     TunnelDesc& d = b->desc[i];
     d.link_block0 = blocks;
-    blocks += link_blocks(d.lt.n_link);
+    blocks += list_blocks(d.lt.n_link, 256);
   }
   if (blocks != b->link_end) drop_graphs(b);
   b->link_end = blocks;
   for (size_t i = 0; i < n; i++) {
     TunnelDesc& d = b->desc[i];
     d.body_block0 = blocks;
-    d.n_body_blocks = body_blocks(d.lt.n_body, true);
+    d.n_body_blocks = body_list_blocks(d.lt.n_body, 256, true);
     blocks += d.n_body_blocks;
   }
   if (blocks > 0x7fffffffu) return fail(LBM_ERR_INVALID, "batch too large for one launch");
@@ -180,7 +180,7 @@ int refresh(lbm_batch* b) {
   for (size_t i = 0; i < n; i++) {
     const TunnelDesc& d = b->desc[i];
     for (uint32_t k = d.bulk_block0; k < d.bulk_block0 + (b->t[i]->N + 255) / 256; k++) b->block_tunnel[k] = uint16_t(i);
-    for (uint32_t k = d.link_block0; k < d.link_block0 + link_blocks(d.lt.n_link); k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.link_block0; k < d.link_block0 + list_blocks(d.lt.n_link, 256); k++) b->block_tunnel[k] = uint16_t(i);
     for (uint32_t k = d.body_block0; k < d.body_block0 + d.n_body_blocks; k++) b->block_tunnel[k] = uint16_t(i);
   }
   if (blocks > b->block_cap) {

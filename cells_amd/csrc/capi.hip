@@ -33,6 +33,42 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+Row3 row3(const lbm_solver* s, uint32_t y, uint32_t z) {
+  Row3 r{y == 0u, y == uint32_t(s->H - 1), z == 0u, z == uint32_t(s->D - 1), 0u};
+  for (int i = 1; i < 19; i++) {
+    const bool outY = (r.loY && CY3[i] > 0) || (r.hiY && CY3[i] < 0);
+    const bool outZ = (r.loZ && CZ3[i] > 0) || (r.hiZ && CZ3[i] < 0);
+    if (outY || outZ) r.mask |= 1u << i;
+  }
+  return r;
+}
+int bulk_kind3(const Row3& r, uint32_t v) {
+  const bool face = r.loY || r.hiY || r.loZ || r.hiZ;
+  if (v == (FLAG_INLET3 | INLET_DIRS3 | r.mask)) return 2;
+  if (v == r.mask || (!face && v == (FLAG_OUTLET3 | OUTLET_DIRS3))) return 1;
+  return 0;
+}
+uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outlet, int i) {
+  const int64_t W = s->W, WH = int64_t(s->W) * s->H;
+  const int64_t delta = CZ3[i] * WH + CY3[i] * W + CX3[i];
+  const bool outY = (r.loY && CY3[i] > 0) || (r.hiY && CY3[i] < 0);
+  const bool outZ = (r.loZ && CZ3[i] > 0) || (r.hiZ && CZ3[i] < 0);
+  if (outlet && CX3[i] < 0) return uint32_t(i) * s->N + idx - 1u;
+  int sd = i;
+  int64_t sdelta = delta;
+  if (outY && outZ) {
+    sd = opp3(i);
+    sdelta = 0;
+  } else if (outY) {
+    sd = mirror_y3(i);
+    sdelta = CZ3[i] * WH + CX3[i];
+  } else if (outZ) {
+    sd = mirror_z3(i);
+    sdelta = CY3[i] * W + CX3[i];
+  }
+  return uint32_t(int64_t(sd) * s->N + int64_t(idx) - sdelta);
+}
+
 }  // namespace lbm
 
 namespace {
@@ -41,6 +77,7 @@ constexpr double MAGIC_LAMBDA = 3.0 / 16.0;
 
 void free_all(lbm_solver* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  adjoint_free(s);
   for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
                   (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state,
                   (void*)s->xtab, (void*)s->gtab, (void*)s->bmask, (void*)s->xmask, (void*)s->lbase,
@@ -229,7 +266,7 @@ int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
 }
 
 // The boundary list and its gather tables from the host copy of the flag words.
-int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
+int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
   const uint32_t N = s->N;
   const bool d2 = s->nd == 2;
   const uint32_t solid = d2 ? FLAG_SOLID2 : FLAG_SOLID3;
@@ -242,6 +279,7 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
   // mirror node is solid); cell order inside each group. Every pull is resolved here once.
   const int Q = s->Q;
   std::vector<uint32_t> grp[3], body;
+  bool list_marks = false;
   s->n_bulk = 0;
   s->n_solid = 0;
   auto resolve = [&](uint32_t idx, int i, uint32_t* g, BzLink* rec) {
@@ -253,9 +291,31 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
       s->n_solid++;
       continue;
     }
-    if (v == 0 || v == plain_in || v == plain_out) {
-      s->n_bulk++;
-      continue;
+    if (d2) {
+      if (v == 0 || v == plain_in || v == plain_out) {
+        s->n_bulk++;
+        continue;
+      }
+    } else {
+      // A node the bulk kernel would take stays there only if every pull it makes under its row's
+      // pattern is the one the reference's rules give; otherwise it is marked for the list.
+      const uint32_t WH = uint32_t(s->W) * uint32_t(s->H);
+      const Row3 r = row3(s, (k % WH) / uint32_t(s->W), k / WH);
+      const int bk = bulk_kind3(r, v);
+      bool ok = bk != 0;
+      for (int i = 1; i < Q && bk == 1 && ok; i++) {
+        uint32_t gi;
+        BzLink rec;
+        ok = resolve(k, i, &gi, &rec) == PULL_GATHER && gi == row_gather3(s, r, k, (v & outlet_flag) != 0u, i);
+      }
+      if (ok) {
+        s->n_bulk++;
+        continue;
+      }
+      if (bk != 0) {
+        fl[k] |= FLAG_LIST3;
+        list_marks = true;
+      }
     }
     const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
     bool has_body = false;
@@ -277,6 +337,8 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
   const uint32_t nl = uint32_t(lst.size());
   std::vector<uint32_t> gt(size_t(Q - 1) * nl, 0u), bm(nl, 0u), xm(nl, 0u), lb(nl, 0u);
   std::vector<BzLink> recs;
+  std::vector<uint32_t> rec_node;
+  std::vector<uint8_t> rec_dir;
   for (uint32_t k = 0; k < nl; k++) {
     const uint32_t idx = lst[k];
     lb[k] = uint32_t(recs.size());
@@ -292,6 +354,8 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
       if (pk == PULL_BOUZIDI) {
         bm[k] |= 1u << i;
         recs.push_back(rec);
+        rec_node.push_back(idx);
+        rec_dir.push_back(uint8_t(i));
       } else if (pk == PULL_MARKED) {
         xm[k] |= 1u << i;
       }
@@ -303,6 +367,16 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
     if (v >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: gather index out of range");
   for (const BzLink& r : recs)
     if (r.gb != NO_OPERAND && r.gb >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: operand out of range");
+  s->h_flags = fl;
+  s->h_list = lst;
+  s->h_gtab = gt;
+  s->h_bmask = bm;
+  s->h_xmask = xm;
+  s->h_lbase = lb;
+  s->h_links = recs;
+  s->h_link_node = rec_node;
+  s->h_link_dir = rec_dir;
+  s->tables_gen++;
   s->nlist = nl;
   s->n_link = n_link;
   s->n_body = nl - n_link;
@@ -325,6 +399,7 @@ int build_links(lbm_solver* s, const std::vector<uint32_t>& fl) {
       HIP_TRY(hipMemcpyAsync(s->links, recs.data(), sizeof(BzLink) * recs.size(), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
   }
+  if (list_marks) HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
@@ -385,8 +460,8 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMalloc(&s->sdf, 4 * N));
   HIP_TRY(hipMalloc(&s->mac, 16 * N));
   HIP_TRY(hipMalloc(&s->list, 4 * N));
-  const size_t maxblocks = (N + 255) / 256 + 1;
-  HIP_TRY(hipMalloc(&s->partials, sizeof(double) * 3 * maxblocks));
+  const size_t maxwaves = (N + 63) / 64 + 16;  // one force partial per wave of body-link nodes
+  HIP_TRY(hipMalloc(&s->partials, sizeof(double) * 3 * maxwaves));
   HIP_TRY(hipMalloc(&s->counter, 4));
   HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
   HIP_TRY(hipMalloc(&s->state, 4));
@@ -411,35 +486,37 @@ int bulk_block_for(int wg) {
   return wg;
 }
 
-int enqueue_one(lbm_solver* s, hipEvent_t mid = nullptr) {
-  const int every = s->force_every();
-  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0};
-  fs.sample = (every > 0 && (s->step + 1) % every == 0) ? 1 : 0;
-  void* src = s->f[s->parity];
-  void* dst = s->f[s->parity ^ 1];
-  if (s->nd == 2) {
-    if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
-    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
-    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, lt, s->p2, s->xtab, fs,
-                 s->bulk_block, s->stream, mid);
-  } else {
-    if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
-    float fin[19];
-    inlet_feq3(s->p3.uIn, fin);
-    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
-    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, mid);
-  }
-  s->parity ^= 1;
-  s->step++;
-  if (fs.sample) s->force_steps.push_back(s->step);
-  return LBM_OK;
-}
+int enqueue_one(lbm_solver* s) { return enqueue_step(s, nullptr, -1); }
 
 }  // namespace
 
 namespace lbm {
 
 int rebuild_flags(lbm_solver* s) { return rebuild_flags_impl(s); }
+
+int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
+  const int every = s->force_every();
+  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0};
+  fs.sample = sample >= 0 ? sample : ((every > 0 && (s->step + 1) % every == 0) ? 1 : 0);
+  void* src = s->f[s->parity];
+  void* dst = s->f[s->parity ^ 1];
+  if (s->nd == 2) {
+    if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
+    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, lt, s->p2, s->xtab, fs,
+                 s->bulk_block, s->stream, rec);
+  } else {
+    if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
+    float fin[19];
+    inlet_feq3(s->p3.uIn, fin);
+    const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, rec);
+  }
+  s->parity ^= 1;
+  s->step++;
+  if (fs.sample) s->force_steps.push_back(s->step);
+  return LBM_OK;
+}
 
 // Advances the host bookkeeping of a tunnel by n steps that a batch launch performed.
 void record_steps(lbm_solver* s, int64_t n) {
@@ -813,6 +890,8 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n) {
   if (!s || !out || n < s->N) return fail(LBM_ERR_INVALID, "out must hold N words");
   HIP_TRY(hipMemcpyAsync(out, s->flags, 4 * size_t(s->N), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->nd == 3)
+    for (uint32_t k = 0; k < s->N; k++) out[k] &= ~FLAG_LIST3;
   return LBM_OK;
 }
 
@@ -834,31 +913,30 @@ int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms) {
   return LBM_OK;
 }
 
-int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_ms, double* list_ms) {
-  if (!s || n < 0 || !total_ms || !bulk_ms || !list_ms) return fail(LBM_ERR_INVALID, "bad arguments");
-  std::vector<hipEvent_t> ev(size_t(2 * n + 1));
+int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms) {
+  if (!s || n < 0 || !total_ms || !step_ms || !max_step_ms) return fail(LBM_ERR_INVALID, "bad arguments");
+  std::vector<hipEvent_t> ev(size_t(n + 1));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], s->stream));
   for (int64_t k = 0; k < n; k++) {
-    enqueue_one(s, ev[size_t(2 * k + 1)]);
-    HIP_TRY(hipEventRecord(ev[size_t(2 * k + 2)], s->stream));
+    enqueue_one(s);
+    HIP_TRY(hipEventRecord(ev[size_t(k + 1)], s->stream));
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventSynchronize(ev.back()));
-  double tb = 0, tl = 0;
+  double sum = 0, mx = 0;
   for (int64_t k = 0; k < n; k++) {
-    float a = 0, b = 0;
-    HIP_TRY(hipEventElapsedTime(&a, ev[size_t(2 * k)], ev[size_t(2 * k + 1)]));
-    HIP_TRY(hipEventElapsedTime(&b, ev[size_t(2 * k + 1)], ev[size_t(2 * k + 2)]));
-    tb += a;
-    tl += b;
+    float a = 0;
+    HIP_TRY(hipEventElapsedTime(&a, ev[size_t(k)], ev[size_t(k + 1)]));
+    sum += a;
+    mx = std::max(mx, double(a));
   }
   float tot = 0;
   HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev.back()));
   for (auto& e : ev) (void)hipEventDestroy(e);
   *total_ms = tot;
-  *bulk_ms = tb;
-  *list_ms = tl;
+  *step_ms = sum;
+  *max_step_ms = mx;
   return LBM_OK;
 }
 

@@ -36,6 +36,15 @@ __device__ __forceinline__ void zou_he_outlet(float (&f)[9]) {
   f[6] = f[8] - 0.5f * (f[2] - f[4]) - 0.16666666666666666f * u;
 }
 
+// The adjoint's per-cell record: moments of the populations the collision takes (collide2's drho,
+// ux, uy without the body-force shift, which the adjoint does not support).
+__device__ __forceinline__ void record2(const float (&f)[9], const Record& rc, uint32_t idx) {
+  const float drho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+  const float invRho = 1.0f / (1.0f + drho);
+  rc.mom[idx] = make_float4(drho, (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) * invRho,
+                            (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]) * invRho, 0.0f);
+}
+
 // Relaxation coefficients that depend only on x: omega_base (sponge ramp of tau), omega_minus for
 // the no-Smagorinsky case, the absorbing-layer strength sigma (outlet sponge and inlet layer) and
 // 1 / omega_base (src/shaders/step.ts:215-223, 48-63).
@@ -148,9 +157,10 @@ __device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, i
 // One plain fluid / plain Zou-He node of the bulk pass (src/shaders/step.ts:177-227 for flags that
 // are 0 or only the off-grid x bits of an open column). Per direction the source address is a
 // wave-uniform base (src + i*N - delta_i) plus the lane's byte offset.
+template <bool REC>
 __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
-                                           const Params2& p, uint32_t x, int y, uint32_t idx) {
+                                           const Params2& p, uint32_t x, int y, uint32_t idx, const Record& rc) {
   const uint32_t fl = flags[idx];
   const bool inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
   const bool outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
@@ -171,6 +181,7 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   }
   if (inlet) zou_he_inlet(f, y, p);
   if (outlet) zou_he_outlet(f);
+  if (REC) record2(f, rc, idx);
   const float4 t = xtab[x];
   Relax2 r;
   r.omBase = t.x;
@@ -190,10 +201,11 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
 // then apply their Bouzidi links (step.ts:138-160): the gathered f_j becomes the interpolation
 // against the record's second operand, and c_j (f_j + f_i) goes into the force. Zou-He and the
 // collision follow as in the bulk pass.
-template <bool BODY>
+template <bool BODY, bool REC>
 __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
-                                           const ListTables& lt, uint32_t k, const Params2& p, float (&force)[2]) {
+                                           const ListTables& lt, uint32_t k, const Params2& p, const Record& rc,
+                                           float (&force)[2]) {
   const uint32_t N = p.N;
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
@@ -235,6 +247,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
         }
         force[0] += float(CX2[j]) * (fj + fi);
         force[1] += float(CY2[j]) * (fj + fi);
+        if (REC) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[i], gb[i] != NO_OPERAND);
         f[i] = fi;
       }
     }
@@ -247,6 +260,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
   }
   if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
   else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
+  if (REC) record2(f, rc, idx);
   const float4 t = xtab[x];
   Relax2 r;
   r.omBase = t.x;

@@ -33,12 +33,20 @@ struct ListTables {
   uint32_t n_link, n_body;
 };
 
+// What a recording step writes for the discrete adjoint: per cell, the moments (drho, ux, uy, uz) of
+// the populations the collision takes (after the boundary rules); per Bouzidi link (record order of
+// ListTables::links), the derivative of the interpolated population with respect to q.
+struct Record {
+  float4* mom;
+  float* dq;
+};
+
 // ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 ----
-// One time step: the bulk kernel (plain fluid and plain inlet/outlet nodes, one thread per cell)
-// followed by the boundary-list kernel (every other non-solid node, plus the force sample).
+// One time step in one launch: bulk rows (plain fluid, slip-face rows and plain inlet/outlet nodes,
+// one thread per cell) and boundary-list blocks (every other non-solid node, plus the force sample).
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
-                  const Params3& p, const float* feq_in, const float4* xtab,
-                  const ForceSink& fs, int bulk_block, hipStream_t st, hipEvent_t mid = nullptr);
+                  const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
+                  hipStream_t st, const Record* rec = nullptr);
 // Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
 void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
 uint32_t bulk_block_x(int W, int requested);
@@ -46,15 +54,15 @@ void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, 
 void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st);
 void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
                    const Params3& p, hipStream_t st);
-// Blocks of 256 for the nodes without and with body links (at least one body block when a force
-// sample is due, so the sample is always written).
-uint32_t link_blocks(uint32_t n_link);
-uint32_t body_blocks(uint32_t n_body, bool sample);
+// Blocks of bx threads for the list nodes without and with body links (at least one body block when
+// a force sample is due, so the sample is always written).
+uint32_t list_blocks(uint32_t n, uint32_t bx);
+uint32_t body_list_blocks(uint32_t n_body, uint32_t bx, bool sample);
 
 // ---- D2Q9 (FP32 storage) ----
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
                   const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block, hipStream_t st,
-                  hipEvent_t mid = nullptr);
+                  const Record* rec = nullptr);
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st);
 void launch_macro2(const float* f, const uint32_t* flags, float4* mac, const Params2& p, hipStream_t st);
 void launch_init2(float* fa, float* fb, const float4* mac, const Params2& p, hipStream_t st);

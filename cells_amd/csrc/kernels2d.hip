@@ -11,15 +11,37 @@
 
 namespace lbm {
 
-// ============================================================================ bulk kernel
-// Grid (ceil(W / blockDim.x), H): x and y come from the launch coordinates; per direction the source
-// address is a wave-uniform base (src + i*N - delta_i) plus the lane's byte offset.
-__global__ __launch_bounds__(1024) void step2_bulk(const float* __restrict__ src, float* __restrict__ dst,
-                                                   const uint32_t* __restrict__ flags,
-                                                   const float4* __restrict__ xtab, Params2 p) {
+// ============================================================================ step kernel
+// One time step in one launch. Grid (ceil(W / blockDim.x), yl + H): the first yl rows of blocks are
+// boundary-list blocks (dispatched first, so their dependent gathers overlap the streaming): list
+// blocks [0, nlb) take the nodes without body links, the nbb after them the nodes with body links,
+// which carry the force sample. The remaining rows are the bulk rows, where x and y come from the
+// launch coordinates and per direction the source address is a wave-uniform base
+// (src + i*N - delta_i) plus the lane's byte offset.
+template <bool REC>
+__global__ __launch_bounds__(1024) void step2_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                                     const uint32_t* __restrict__ flags,
+                                                     const float4* __restrict__ xtab, Params2 p, ListTables lt,
+                                                     uint32_t yl, uint32_t nlb, uint32_t nbb, ForceSink fs,
+                                                     Record rc) {
+  if (blockIdx.y < yl) {
+    const uint32_t lb = blockIdx.y * gridDim.x + blockIdx.x;
+    float force[2] = {0.0f, 0.0f};
+    if (lb < nlb) {
+      const uint32_t k = lb * blockDim.x + threadIdx.x;
+      if (k < lt.n_link) list_cell2<false, REC>(src, dst, flags, xtab, lt, k, p, rc, force);
+      return;
+    }
+    if (lb >= nlb + nbb) return;
+    const uint32_t kb = (lb - nlb) * blockDim.x + threadIdx.x;
+    if (kb < lt.n_body) list_cell2<true, REC>(src, dst, flags, xtab, lt, lt.n_link + kb, p, rc, force);
+    if (fs.sample) force_sample<2>(force, fs, lb - nlb, nbb);
+    return;
+  }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  bulk_cell2(src, dst, flags, xtab, p, x, int(blockIdx.y), x + uint32_t(p.W) * blockIdx.y);
+  const uint32_t y = blockIdx.y - yl;
+  bulk_cell2<REC>(src, dst, flags, xtab, p, x, int(y), x + uint32_t(p.W) * y, rc);
 }
 
 __global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restrict__ tab) {
@@ -27,22 +49,6 @@ __global__ __launch_bounds__(256) void xtab2_kernel(Params2 p, float4* __restric
   if (x >= p.W) return;
   const Relax2 r = relax2(x, p);
   tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
-}
-
-// Blocks [0, nlb) take the nodes without body links; the rest take the nodes with body links and
-// carry the force sample.
-__global__ __launch_bounds__(256) void step2_list(const float* __restrict__ src, float* __restrict__ dst,
-                                                  const uint32_t* __restrict__ flags, ListTables lt, uint32_t nlb,
-                                                  Params2 p, const float4* __restrict__ xtab, ForceSink fs) {
-  float force[2] = {0.0f, 0.0f};
-  if (blockIdx.x < nlb) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < lt.n_link) list_cell2<false>(src, dst, flags, xtab, lt, k, p, force);
-    return;
-  }
-  const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
-  if (kb < lt.n_body) list_cell2<true>(src, dst, flags, xtab, lt, lt.n_link + kb, p, force);
-  if (fs.sample) force_sample<2>(force, fs, blockIdx.x - nlb, gridDim.x - nlb);
 }
 
 // ============================================================================ aux kernels
@@ -160,15 +166,18 @@ __global__ __launch_bounds__(256) void flags2_kernel(const float* __restrict__ s
 // ============================================================================ launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
-void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
-                  const Params2& p, const float4* xtab, const ForceSink& fs, int bb, hipStream_t st, hipEvent_t mid) {
+void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt, const Params2& p,
+                  const float4* xtab, const ForceSink& fs, int bb, hipStream_t st, const Record* rec) {
   const uint32_t bx = bulk_block_x(p.W, bb);
-  const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H));
-  step2_bulk<<<grid, bx, 0, st>>>(src, dst, flags, xtab, p);
-  if (mid) (void)hipEventRecord(mid, st);
-  const uint32_t nlb = link_blocks(lt.n_link);
-  const uint32_t nbb = body_blocks(lt.n_body, fs.sample != 0);
-  if (nlb + nbb) step2_list<<<nlb + nbb, 256, 0, st>>>(src, dst, flags, lt, nlb, p, xtab, fs);
+  const uint32_t gx = (uint32_t(p.W) + bx - 1) / bx;
+  const uint32_t nlb = list_blocks(lt.n_link, bx);
+  const uint32_t nbb = body_list_blocks(lt.n_body, bx, fs.sample != 0);
+  const uint32_t yl = (nlb + nbb + gx - 1) / gx;
+  const dim3 grid(gx, yl + uint32_t(p.H));
+  if (rec)
+    step2_kernel<true><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, *rec);
+  else
+    step2_kernel<false><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, Record{nullptr, nullptr});
 }
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st) {
   xtab2_kernel<<<blocks_for(uint32_t(p.W), 256), 256, 0, st>>>(p, tab);

@@ -160,21 +160,56 @@ struct FeqIn {
 
 // ============================================================================ bulk kernel
 // Grid (ceil(W / blockDim.x), H, D): one block row of threads per x row, so x, y, z come from the
-// launch coordinates. Per direction the source address is a wave-uniform base (src + i*N - delta_i,
+// launch coordinates. Per direction the source address is a wave-uniform base (src + d_i*N - delta_i,
 // kept in SGPRs) plus the lane's byte offset, shared by all 19 loads; stores likewise.
-template <class S>
-__global__ __launch_bounds__(1024) void step3_bulk(const typename S::elem* __restrict__ src,
-                                                   typename S::elem* __restrict__ dst,
-                                                   const uint32_t* __restrict__ flags,
-                                                   const float4* __restrict__ xtab, Params3 p, FeqIn fin) {
+//
+// A row on a slip face (y = 0 or H-1, z = 0 or D-1) follows one pull pattern along its whole
+// length (src/shaders/step3d.ts:132-145): a direction entering across one face reads the mirrored
+// population at the node reflected through that face, one entering across two faces bounces back.
+// The pattern is uniform per row, so it only changes the per-direction base (d_i, delta_i); the
+// node's flag word must be exactly the face bits of its row (the host moves nodes whose pulls
+// differ, e.g. where the mirror node is solid, to the boundary list and marks them FLAG_LIST3).
+// Outlet nodes of face rows go to the boundary list as well.
+template <class S, bool REC>
+__device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+                                           const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
+                                           uint32_t nbb, const Params3& p, const FeqIn& fin,
+                                           const float4* __restrict__ xtab, const ForceSink& fs, const Record& rc,
+                                           uint32_t lb);
+
+// One time step in one launch. Grid (ceil(W / blockDim.x), H, zl + D): the first zl z-slices are
+// boundary-list blocks (dispatched first, so their dependent gathers overlap the streaming), the
+// rest are the bulk rows. REC additionally writes the adjoint's per-step record.
+template <class S, bool REC>
+__global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __restrict__ src,
+                                                     typename S::elem* __restrict__ dst,
+                                                     const uint32_t* __restrict__ flags,
+                                                     const float4* __restrict__ xtab, Params3 p, FeqIn fin,
+                                                     ListTables lt, uint32_t zl, uint32_t nlb, uint32_t nbb,
+                                                     ForceSink fs, Record rc) {
   using E = typename S::elem;
+  if (blockIdx.z < zl) {
+    const uint32_t lb = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (lb < nlb + nbb) list_part3<S, REC>(src, dst, flags, lt, nlb, nbb, p, fin, xtab, fs, rc, lb);
+    return;
+  }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  const uint32_t idx = x + uint32_t(p.W) * (blockIdx.y + uint32_t(p.H) * blockIdx.z);
+  const uint32_t y = blockIdx.y, z = blockIdx.z - zl;
+  const bool loY = y == 0u, hiY = y == uint32_t(p.H - 1), loZ = z == 0u, hiZ = z == uint32_t(p.D - 1);
+  const bool face = loY || hiY || loZ || hiZ;
+  uint32_t M = 0u;  // face bits of this row
+#pragma unroll
+  for (int i = 1; i < 19; i++) {
+    const bool outY = (loY && CY3[i] > 0) || (hiY && CY3[i] < 0);
+    const bool outZ = (loZ && CZ3[i] > 0) || (hiZ && CZ3[i] < 0);
+    if (outY || outZ) M |= 1u << i;
+  }
+  const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
   const uint32_t fl = flags[idx];
-  const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3);
-  const bool outlet = fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
-  if (fl != 0u && !inlet && !outlet) return;  // handled by the boundary list (or solid)
+  const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3 | M);
+  const bool outlet = !face && fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
+  if (fl != M && !inlet && !outlet) return;  // handled by the boundary list (or solid)
   const size_t N = p.N;
   const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
@@ -189,14 +224,30 @@ __global__ __launch_bounds__(1024) void step3_bulk(const typename S::elem* __res
 #pragma unroll
   for (int i = 0; i < 19; i++) {
     const ptrdiff_t delta = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W + CX3[i];
-    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * ptrdiff_t(sizeof(E));
+    const bool outY = (loY && CY3[i] > 0) || (hiY && CY3[i] < 0);
+    const bool outZ = (loZ && CZ3[i] > 0) || (hiZ && CZ3[i] < 0);
+    int sd = i;
+    ptrdiff_t sdelta = delta;
+    if (outY && outZ) {
+      sd = opp3(i);
+      sdelta = 0;
+    } else if (outY) {
+      sd = mirror_y3(i);
+      sdelta = ptrdiff_t(CZ3[i]) * p.WH + CX3[i];
+    } else if (outZ) {
+      sd = mirror_z3(i);
+      sdelta = ptrdiff_t(CY3[i]) * p.W + CX3[i];
+    }
+    const char* base = sb + (ptrdiff_t(sd) * ptrdiff_t(N) - sdelta) * ptrdiff_t(sizeof(E));
     uint32_t o = off;
     // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z),
-    // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements.
+    // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements (rows off the faces
+    // only, where base is the plain one).
     if (CX3[i] < 0 && outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
     f[i] = S::load_at(base, o);
   }
   const Mom3 m = moments3(f);
+  if (REC) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
   if (outlet) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
@@ -229,11 +280,11 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
 // replaces the gathered f_j with the interpolation against its record's second operand and adds
 // c_j (f_j + f_i) to the force; a slip face mirroring a solid node (step3d.ts:139-146) keeps f_j and
 // adds c_j 2 f_j.
-template <class S, bool BODY>
+template <class S, bool BODY, bool REC>
 __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
                                            const Params3& p, const FeqIn& fin, const float4* __restrict__ xtab,
-                                           float (&force)[3]) {
+                                           const Record& rc, float (&force)[3]) {
   const uint32_t N = p.N;
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
@@ -280,6 +331,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
         force[0] += float(CX3[j]) * (fj + fi);
         force[1] += float(CY3[j]) * (fj + fi);
         force[2] += float(CZ3[j]) * (fj + fi);
+        if (REC) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[i], gb[i] != NO_OPERAND);
         f[i] = fi;
       } else if (xm & (1u << i)) {
         force[0] += float(CX3[j]) * (2.0f * fj);
@@ -289,6 +341,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
     }
   }
   const Mom3 m = moments3(f);
+  if (REC) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
   if (fl & FLAG_OUTLET3) {
 #pragma unroll
     for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
@@ -306,23 +359,24 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
 }
 
-// Boundary-list kernel. Blocks [0, nlb) take the nodes without body links; the remaining blocks take
-// the nodes with body links and carry the force sample (an FP64 reduction over those blocks).
-template <class S>
-__global__ __launch_bounds__(256) void step3_list(const typename S::elem* __restrict__ src,
-                                                  typename S::elem* __restrict__ dst,
-                                                  const uint32_t* __restrict__ flags, ListTables lt, uint32_t nlb,
-                                                  Params3 p, FeqIn fin, const float4* __restrict__ xtab,
-                                                  ForceSink fs) {
+// Boundary-list part of the step: list block lb of the launch. List blocks [0, nlb) take the nodes
+// without body links; the nbb blocks after them take the nodes with body links and carry the force
+// sample (an FP64 reduction over those blocks).
+template <class S, bool REC>
+__device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+                                           const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
+                                           uint32_t nbb, const Params3& p, const FeqIn& fin,
+                                           const float4* __restrict__ xtab, const ForceSink& fs, const Record& rc,
+                                           uint32_t lb) {
   float force[3] = {0.0f, 0.0f, 0.0f};
-  if (blockIdx.x < nlb) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < lt.n_link) list_node3<S, false>(src, dst, flags, lt, k, p, fin, xtab, force);
+  if (lb < nlb) {
+    const uint32_t k = lb * blockDim.x + threadIdx.x;
+    if (k < lt.n_link) list_node3<S, false, REC>(src, dst, flags, lt, k, p, fin, xtab, rc, force);
     return;
   }
-  const uint32_t kb = (blockIdx.x - nlb) * blockDim.x + threadIdx.x;
-  if (kb < lt.n_body) list_node3<S, true>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, force);
-  if (fs.sample) force_sample<3>(force, fs, blockIdx.x - nlb, gridDim.x - nlb);
+  const uint32_t kb = (lb - nlb) * blockDim.x + threadIdx.x;
+  if (kb < lt.n_body) list_node3<S, true, REC>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, rc, force);
+  if (fs.sample) force_sample<3>(force, fs, lb - nlb, nbb);
 }
 
 // ============================================================================ aux kernels
@@ -439,8 +493,10 @@ __global__ void reset_state_kernel(uint32_t* state) { state[0] = 0u; }
 
 // ============================================================================ launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
-uint32_t link_blocks(uint32_t n_link) { return blocks_for(n_link, 256); }
-uint32_t body_blocks(uint32_t n_body, bool sample) { return n_body ? blocks_for(n_body, 256) : (sample ? 1u : 0u); }
+uint32_t list_blocks(uint32_t n, uint32_t bx) { return blocks_for(n, bx); }
+uint32_t body_list_blocks(uint32_t n_body, uint32_t bx, bool sample) {
+  return n_body ? blocks_for(n_body, bx) : (sample ? 1u : 0u);
+}
 
 // Threads per block along x: the widest multiple of 64 not above the requested size that divides W
 // (no idle lanes), else one block of ceil(W / 64) waves per row.
@@ -453,30 +509,34 @@ uint32_t bulk_block_x(int W, int requested) {
 }
 
 template <class S>
-static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
-                    const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bb,
-                    hipStream_t st, hipEvent_t mid) {
+static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt, const Params3& p,
+                    const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, hipStream_t st,
+                    const Record* rec) {
   using E = typename S::elem;
   FeqIn fin;
   for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
   const uint32_t bx = bulk_block_x(p.W, bb);
-  const dim3 grid((uint32_t(p.W) + bx - 1) / bx, uint32_t(p.H), uint32_t(p.D));
-  step3_bulk<S><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin);
-  if (mid) (void)hipEventRecord(mid, st);
-  const uint32_t nlb = link_blocks(lt.n_link);
-  const uint32_t nbb = body_blocks(lt.n_body, fs.sample != 0);
-  if (nlb + nbb)
-    step3_list<S><<<nlb + nbb, 256, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, lt, nlb, p, fin,
-                                              xtab, fs);
+  const uint32_t gx = (uint32_t(p.W) + bx - 1) / bx;
+  const uint32_t nlb = list_blocks(lt.n_link, bx);
+  const uint32_t nbb = body_list_blocks(lt.n_body, bx, fs.sample != 0);
+  const uint32_t per_slice = gx * uint32_t(p.H);
+  const uint32_t zl = (nlb + nbb + per_slice - 1) / per_slice;
+  const dim3 grid(gx, uint32_t(p.H), zl + uint32_t(p.D));
+  if (rec)
+    step3_kernel<S, true><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin,
+                                                lt, zl, nlb, nbb, fs, *rec);
+  else
+    step3_kernel<S, false><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin,
+                                                 lt, zl, nlb, nbb, fs, Record{nullptr, nullptr});
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  hipStream_t st, hipEvent_t mid) {
+                  hipStream_t st, const Record* rec) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
-    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, mid); break;
+    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
+    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
   }
 }
 

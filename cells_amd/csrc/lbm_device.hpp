@@ -34,6 +34,8 @@ constexpr float W3[19] = {1.0f / 3,  1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18,
 constexpr uint32_t FLAG_SOLID3 = 1u << 19;
 constexpr uint32_t FLAG_INLET3 = 1u << 20;
 constexpr uint32_t FLAG_OUTLET3 = 1u << 21;
+// Set by the host on nodes whose pulls the bulk kernel's row patterns do not cover; never reported.
+constexpr uint32_t FLAG_LIST3 = 1u << 31;
 constexpr uint32_t INLET_DIRS3 = (1u << 1) | (1u << 7) | (1u << 9) | (1u << 13) | (1u << 15);
 constexpr uint32_t OUTLET_DIRS3 = (1u << 2) | (1u << 8) | (1u << 10) | (1u << 14) | (1u << 16);
 
@@ -54,6 +56,14 @@ __device__ __forceinline__ float smoothstep(float lo, float hi, float x) {
   return t * t * (3.0f - 2.0f * t);
 }
 __device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// d f_i / d q of a Bouzidi link (src/shaders/step.ts:145-156) at its operands f_j (at the node) and
+// b (f_j at x + c_i for q < 1/2, f_i at x otherwise): 2 (f_j - b), or 0 when the q < 1/2 operand is
+// missing; (b - f_j) / (2 q^2) for q >= 1/2.
+__device__ __forceinline__ float bouzidi_dq(float q, float fj, float b, bool has_operand) {
+  if (q < 0.5f) return has_operand ? 2.0f * (fj - b) : 0.0f;
+  return (b - fj) / (2.0f * q * q);
+}
 
 // ---------------------------------------------------------------- storage codecs
 // decode / encode one stored element; load(p, k) and store(p, k, v) address element k of p, while

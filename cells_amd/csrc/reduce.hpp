@@ -1,9 +1,9 @@
-// Deterministic FP64 force reduction used by the boundary-list step kernels.
+// Deterministic FP64 force reduction used by the step kernels.
 //
 // Replaces the reference's single-workgroup FP32 reduce pass (src/shaders/aux.ts:144-172,
-// src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every block of the group sums its
-// cells' momentum-exchange forces in double with wave64 butterfly shuffles and LDS, publishes one
-// partial, and the last block to finish (completion counter) sums the partials in block order and
+// src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every wave of the group sums its
+// cells' momentum-exchange forces in double with wave64 butterfly shuffles and publishes one
+// partial, and the last block to finish (completion counter) sums the partials in wave order and
 // appends the sample to the history ring. The summation order is fixed, so a sample is
 // bit-reproducible run to run.
 //
@@ -24,56 +24,43 @@ __device__ __forceinline__ void wave_sum(double (&v)[DIM]) {
     for (int d = 0; d < DIM; d++) v[d] += __shfl_xor(v[d], off, 64);
 }
 
-// Block-wide sum; the result is valid in thread 0. blockDim.x must be a multiple of 64, <= 1024.
-template <int DIM>
-__device__ __forceinline__ void block_sum(double (&v)[DIM], double (*lds)[16]) {
-  wave_sum<DIM>(v);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  if (lane == 0)
-#pragma unroll
-    for (int d = 0; d < DIM; d++) lds[d][wave] = v[d];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-      double s = 0.0;
-      for (int w = 0; w < nwaves; w++) s += lds[d][w];
-      v[d] = s;
-    }
-  }
-  __syncthreads();
-}
-
 // Every thread of every block of the group must call this (no early return before it). A group is
-// the whole grid (single tunnel) or the contiguous list tiles of one tunnel (batch): block is this
-// block's index inside the group, nblocks the group's size.
+// a contiguous run of blocks whose threads hold the cells of the sample in order (the body-link
+// nodes of one tunnel): block is this block's index inside the group, nblocks the group's size.
+// Partials are per wave (64 consecutive nodes), and the last block sums them in wave order with a
+// fixed lane assignment, so the sample does not depend on the block size.
 template <int DIM>
 __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uint32_t block, uint32_t nblocks) {
-  __shared__ double lds[DIM][16];
   __shared__ int is_last;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   double v[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) v[d] = double(force[d]);
-  block_sum<DIM>(v, lds);
-  if (threadIdx.x == 0) {
+  wave_sum<DIM>(v);
+  if (lane == 0) {
+    const size_t part = size_t(block) * wpb + wave;
 #pragma unroll
     for (int d = 0; d < DIM; d++)
-      __hip_atomic_store(fs.partials + size_t(block) * DIM + d, v[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);  // the partial stores have completed before the counter moves
+      __hip_atomic_store(fs.partials + part * DIM + d, v[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's partial has landed before the block reports
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
     const uint32_t prev = __hip_atomic_fetch_add(fs.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == nblocks - 1);
   }
   __syncthreads();
-  if (!is_last) return;
+  if (!is_last || wave != 0) return;
+  const uint32_t nparts = nblocks * wpb;
   double acc[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) acc[d] = 0.0;
-  for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x)
+  for (uint32_t k = lane; k < nparts; k += 64u)
 #pragma unroll
     for (int d = 0; d < DIM; d++)
       acc[d] += __hip_atomic_load(fs.partials + size_t(k) * DIM + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<DIM>(acc, lds);
-  if (threadIdx.x == 0) {
+  wave_sum<DIM>(acc);
+  if (lane == 0) {
     const uint32_t st = __hip_atomic_load(fs.state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t slot = st % HISTORY_LEN;
 #pragma unroll
@@ -81,11 +68,6 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uin
     __hip_atomic_store(fs.state, st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(fs.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-}
-
-template <int DIM>
-__device__ __forceinline__ void force_sample(const float (&force)[DIM], const ForceSink& fs) {
-  force_sample<DIM>(force, fs, blockIdx.x, gridDim.x);
 }
 
 }  // namespace lbm

@@ -80,6 +80,14 @@ struct lbm_solver {
   lbm::BzLink* links = nullptr;     // Bouzidi records
   uint32_t n_links = 0;
   std::vector<float> sdf_host;      // the SDF as last set (q is computed from it on the host)
+  // Host copies of the boundary-list tables and the flag words as last built, from which the
+  // discrete adjoint builds its transposed tables; bumped generation on every rebuild.
+  std::vector<uint32_t> h_flags, h_list, h_gtab, h_bmask, h_xmask, h_lbase;
+  std::vector<lbm::BzLink> h_links;
+  std::vector<uint32_t> h_link_node;  // cell of each Bouzidi record
+  std::vector<uint8_t> h_link_dir;    // direction of each Bouzidi record
+  uint64_t tables_gen = 0;
+  struct lbm_adjoint* adj = nullptr;  // discrete-adjoint state, created on first use
   int64_t n_bulk = 0, n_solid = 0;
   double* partials = nullptr;
   uint32_t* counter = nullptr;
@@ -103,8 +111,25 @@ struct lbm_solver {
 };
 
 namespace lbm {
-// Helpers of capi.hip used by the batch driver.
+// Helpers of capi.hip used by the batch driver and the adjoint.
 int create_2d_on(const lbm_config2d* cfg, int device, hipStream_t stream, lbm_solver** out);
 int rebuild_flags(lbm_solver* s);
 void record_steps(lbm_solver* s, int64_t n);
+// One step on the solver's stream: force sample as configured (sample < 0) or forced on / off
+// (1 / 0); with rec, the step also writes the adjoint's record.
+int enqueue_step(lbm_solver* s, const Record* rec, int sample);
+
+// The bulk kernel's view of a D3Q19 node (step3_kernel): the face bits of its row, whether the flag
+// word selects the bulk pass, and the gather index of each pull under the row's pattern.
+struct Row3 {
+  bool loY, hiY, loZ, hiZ;
+  uint32_t mask;
+};
+Row3 row3(const lbm_solver* s, uint32_t y, uint32_t z);
+// 0: not taken by the bulk kernel, 1: fluid pulls, 2: inlet (stores the inlet equilibrium)
+int bulk_kind3(const Row3& r, uint32_t v);
+uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outlet, int i);
+
+// Frees the adjoint state of a solver (adjoint.hip).
+void adjoint_free(lbm_solver* s);
 }  // namespace lbm

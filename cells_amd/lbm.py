@@ -49,6 +49,15 @@ class Config3D(C.Structure):
     ]
 
 
+class AdjointConfig(C.Structure):
+    """lbm_adjoint_config (include/lbm.h)."""
+    _fields_ = [
+        ("steps", C.c_int64), ("window_start", C.c_int64), ("direction", C.c_double * 3), ("scale", C.c_double),
+        ("checkpoint_every", C.c_int32), ("ramp_kind", C.c_int32), ("ramp_u", C.c_double),
+        ("ramp_steps", C.c_double), ("record_budget_bytes", C.c_int64),
+    ]
+
+
 # Reference config keys (camelCase, src/solver.ts:15-40, src/solver3d.ts:11-31) -> struct fields.
 KEYS2 = {
     "width": "width", "height": "height", "left": "left", "right": "right", "bottom": "bottom", "top": "top",
@@ -152,6 +161,10 @@ def load_library(path: str = LIB_PATH):
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
+        "lbm_adjoint_backward": ([P, C.c_double, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_adjoint_info": ([P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
+        "lbm_adjoint_release": ([P], C.c_int),
         "lbm_batch_create_2d": ([C.POINTER(Config2D), C.c_int32, C.c_int, C.POINTER(P)], C.c_int),
         "lbm_batch_destroy": ([P], None),
         "lbm_batch_size": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
@@ -280,7 +293,7 @@ class _Base:
         return float(ms.value)
 
     def profile_steps(self, n: int):
-        """(total, bulk-kernel, list-kernel) milliseconds of n steps, events around every kernel."""
+        """(total, summed per-step, slowest step) milliseconds of n steps, an event after every step."""
         t, b, l = C.c_double(), C.c_double(), C.c_double()
         _check(_lib.lbm_profile_steps(self._h, int(n), C.byref(t), C.byref(b), C.byref(l)))
         return float(t.value), float(b.value), float(l.value)

@@ -161,11 +161,11 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n);
 
 /* Timing, replacing the timestamp queries around one compute pass of src/bench.ts:35-58.
  * lbm_time_steps enqueues n steps between two HIP events on the handle's stream and returns the
- * elapsed milliseconds. lbm_profile_steps does the same with events around every kernel of every
- * step and returns the summed time of the bulk (plain-node) kernels and of the boundary-list
- * kernels; it perturbs the schedule slightly and is for per-kernel roofline figures. */
+ * elapsed milliseconds. lbm_profile_steps does the same with an event after every step (one kernel
+ * launch per step) and returns the total, the summed per-step times and the slowest step; it
+ * perturbs the schedule slightly and is for per-kernel roofline figures. */
 int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
-int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* bulk_ms, double* list_ms);
+int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms);
 /* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
  * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid on a copy of the
  * solver's state, keeps the fastest (cached per lattice, precision and grid size for the process),
@@ -177,6 +177,40 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
 int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t iters, double* ms_total);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
+
+/* ---- Discrete adjoint (shape gradients) ----
+ * No counterpart in the reference. A forward/backward pair with autograd-Function semantics:
+ * lbm_adjoint_forward runs the tunnel from its initial field (lbm_init_field: rest in 2D, inflow
+ * equilibrium in 3D) for `steps` steps with the given inlet ramp and a force sample every step, and
+ * returns J = scale * mean over the samples of steps window_start+1 .. steps of direction . F (the
+ * sample of step k is taken from the populations entering it, as lbm_read_forces numbers them). It
+ * keeps a checkpoint of the populations every checkpoint_every steps, and records the last segments
+ * directly while record_budget_bytes allows (16 B per cell per step plus 4 B per Bouzidi link per
+ * step), so that the backward pass does not recompute them.
+ * lbm_adjoint_backward(seed) then returns seed * dJ/dsdf at every node (N doubles) of the discrete
+ * step: it recomputes each remaining segment from its checkpoint, runs the reverse-time kernels
+ * (FP32 adjoint populations, FP64 accumulation of dJ/dq per Bouzidi link) and chains dq/dsdf through
+ * q = clamp(dF / max(dF - dS, 1e-6), 1e-3, 1) (src/shaders/step.ts:140-142); the flag topology is
+ * that of the current SDF. Afterwards the solver holds the state after the forward run.
+ * Not supported: Smagorinsky (C_s > 0) and the 2D body force. */
+typedef struct lbm_adjoint_config {
+  int64_t steps;
+  int64_t window_start;
+  double direction[3]; /* 2D uses the first two */
+  double scale;
+  int32_t checkpoint_every;
+  int32_t ramp_kind; /* as lbm_set_inlet_ramp; 0 keeps u_in constant */
+  double ramp_u, ramp_steps;
+  int64_t record_budget_bytes; /* device memory for recorded segments; at least one segment is kept */
+} lbm_adjoint_config;
+int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J);
+int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n);
+/* Node counts of the reverse-time kernel (streamed and listed nodes, list entries) and the number of
+ * recorded and total segments of the last forward run. */
+int lbm_adjoint_info(lbm_solver* s, int64_t* regular_nodes, int64_t* list_nodes, int64_t* entries,
+                     int64_t* stored_segments, int64_t* segments);
+/* Frees the checkpoints, records and tables (also freed by lbm_destroy). */
+int lbm_adjoint_release(lbm_solver* s);
 
 /* ---- Batched D2Q9 tunnels ----
  * No counterpart in the reference (one tunnel per page); the nearest analogue is encoding many steps

@@ -1,0 +1,136 @@
+"""Discrete adjoint on the GPU: the gradient of a drag objective with respect to the SDF against the
+FP64 torch reference (reverse mode through the same discrete step, tests/adjoint_ref.py), against
+central finite differences of the GPU forward (h = 1e-3 in the shape parameters), the
+checkpoint-recompute path against the fully recorded one, and the autograd-Function wrapper."""
+import math
+
+import numpy as np
+import pytest
+
+from cells_amd import geometry as g
+from cells_amd import shapes
+from cells_amd.adjoint import ShapeAdjoint, TunnelObjective, parameter_gradient
+from cells_amd.lbm import X_OPEN, Y_SLIP, Solver, Solver3D
+
+pytestmark = pytest.mark.gpu
+
+
+def tunnel2(W, H, u, tau=0.56):
+    return dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=tau, uIn=u,
+                spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=1)
+
+
+def test_adjoint2d_matches_fp64_reference():
+    from tests.adjoint_ref import reference_gradient
+
+    W, H, u, L = 64, 24, 0.05, 8.0
+    steps, window, ramp = 200, 80, 60
+    cfg = tunnel2(W, H, u)
+    sdf = g.empty_sdf(W, H)
+    g.add_circle(sdf, W, H, 16, 11.5, 4)
+    print(f"adjoint2d vs FP64: {W}x{H}, {steps} steps", flush=True)
+    s = Solver.create(dict(cfg))
+    adj = ShapeAdjoint(s, steps, window, (1.0, 0.0), 2 / (u * u * L), checkpoint_every=50,
+                       ramp=("smoothstep", u, ramp), record_budget_bytes=1 << 40)
+    J = adj.forward(sdf)
+    gs = adj.backward()
+    info = adj.info()
+    assert info["stored_segments"] == info["segments"] == 4
+    J_ref, g_ref = reference_gradient(cfg, sdf, steps, window, u, ramp, L, segment=50)
+    assert abs(J - J_ref) <= 1e-4 * abs(J_ref)
+    err = np.linalg.norm(gs - g_ref) / np.linalg.norm(g_ref)
+    print(f"J {J:.8f} ref {J_ref:.8f}; |g - g_ref| / |g_ref| = {err:.2e}; nonzero {np.count_nonzero(g_ref)}")
+    assert err < 2e-3
+    assert np.count_nonzero(gs) == np.count_nonzero(g_ref)
+    # the checkpoint-recompute path (one recorded segment) gives the same result bit for bit
+    adj2 = ShapeAdjoint(s, steps, window, (1.0, 0.0), 2 / (u * u * L), checkpoint_every=50,
+                        ramp=("smoothstep", u, ramp), record_budget_bytes=0)
+    assert adj2.forward(sdf) == J
+    assert adj2.info()["stored_segments"] == 1
+    np.testing.assert_array_equal(adj2.backward(), gs)
+    # the seed scales the gradient
+    np.testing.assert_allclose(adj2.backward(seed=-2.5), -2.5 * gs, rtol=1e-6, atol=1e-12)
+    s.destroy()
+
+
+def test_adjoint2d_naca_bumps_match_finite_differences():
+    W, H, u = 128, 48, 0.05
+    chord, alpha = 24.0, 5.0
+    steps, window, ramp = 600, 200, 200
+    L = chord
+    cfg = tunnel2(W, H, u, tau=0.58)
+    theta = np.array([alpha, 0.0, 0.0, 0.0, 0.0])
+
+    def sdf_of(th):
+        return shapes.naca_bumps(W, H, 32.0, 23.5, chord, th[0], th[1:])
+
+    s = Solver.create(dict(cfg))
+    adj = ShapeAdjoint(s, steps, window, (1.0, 0.0), 2 / (u * u * L), checkpoint_every=100,
+                       ramp=("smoothstep", u, ramp))
+    sdf0, dsdf = sdf_of(theta)
+    J0 = adj.forward(sdf0)
+    grad = parameter_gradient(adj.backward(), dsdf)
+    h = 1e-3
+    fd = np.zeros_like(theta)
+    for p in range(len(theta)):
+        e = np.zeros_like(theta)
+        e[p] = h
+        fd[p] = (adj.forward(sdf_of(theta + e)[0]) - adj.forward(sdf_of(theta - e)[0])) / (2 * h)
+    print(f"J {J0:.6f}\nadjoint {grad}\nFD      {fd}")
+    assert np.all(np.isfinite(grad))
+    for p in range(len(theta)):
+        assert abs(grad[p] - fd[p]) <= 0.02 * abs(fd[p]) + 1e-4 * np.abs(fd).max(), p
+    s.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_adjoint3d_sphere_harmonics_match_finite_differences(precision):
+    W, H, D, u = 48, 24, 24, 0.05
+    r = 5.0
+    steps, window = 300, 100
+    tau = 0.56
+    cfg = dict(width=W, height=H, depth=D, precision=precision, tau=tau, uIn=u, spongeFraction=0.15,
+               spongeTau=1, absorb=0.02, forceEvery=1)
+    coeffs = np.zeros(16)
+    area = math.pi * r * r
+
+    def sdf_of(b):
+        return shapes.sphere_sh(W, H, D, (12.0, 11.5, 11.5), r, b)
+
+    s = Solver3D.create(dict(cfg))
+    adj = ShapeAdjoint(s, steps, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=50,
+                       ramp=("smoothstep", u, 100))
+    sdf0, dsdf = sdf_of(coeffs)
+    J0 = adj.forward(sdf0)
+    grad = parameter_gradient(adj.backward(), dsdf)
+    h = 1e-3 if precision == "fp32" else 2e-2  # FP16 storage rounds J at ~1e-4
+    check = [0, 1, 3, 6]
+    fd = {}
+    for p in check:
+        e = np.zeros(16)
+        e[p] = h
+        fd[p] = (adj.forward(sdf_of(coeffs + e)[0]) - adj.forward(sdf_of(coeffs - e)[0])) / (2 * h)
+    print(f"{precision}: J {J0:.6f}; adjoint {[round(grad[p], 5) for p in check]}; FD {[round(fd[p], 5) for p in check]}")
+    tol = 0.02 if precision == "fp32" else 0.05
+    scale = max(abs(v) for v in fd.values())
+    for p in check:
+        assert abs(grad[p] - fd[p]) <= tol * abs(fd[p]) + 2e-3 * scale, p
+    s.destroy()
+
+
+def test_autograd_function_semantics():
+    import torch
+
+    W, H, u = 64, 24, 0.05
+    cfg = tunnel2(W, H, u)
+    sdf = g.empty_sdf(W, H)
+    g.add_circle(sdf, W, H, 16, 11.5, 4)
+    s = Solver.create(dict(cfg))
+    adj = ShapeAdjoint(s, 120, 40, (1.0, 0.0), 1.0, checkpoint_every=40, ramp=("smoothstep", u, 40))
+    t = torch.tensor(sdf, dtype=torch.float32, requires_grad=True)
+    J = TunnelObjective.apply(t, adj)
+    (3.0 * J).backward()
+    direct = adj.backward(seed=3.0)
+    np.testing.assert_array_equal(t.grad.numpy(), direct.astype(np.float32))
+    assert float(J) == adj.forward(sdf)
+    s.destroy()
