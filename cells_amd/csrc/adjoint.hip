@@ -293,7 +293,10 @@ __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, co
   if (at.aflag[idx]) return;
   float lam[9];
 #pragma unroll
-  for (int a = 0; a < 9; a++) lam[a] = as.mu_next[size_t(a) * N + idx + uint32_t(CY2[a] * p.W + CX2[a])];
+  for (int a = 0; a < 9; a++) {
+    const int64_t nb = int64_t(idx) + int64_t(CY2[a]) * p.W + CX2[a];  // the downstream neighbour (in the grid)
+    lam[a] = as.mu_next[size_t(a) * N + size_t(nb)];
+  }
   const float4 m = as.mom[idx];
   const float u[3] = {m.y, m.z, 0.0f};
   const float4 t = xtab[x];
@@ -505,6 +508,21 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
     else kind[k] = (fl & FLAG_INLET3) ? AK_INLET3 : (fl & FLAG_OUTLET3) ? AK_OUTLET3 : AK_PLAIN;
     lk[k] = (pos[y] != NO_OPERAND && s->h_bmask[pos[y]]) ? pos[y] : NO_OPERAND;
   }
+  // Every index a launch will use must address the grid: the streamed nodes' downstream neighbours
+  // and every entry's source and target.
+  for (uint32_t y = 0; y < N; y++) {
+    if (aflag[y]) continue;
+    const int64_t cx = y % W, cy = (y / W) % H, cz = y / WH;
+    for (int i = 0; i < Q; i++) {
+      const int64_t nx = cx + (d2 ? CX2[i] : CX3[i]), ny = cy + (d2 ? CY2[i] : CY3[i]), nz = cz + (d2 ? 0 : CZ3[i]);
+      if (nx < 0 || nx >= W || ny < 0 || ny >= H || nz < 0 || nz >= D)
+        return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node has a neighbour off the grid");
+    }
+  }
+  for (const AdjEntry& en : ent)
+    if (en.src >= uint64_t(Q) * N || en.a >= uint32_t(Q)) return fail(LBM_ERR_DEVICE, "adjoint tables: entry out of range");
+  for (uint32_t v : lk)
+    if (v != NO_OPERAND && v >= s->h_list.size()) return fail(LBM_ERR_DEVICE, "adjoint tables: link node out of range");
   free_tables(a);
   const size_t nl = alist.size();
   HIP_TRY(hipMalloc(&a->aflag, N));
@@ -602,19 +620,23 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   a->nseg = (a->T + a->K - 1) / a->K;
   const size_t fbytes = size_t(s->Q) * s->N * s->elem;
   const size_t seg_bytes = size_t(a->K) * s->N * sizeof(float4) + size_t(a->K) * std::max<uint32_t>(s->n_links, 1) * 4;
-  const int64_t budget = std::max<int64_t>(cfg->record_budget_bytes, int64_t(seg_bytes));
-  a->nstored = std::min<int64_t>(a->nseg, std::max<int64_t>(1, budget / int64_t(seg_bytes)));
+  a->nstored = std::min<int64_t>(a->nseg, std::max<int64_t>(0, cfg->record_budget_bytes) / int64_t(seg_bytes));
   a->ckpt.assign(size_t(a->nseg), nullptr);
   for (int64_t g = 0; g < a->nseg - a->nstored; g++) HIP_TRY(hipMalloc(&a->ckpt[size_t(g)], fbytes));
-  a->rec_mom.assign(size_t(a->nstored), nullptr);
-  a->rec_dq.assign(size_t(a->nstored), nullptr);
-  for (int64_t g = 0; g < a->nstored; g++) {
+  // slots [0, nstored) hold the recorded segments for good; one more slot takes the recomputed ones
+  const int64_t nslots = a->nstored + (a->nstored < a->nseg ? 1 : 0);
+  a->rec_mom.assign(size_t(nslots), nullptr);
+  a->rec_dq.assign(size_t(nslots), nullptr);
+  for (int64_t g = 0; g < nslots; g++) {
     HIP_TRY(hipMalloc(&a->rec_mom[size_t(g)], size_t(a->K) * s->N * sizeof(float4)));
     HIP_TRY(hipMalloc(&a->rec_dq[size_t(g)], size_t(a->K) * std::max<uint32_t>(s->n_links, 1) * 4));
   }
   HIP_TRY(hipMalloc(&a->final_state, fbytes));
   // the run: from the initial field, with the configured inlet ramp and a force sample every step
   if (int rc = lbm_set_inlet_ramp(s, cfg->ramp_kind, cfg->ramp_u, cfg->ramp_steps)) return rc;
+  // The 3D initial field is the equilibrium at the inlet speed of the first step (rest under a ramp
+  // from zero), whatever speed earlier steps left in the parameters.
+  if (s->nd == 3) s->p3.uIn = float(s->ramp.kind ? s->ramp.at(0) : s->cfg3.u_in);
   if (int rc = lbm_init_field(s, nullptr, 0)) return rc;
   const int64_t count = a->T - cfg->window_start;
   a->w.assign(size_t(a->T + 1), 0.0);
@@ -669,8 +691,13 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
   const uint32_t N = s->N;
   const size_t fbytes = size_t(s->Q) * N * s->elem;
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
-  for (float*& m : a->mu)
+  for (float*& m : a->mu) {
+    if (m) HIP_TRY(hipFree(m));
+    m = nullptr;
     if (int rc = alloc_zero(reinterpret_cast<void**>(&m), size_t(s->Q) * N * 4, s->stream)) return rc;
+  }
+  if (a->acc) HIP_TRY(hipFree(a->acc));
+  a->acc = nullptr;
   if (int rc = alloc_zero(reinterpret_cast<void**>(&a->acc), nlk * 8, s->stream)) return rc;
   // weights scaled by the upstream gradient
   const std::vector<double> w0 = a->w;
@@ -684,8 +711,8 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
     if (g >= first_stored) {
       slot = int(g - first_stored);
     } else {
-      // recompute the segment from its checkpoint into slot 0 (its stored segment was consumed)
-      slot = 0;
+      // recompute the segment from its checkpoint into the scratch slot
+      slot = int(a->nstored);
       if (hipMemcpyAsync(s->f[0], a->ckpt[size_t(g)], fbytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
         rc = fail(LBM_ERR_DEVICE, "adjoint: checkpoint restore failed");
         break;
@@ -693,7 +720,7 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
       s->parity = 0;
       s->step = n0;
       for (int64_t k = n0; k < n1 && rc == LBM_OK; k++) {
-        Record rec{a->rec_mom[0] + size_t(k - n0) * N, a->rec_dq[0] + size_t(k - n0) * nlk};
+        Record rec{a->rec_mom[size_t(slot)] + size_t(k - n0) * N, a->rec_dq[size_t(slot)] + size_t(k - n0) * nlk};
         rc = enqueue_step(s, &rec, 0);
       }
     }
@@ -703,7 +730,9 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
     }
   }
   a->w = w0;
-  // restore the state after the forward run
+  // restore the state after the forward run (the recomputation moved the inlet parameter too)
+  if (s->nd == 2) s->p2.uIn = float(s->ramp.kind ? s->ramp.at(a->T - 1) : s->cfg2.u_in);
+  else s->p3.uIn = float(s->ramp.kind ? s->ramp.at(a->T - 1) : s->cfg3.u_in);
   if (hipMemcpyAsync(s->f[0], a->final_state, fbytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess && rc == LBM_OK)
     rc = fail(LBM_ERR_DEVICE, "adjoint: state restore failed");
   s->parity = 0;

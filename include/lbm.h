@@ -180,13 +180,14 @@ int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid)
 
 /* ---- Discrete adjoint (shape gradients) ----
  * No counterpart in the reference. A forward/backward pair with autograd-Function semantics:
- * lbm_adjoint_forward runs the tunnel from its initial field (lbm_init_field: rest in 2D, inflow
- * equilibrium in 3D) for `steps` steps with the given inlet ramp and a force sample every step, and
+ * lbm_adjoint_forward runs the tunnel from its initial field (lbm_init_field: rest in 2D, the
+ * equilibrium at the first step's inlet speed in 3D) for `steps` steps with the given inlet ramp and a force sample every step, and
  * returns J = scale * mean over the samples of steps window_start+1 .. steps of direction . F (the
  * sample of step k is taken from the populations entering it, as lbm_read_forces numbers them). It
- * keeps a checkpoint of the populations every checkpoint_every steps, and records the last segments
- * directly while record_budget_bytes allows (16 B per cell per step plus 4 B per Bouzidi link per
- * step), so that the backward pass does not recompute them.
+ * records the last segments directly while record_budget_bytes allows (16 B per cell per step plus
+ * 4 B per Bouzidi link per step), so that the backward pass does not recompute them, and keeps a
+ * checkpoint of the populations at the start of every other segment of checkpoint_every steps.
+ * lbm_adjoint_backward may be called any number of times after one forward run.
  * lbm_adjoint_backward(seed) then returns seed * dJ/dsdf at every node (N doubles) of the discrete
  * step: it recomputes each remaining segment from its checkpoint, runs the reverse-time kernels
  * (FP32 adjoint populations, FP64 accumulation of dJ/dq per Bouzidi link) and chains dq/dsdf through
@@ -201,7 +202,7 @@ typedef struct lbm_adjoint_config {
   int32_t checkpoint_every;
   int32_t ramp_kind; /* as lbm_set_inlet_ramp; 0 keeps u_in constant */
   double ramp_u, ramp_steps;
-  int64_t record_budget_bytes; /* device memory for recorded segments; at least one segment is kept */
+  int64_t record_budget_bytes; /* device memory for segments recorded by the forward pass (0: none) */
 } lbm_adjoint_config;
 int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J);
 int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n);

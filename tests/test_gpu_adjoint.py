@@ -42,14 +42,14 @@ def test_adjoint2d_matches_fp64_reference():
     print(f"J {J:.8f} ref {J_ref:.8f}; |g - g_ref| / |g_ref| = {err:.2e}; nonzero {np.count_nonzero(g_ref)}")
     assert err < 2e-3
     assert np.count_nonzero(gs) == np.count_nonzero(g_ref)
-    # the checkpoint-recompute path (one recorded segment) gives the same result bit for bit
+    # the checkpoint-recompute path (no recorded segment) gives the same result bit for bit
     adj2 = ShapeAdjoint(s, steps, window, (1.0, 0.0), 2 / (u * u * L), checkpoint_every=50,
                         ramp=("smoothstep", u, ramp), record_budget_bytes=0)
     assert adj2.forward(sdf) == J
-    assert adj2.info()["stored_segments"] == 1
+    assert adj2.info()["stored_segments"] == 0
     np.testing.assert_array_equal(adj2.backward(), gs)
     # the seed scales the gradient
-    np.testing.assert_allclose(adj2.backward(seed=-2.5), -2.5 * gs, rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(adj2.backward(seed=-2.5), -2.5 * gs, rtol=1e-5, atol=1e-6 * np.abs(gs).max())
     s.destroy()
 
 
@@ -83,39 +83,49 @@ def test_adjoint2d_naca_bumps_match_finite_differences():
     s.destroy()
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp16"])
-def test_adjoint3d_sphere_harmonics_match_finite_differences(precision):
+def test_adjoint3d_sphere_harmonics_match_finite_differences():
+    """FP32 and FP16-storage forward passes, FP32 adjoint: both gradients against central differences
+    of the FP32 forward (the FP16 map itself is too rough at h = 1e-3 for differences: its rounding
+    moves J by about 1e-4 between neighbouring shapes)."""
     W, H, D, u = 48, 24, 24, 0.05
     r = 5.0
     steps, window = 300, 100
-    tau = 0.56
-    cfg = dict(width=W, height=H, depth=D, precision=precision, tau=tau, uIn=u, spongeFraction=0.15,
-               spongeTau=1, absorb=0.02, forceEvery=1)
-    coeffs = np.zeros(16)
     area = math.pi * r * r
+    coeffs = np.zeros(16)
 
     def sdf_of(b):
         return shapes.sphere_sh(W, H, D, (12.0, 11.5, 11.5), r, b)
 
-    s = Solver3D.create(dict(cfg))
-    adj = ShapeAdjoint(s, steps, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=50,
-                       ramp=("smoothstep", u, 100))
+    def make(precision):
+        cfg = dict(width=W, height=H, depth=D, precision=precision, tau=0.56, uIn=u, spongeFraction=0.15,
+                   spongeTau=1, absorb=0.02, forceEvery=1)
+        s = Solver3D.create(cfg)
+        return s, ShapeAdjoint(s, steps, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=50,
+                               ramp=("smoothstep", u, 100))
+
     sdf0, dsdf = sdf_of(coeffs)
-    J0 = adj.forward(sdf0)
-    grad = parameter_gradient(adj.backward(), dsdf)
-    h = 1e-3 if precision == "fp32" else 2e-2  # FP16 storage rounds J at ~1e-4
-    check = [0, 1, 3, 6]
-    fd = {}
-    for p in check:
-        e = np.zeros(16)
-        e[p] = h
-        fd[p] = (adj.forward(sdf_of(coeffs + e)[0]) - adj.forward(sdf_of(coeffs - e)[0])) / (2 * h)
-    print(f"{precision}: J {J0:.6f}; adjoint {[round(grad[p], 5) for p in check]}; FD {[round(fd[p], 5) for p in check]}")
-    tol = 0.02 if precision == "fp32" else 0.05
+    grads = {}
+    for precision in ("fp32", "fp16"):
+        s, adj = make(precision)
+        J0 = adj.forward(sdf0)
+        grads[precision] = parameter_gradient(adj.backward(), dsdf)
+        # the forward run does not depend on what ran before it (here: a backward pass)
+        assert adj.forward(sdf0) == J0
+        if precision == "fp32":
+            check = [0, 1, 3, 6, 9]
+            fd = {}
+            h = 1e-3
+            for p in check:
+                e = np.zeros(16)
+                e[p] = h
+                fd[p] = (adj.forward(sdf_of(coeffs + e)[0]) - adj.forward(sdf_of(coeffs - e)[0])) / (2 * h)
+        s.destroy()
     scale = max(abs(v) for v in fd.values())
-    for p in check:
-        assert abs(grad[p] - fd[p]) <= tol * abs(fd[p]) + 2e-3 * scale, p
-    s.destroy()
+    for precision, tol in (("fp32", 0.01), ("fp16", 0.02)):
+        g = grads[precision]
+        print(f"{precision}: adjoint {[round(g[p], 5) for p in check]}; FD(fp32) {[round(fd[p], 5) for p in check]}")
+        for p in check:
+            assert abs(g[p] - fd[p]) <= tol * abs(fd[p]) + 2e-3 * scale, (precision, p)
 
 
 def test_autograd_function_semantics():
