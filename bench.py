@@ -31,6 +31,21 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 REF_M5_FP32_MLUPS = 708.0  # reference, D3Q19 192x96x96 FP32 on Apple M5 (docs/dietpowers/2026-09-27-3d-tunnel-spec.md:291)
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "pmc_step3_kernel_d3q19_192x96x96_fp32.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the FP32 step kernel on this workload, from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes calibrated against a copy kernel of known traffic)."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return int(json.loads(block)["traffic_bytes_per_launch"])
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def tunnel_config(W, H, D, precision):
     from cells_amd.analysis import derive_tau
 
@@ -143,6 +158,151 @@ def sweep_bench(steps):
                 finite=not bad, nonfinite_tunnels=bad)
 
 
+def adjoint2d_case(fd=True):
+    """2D shape gradient: D2Q9 FP32 384 x 128, NACA0012 of chord 64 at 5 degrees (leading edge at
+    (96, 63.5)), Re 400 (tau 0.548), Zou-He inlet u = 0.1 ramped linearly over 3000 steps, Zou-He
+    outlet, slip walls, 15% sponge, absorbing layer 0.02; J = mean C_D over steps 4001..6000;
+    parameters (alpha, a_1..a_8) with bump width chord / 16; checkpoints every 200 steps."""
+    from cells_amd import shapes
+    from cells_amd.adjoint import ShapeAdjoint, parameter_gradient
+    from cells_amd.lbm import X_OPEN, Y_SLIP, Solver
+
+    W, H, chord, u, T, window, K = 384, 128, 64.0, 0.1, 6000, 2000, 200
+    tau = 3 * u * chord / 400 + 0.5
+    cfg = dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=tau, uIn=u,
+               spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=1)
+    theta = np.array([5.0] + [0.0] * 8)
+
+    def sdf_of(th):
+        return shapes.naca_bumps(W, H, 96.0, 63.5, chord, th[0], th[1:])
+
+    s = Solver.create(dict(cfg))
+    sdf0, dsdf = sdf_of(theta)
+    s.set_sdf(sdf0)
+    # plain forward: the same run through the ordinary step path
+    s.set_inlet_ramp("linear", u, 3000)
+    s.init_field()
+    s.sync()
+    t0 = time.perf_counter()
+    s.encode_steps(T)
+    s.sync()
+    t_plain = time.perf_counter() - t0
+    adj = ShapeAdjoint(s, T, window, (1.0, 0.0), 2 / (u * u * chord), checkpoint_every=K, ramp=("linear", u, 3000),
+                       record_budget_bytes=16 << 30)
+    adj.forward()
+    info = adj.info()  # builds the reverse-time tables (once per geometry)
+    t0 = time.perf_counter()
+    J = adj.forward()
+    t1 = time.perf_counter()
+    g_sdf = adj.backward()
+    t2 = time.perf_counter()
+    grad = parameter_gradient(g_sdf, dsdf)
+    out = dict(grid=[W, H], steps=T, J=J, grad=grad.tolist(), info=info, plain_forward_s=t_plain,
+               adjoint_forward_s=t1 - t0, backward_s=t2 - t1, ratio=(t2 - t0) / t_plain,
+               backward_over_forward=(t2 - t1) / t_plain)
+    if fd:
+        # central differences at h = 1e-3 and, because the smallest components are then within a
+        # few FP32 roundings of J, also at h = 1e-2
+        for h, key in ((1e-3, "fd"), (1e-2, "fd_h1e-2")):
+            fdv = []
+            for p in range(len(theta)):
+                e = np.zeros_like(theta)
+                e[p] = h
+                fdv.append((adj.forward(sdf_of(theta + e)[0]) - adj.forward(sdf_of(theta - e)[0])) / (2 * h))
+            fdv = np.array(fdv)
+            out[key] = fdv.tolist()
+            out["max_rel_err_vs_" + key] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
+            out["max_err_over_max_grad_vs_" + key] = float(np.max(np.abs(grad - fdv)) / np.abs(fdv).max())
+    s.destroy()
+    return out
+
+
+def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
+    """3D shape gradient: D3Q19 256 x 128 x 128, sphere of diameter 24 at (64, 63.5, 63.5) (off the
+    lattice in y and z, so that no node lies on the surface and a small radius change flips no node
+    between fluid and solid) with its radius
+    perturbed by 16 real spherical harmonics (l <= 3, all 0), Re 300 (tau 0.524), equilibrium inlet
+    u = 0.1 ramped (smoothstep) over 1000 steps, slip walls, 15% sponge, absorbing layer 0.02.
+    Forward in FP16 x 2^15 storage, adjoint in FP32 with FP64 dJ/dq; J = mean C_D over steps
+    2001..3000 (area pi r^2); checkpoints every 100 steps, the last segments recorded directly within
+    the device-memory budget (the first of the budgets that can be allocated). Finite differences
+    (h = 1e-3) use the FP32-storage forward, whose own adjoint gradient is reported alongside."""
+    from cells_amd import shapes
+    from cells_amd.adjoint import ShapeAdjoint, parameter_gradient
+    from cells_amd.lbm import Solver3D
+
+    W, H, D, u, T, window, K, r = 256, 128, 128, 0.1, 3000, 1000, 100, 12.0
+    tau = 3 * u * 2 * r / 300 + 0.5
+    area = np.pi * r * r
+    coeffs = np.zeros(16)
+
+    def sdf_of(b):
+        return shapes.sphere_sh(W, H, D, (64.0, 63.5, 63.5), r, b)
+
+    def cfg(precision):
+        return dict(width=W, height=H, depth=D, precision=precision, tau=tau, uIn=u, spongeFraction=0.15,
+                    spongeTau=1, absorb=0.02, forceEvery=1)
+
+    sdf0, dsdf = sdf_of(coeffs)
+    s = Solver3D.create(cfg("fp16"))
+    s.set_sdf(sdf0)
+    s.autotune(20)
+    s.set_inlet_ramp("smoothstep", u, 1000)
+    s.init_field()
+    s.sync()
+    t0 = time.perf_counter()
+    s.encode_steps(T)
+    s.sync()
+    t_plain = time.perf_counter() - t0
+    from cells_amd.lbm import LbmError
+
+    for gb in budget_gb:
+        adj = ShapeAdjoint(s, T, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=K,
+                           ramp=("smoothstep", u, 1000), record_budget_bytes=int(gb * (1 << 30)))
+        try:
+            adj.forward()
+            break
+        except LbmError:
+            adj.release()
+    info = adj.info()
+    t0 = time.perf_counter()
+    J = adj.forward()
+    t1 = time.perf_counter()
+    g_sdf = adj.backward()
+    t2 = time.perf_counter()
+    grad = parameter_gradient(g_sdf, dsdf)
+    adj.release()
+    s.destroy()
+    out = dict(grid=[W, H, D], steps=T, J=J, grad=grad.tolist(), info=info, plain_forward_s=t_plain,
+               adjoint_forward_s=t1 - t0, backward_s=t2 - t1, ratio=(t2 - t0) / t_plain,
+               backward_over_forward=(t2 - t1) / t_plain, record_budget_gb=gb)
+    if fd_coeffs:
+        s32 = Solver3D.create(cfg("fp32"))
+        a32 = ShapeAdjoint(s32, T, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=K,
+                           ramp=("smoothstep", u, 1000), record_budget_bytes=64 << 30)
+        out["J_fp32"] = a32.forward(sdf0)
+        g32 = parameter_gradient(a32.backward(), dsdf)
+        out["grad_fp32"] = g32.tolist()
+        a32.release()
+        h = 1e-3
+        fdv = {}
+        for p in fd_coeffs:
+            e = np.zeros(16)
+            e[p] = h
+            fdv[p] = (a32.forward(sdf_of(coeffs + e)[0]) - a32.forward(sdf_of(coeffs - e)[0])) / (2 * h)
+        s32.destroy()
+        scale = max(abs(v) for v in fdv.values())
+        out["fd_fp32"] = {str(p): v for p, v in fdv.items()}
+        # relative error over the components above 1% of the largest (the others are zero by the
+        # body's symmetry, where differences only see rounding), and the largest error over all of
+        # them relative to the largest component
+        big = [p for p, v in fdv.items() if abs(v) >= 0.01 * scale]
+        for name, g in (("", grad), ("fp32_adjoint_", g32)):
+            out[f"max_rel_err_{name}vs_fd"] = float(max(abs(g[p] - fdv[p]) / abs(fdv[p]) for p in big))
+            out[f"max_err_over_max_grad_{name}vs_fd"] = float(max(abs(g[p] - v) for p, v in fdv.items()) / scale)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +311,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="FP32 192x96x96 only (for profiling)")
     ap.add_argument("--sweep-steps", type=int, default=10000, help="steps of the 64-tunnel sweep (0 = skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D and 3D shape-gradient runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
@@ -171,6 +332,9 @@ def main():
     copy = {"fp32_192x96x96_gbs": copy_roofline(W * H * D, 19, 4), "fp16_192x96x96_gbs": copy_roofline(W * H * D, 19, 2),
             "fp32_256x128x128_gbs": copy_roofline(256 * 128 * 128, 19, 4)}
     sweep_res = sweep_bench(args.sweep_steps) if (args.sweep_steps > 0 and not args.quick) else None
+    adjoint = None
+    if not (args.quick or args.no_adjoint):
+        adjoint = {"naca_384x128": adjoint2d_case(), "sphere_256x128x128_fp16": adjoint3d_case()}
     cpu = None if args.no_cpu else cpu_baseline(W, H, D)
 
     achieved = main_run["kernel_achieved_gbs"]
@@ -191,7 +355,7 @@ def main():
                                "equilibrium inlet/outlet, sponge 15%, force sample every step",
                    "grid": [W, H, D], "cells": W * H * D, "bytes_per_cell": 156},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "step3_kernel<StoreF32>",
                      "bytes_per_launch": main_run["bytes_per_launch"],
                      "avg_launch_us": main_run["step_kernel_us"]},
@@ -205,6 +369,8 @@ def main():
         "bulk_block": main_run["bulk_block"],
         "copy_roofline": copy,
         "sweep": sweep_res,
+        "adjoint": adjoint,
+        "adjoint_over_forward": adjoint["sphere_256x128x128_fp16"]["ratio"] if adjoint else None,
     }
     print(json.dumps(line))
 

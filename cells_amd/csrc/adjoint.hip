@@ -78,34 +78,58 @@ struct AdjStep {
 // symmetric and antisymmetric halves. Moment part: the scalar Phi(drho, u) = sum over outputs of
 // lambda . (equilibrium and source terms) is differentiated in drho and u, then chained through
 // drho = sum f and u = j / rho: d/df_i = dPhi/ddrho + (c_i - u) . dPhi/du / rho.
-template <int Q, int NP>
+// Lattice accessors by population count (D2Q9 / D3Q19), folded at compile time in unrolled loops.
+template <int Q>
+__device__ __forceinline__ constexpr int cvec(int d, int i) {
+  return Q == 19 ? (d == 0 ? CX3[i] : d == 1 ? CY3[i] : CZ3[i]) : (d == 0 ? CX2[i] : d == 1 ? CY2[i] : 0);
+}
+template <int Q>
+__device__ __forceinline__ constexpr float wgt(int i) {
+  return Q == 19 ? W3[i] : W2[i];
+}
+// TRT pairs (a, b = opposite of a): D3Q19 (2k+1, 2k+2); D2Q9 (1,3), (2,4), (5,7), (6,8).
+template <int Q>
+__device__ __forceinline__ constexpr int pair_a(int k) {
+  return Q == 19 ? 2 * k + 1 : (k < 2 ? k + 1 : k + 3);
+}
+template <int Q>
+__device__ __forceinline__ constexpr int pair_b(int k) {
+  return Q == 19 ? 2 * k + 2 : (k < 2 ? k + 3 : k + 5);
+}
+template <int Q>
+__device__ __forceinline__ constexpr int opposite(int i) {
+  return Q == 19 ? opp3(i) : OPP2[i];
+}
+
+template <int Q>
 __device__ __forceinline__ void trt_vjp(const float (&lam)[Q], float drho, const float (&u)[3], float omP, float omM,
-                                        float sigma, float ut, const int (&PA)[NP], const int (&PB)[NP],
-                                        const int (&C)[3][Q], const float (&Wt)[Q], float (&mu)[Q]) {
+                                        float sigma, float ut, float (&mu)[Q]) {
+  constexpr int NP = (Q - 1) / 2;
   const float rho = 1.0f + drho;
   const float invRho = 1.0f / rho;
   const float u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   const float usq = 1.5f * u2;
   const float dut = 1.5f * (ut * ut - u2);
   mu[0] = (1.0f - omP) * lam[0];
-  float gd = omP * lam[0] * Wt[0] * (1.0f - usq) - lam[0] * sigma * Wt[0] * dut;
-  const float E = lam[0] * Wt[0] * (sigma - omP);
+  float gd = omP * lam[0] * wgt<Q>(0) * (1.0f - usq) - lam[0] * sigma * wgt<Q>(0) * dut;
+  const float E = lam[0] * wgt<Q>(0) * (sigma - omP);
   float P = 0.0f;
   float g[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int k = 0; k < NP; k++) {
-    const int a = PA[k], b = PB[k];
+    const int a = pair_a<Q>(k), b = pair_b<Q>(k);
     const float Ls = lam[a] + lam[b], La = lam[a] - lam[b];
     mu[a] = lam[a] - 0.5f * omP * Ls - 0.5f * omM * La;
     mu[b] = lam[b] - 0.5f * omP * Ls + 0.5f * omM * La;
-    const float cu = 3.0f * (float(C[0][a]) * u[0] + float(C[1][a]) * u[1] + float(C[2][a]) * u[2]);
-    const float cut = 3.0f * float(C[0][a]) * ut;
-    const float A = omP * Ls * Wt[a], B = omM * La * Wt[a], Cs = sigma * Ls * Wt[a], Ca = sigma * La * Wt[a];
+    const float cu = 3.0f * (float(cvec<Q>(0, a)) * u[0] + float(cvec<Q>(1, a)) * u[1] + float(cvec<Q>(2, a)) * u[2]);
+    const float cut = 3.0f * float(cvec<Q>(0, a)) * ut;
+    const float wa = wgt<Q>(a);
+    const float A = omP * Ls * wa, B = omM * La * wa, Cs = sigma * Ls * wa, Ca = sigma * La * wa;
     gd += A * (1.0f + 0.5f * cu * cu - usq) + B * cu + Cs * (0.5f * (cut * cut - cu * cu) - dut) + Ca * (cut - cu);
     P += A - Cs;
     const float e = (A - Cs) * cu + B - Ca;
 #pragma unroll
-    for (int d = 0; d < 3; d++) g[d] += float(C[d][a]) * e;
+    for (int d = 0; d < 3; d++) g[d] += float(cvec<Q>(d, a)) * e;
   }
   float gu[3];
 #pragma unroll
@@ -113,7 +137,8 @@ __device__ __forceinline__ void trt_vjp(const float (&lam)[Q], float drho, const
   const float ug = u[0] * gu[0] + u[1] * gu[1] + u[2] * gu[2];
 #pragma unroll
   for (int i = 0; i < Q; i++)
-    mu[i] += gd + (float(C[0][i]) * gu[0] + float(C[1][i]) * gu[1] + float(C[2][i]) * gu[2] - ug) * invRho;
+    mu[i] += gd + (float(cvec<Q>(0, i)) * gu[0] + float(cvec<Q>(1, i)) * gu[1] + float(cvec<Q>(2, i)) * gu[2] - ug) *
+                      invRho;
 }
 
 // mu = (d feq(1, u(f)) / df)^T lambda for the 3D outlet plane (step3d.ts:205-209).
@@ -161,32 +186,12 @@ __device__ __forceinline__ void zou_he_outlet_vjp(float (&nu)[9]) {
   nu[3] = nu[6] = nu[7] = 0.0f;
 }
 
-struct Lat2 {
-  static constexpr int Q = 9;
-  static constexpr int PA[4] = {1, 2, 5, 6};
-  static constexpr int PB[4] = {3, 4, 7, 8};
-  static constexpr int C[3][9] = {{0, 1, 0, -1, 0, 1, -1, -1, 1}, {0, 0, 1, 0, -1, 1, 1, -1, -1}, {0, 0, 0, 0, 0, 0, 0, 0, 0}};
-  static constexpr float Wt[9] = {4.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 9, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36};
-};
-struct Lat3 {
-  static constexpr int Q = 19;
-  static constexpr int PA[9] = {1, 3, 5, 7, 9, 11, 13, 15, 17};
-  static constexpr int PB[9] = {2, 4, 6, 8, 10, 12, 14, 16, 18};
-  static constexpr int C[3][19] = {{0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0},
-                                   {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1},
-                                   {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1}};
-  static constexpr float Wt[19] = {1.0f / 3,  1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18,
-                                   1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36,
-                                   1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36, 1.0f / 36};
-};
-
 // ---------------------------------------------------------------------------- adjoint list node
 // kind decides the local transpose; ut is the inlet speed seen by this node (the 2D parabolic profile
 // is evaluated by the caller).
-template <class L>
+template <int Q>
 __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep& as, const float4* __restrict__ xtab,
                                               uint32_t W, uint32_t N, uint32_t k, float ut) {
-  constexpr int Q = L::Q;
   const uint32_t idx = at.aidx[k];
   const uint32_t kind = at.akind[k];
   float lam[Q];
@@ -210,7 +215,7 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
     if constexpr (Q == 19) outlet3_vjp(lam, m.x, u, mu);
   } else {
     const float4 t = xtab[idx % W];
-    trt_vjp<Q>(lam, m.x, u, t.x, t.y, t.z, ut, L::PA, L::PB, L::C, L::Wt, mu);
+    trt_vjp<Q>(lam, m.x, u, t.x, t.y, t.z, ut, mu);
     if constexpr (Q == 9) {
       if (kind == AK_ZOU_IN) zou_he_inlet_vjp(mu, ut);
       else if (kind == AK_ZOU_OUT) zou_he_outlet_vjp(mu);
@@ -225,8 +230,8 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
 #pragma unroll
     for (int i = 1; i < Q; i++) {
       if (bm & (1u << i)) {
-        const int j = Q == 19 ? opp3(i) : OPP2[i];
-        const float cd = float(L::C[0][j]) * as.dx + float(L::C[1][j]) * as.dy + float(L::C[2][j]) * as.dz;
+        const int j = opposite<Q>(i);
+        const float cd = float(cvec<Q>(0, j)) * as.dx + float(cvec<Q>(1, j)) * as.dy + float(cvec<Q>(2, j)) * as.dz;
         const uint32_t l = base + __popc(bm & ((1u << i) - 1u));
         as.acc[l] += double(as.dq[l]) * double(mu[i] + as.w_cur * cd);
       }
@@ -241,7 +246,7 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
   const uint32_t N = p.N;
   if (blockIdx.z < zl) {
     const uint32_t k = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k < at.n_list) adj_list_node<Lat3>(at, as, xtab, uint32_t(p.W), N, k, as.ut);
+    if (k < at.n_list) adj_list_node<19>(at, as, xtab, uint32_t(p.W), N, k, as.ut);
     return;
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -262,7 +267,7 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
   const float u[3] = {m.y, m.z, m.w};
   const float4 t = xtab[x];
   float mu[19];
-  trt_vjp<19>(lam, m.x, u, t.x, t.y, t.z, as.ut, Lat3::PA, Lat3::PB, Lat3::C, Lat3::Wt, mu);
+  trt_vjp<19>(lam, m.x, u, t.x, t.y, t.z, as.ut, mu);
   char* const d = reinterpret_cast<char*>(as.mu);
 #pragma unroll
   for (int i = 0; i < 19; i++) *reinterpret_cast<float*>(d + size_t(i) * N * 4 + off) = mu[i];
@@ -282,7 +287,7 @@ __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, co
     const uint32_t k = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < at.n_list) {
       const int y = int(at.aidx[k] / uint32_t(p.W));
-      adj_list_node<Lat2>(at, as, xtab, uint32_t(p.W), N, k, inlet_profile2(y, as.ut, p));
+      adj_list_node<9>(at, as, xtab, uint32_t(p.W), N, k, inlet_profile2(y, as.ut, p));
     }
     return;
   }
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, co
   const float u[3] = {m.y, m.z, 0.0f};
   const float4 t = xtab[x];
   float mu[9];
-  trt_vjp<9>(lam, m.x, u, t.x, t.y, t.z, inlet_profile2(int(y), as.ut, p), Lat2::PA, Lat2::PB, Lat2::C, Lat2::Wt, mu);
+  trt_vjp<9>(lam, m.x, u, t.x, t.y, t.z, inlet_profile2(int(y), as.ut, p), mu);
 #pragma unroll
   for (int i = 0; i < 9; i++) as.mu[size_t(i) * N + idx] = mu[i];
 }
@@ -314,6 +319,7 @@ using namespace lbm;
 struct lbm_adjoint {
   lbm_adjoint_config cfg{};
   bool have_forward = false;
+  uint64_t forward_gen = 0;  // geometry (list tables generation) of the forward run
   int64_t T = 0, K = 0, nseg = 0, nstored = 0;  // segments [nseg - nstored, nseg) were recorded by the forward pass
   double J = 0;
   std::vector<double> w;  // objective weight of the force sample of step n (n = 0 .. T), w[T] = 0
@@ -332,6 +338,7 @@ struct lbm_adjoint {
   uint32_t n_list = 0;
   size_t n_ent = 0;
   int64_t n_regular = 0;
+  std::vector<int64_t> layout;  // sizes the device buffers were allocated for
 };
 
 namespace lbm {
@@ -363,6 +370,7 @@ void free_buffers(lbm_adjoint* a) {
   a->mu[0] = a->mu[1] = nullptr;
   a->acc = nullptr;
   a->have_forward = false;
+  a->layout.clear();
 }
 
 // One forward pull source: population index, weight and force weight (direction folded in).
@@ -545,11 +553,6 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   return LBM_OK;
 }
 
-int alloc_zero(void** p, size_t bytes, hipStream_t st) {
-  HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 4)));
-  HIP_TRY(hipMemsetAsync(*p, 0, std::max<size_t>(bytes, 4), st));
-  return LBM_OK;
-}
 
 // One reverse step n on the solver's stream.
 int adjoint_step(lbm_solver* s, lbm_adjoint* a, int64_t n, int slot, const float* mu_next, float* mu_cur) {
@@ -613,25 +616,35 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   if (int rc = check_supported(s)) return rc;
   if (!s->adj) s->adj = new lbm_adjoint();
   lbm_adjoint* a = s->adj;
-  free_buffers(a);
-  a->cfg = *cfg;
-  a->T = cfg->steps;
-  a->K = cfg->checkpoint_every;
-  a->nseg = (a->T + a->K - 1) / a->K;
+  const int64_t T = cfg->steps, K = cfg->checkpoint_every, nseg = (T + K - 1) / K;
   const size_t fbytes = size_t(s->Q) * s->N * s->elem;
-  const size_t seg_bytes = size_t(a->K) * s->N * sizeof(float4) + size_t(a->K) * std::max<uint32_t>(s->n_links, 1) * 4;
-  a->nstored = std::min<int64_t>(a->nseg, std::max<int64_t>(0, cfg->record_budget_bytes) / int64_t(seg_bytes));
-  a->ckpt.assign(size_t(a->nseg), nullptr);
-  for (int64_t g = 0; g < a->nseg - a->nstored; g++) HIP_TRY(hipMalloc(&a->ckpt[size_t(g)], fbytes));
-  // slots [0, nstored) hold the recorded segments for good; one more slot takes the recomputed ones
-  const int64_t nslots = a->nstored + (a->nstored < a->nseg ? 1 : 0);
-  a->rec_mom.assign(size_t(nslots), nullptr);
-  a->rec_dq.assign(size_t(nslots), nullptr);
-  for (int64_t g = 0; g < nslots; g++) {
-    HIP_TRY(hipMalloc(&a->rec_mom[size_t(g)], size_t(a->K) * s->N * sizeof(float4)));
-    HIP_TRY(hipMalloc(&a->rec_dq[size_t(g)], size_t(a->K) * std::max<uint32_t>(s->n_links, 1) * 4));
+  const size_t nlk = std::max<uint32_t>(s->n_links, 1);
+  const size_t seg_bytes = size_t(K) * s->N * sizeof(float4) + size_t(K) * nlk * 4;
+  const int64_t nstored = std::min<int64_t>(nseg, std::max<int64_t>(0, cfg->record_budget_bytes) / int64_t(seg_bytes));
+  // The device buffers are kept from one forward run to the next while their layout is unchanged
+  // (the usual case of finite differences or an optimisation loop over one tunnel).
+  const std::vector<int64_t> layout{T, K, nstored, int64_t(s->N), int64_t(fbytes), int64_t(nlk)};
+  a->have_forward = false;
+  if (layout != a->layout) {
+    free_buffers(a);
+    a->ckpt.assign(size_t(nseg), nullptr);
+    for (int64_t g = 0; g < nseg - nstored; g++) HIP_TRY(hipMalloc(&a->ckpt[size_t(g)], fbytes));
+    // slots [0, nstored) hold the recorded segments for good; one more slot takes the recomputed ones
+    const int64_t nslots = nstored + (nstored < nseg ? 1 : 0);
+    a->rec_mom.assign(size_t(nslots), nullptr);
+    a->rec_dq.assign(size_t(nslots), nullptr);
+    for (int64_t g = 0; g < nslots; g++) {
+      HIP_TRY(hipMalloc(&a->rec_mom[size_t(g)], size_t(K) * s->N * sizeof(float4)));
+      HIP_TRY(hipMalloc(&a->rec_dq[size_t(g)], size_t(K) * nlk * 4));
+    }
+    HIP_TRY(hipMalloc(&a->final_state, fbytes));
+    a->layout = layout;
   }
-  HIP_TRY(hipMalloc(&a->final_state, fbytes));
+  a->cfg = *cfg;
+  a->T = T;
+  a->K = K;
+  a->nseg = nseg;
+  a->nstored = nstored;
   // the run: from the initial field, with the configured inlet ramp and a force sample every step
   if (int rc = lbm_set_inlet_ramp(s, cfg->ramp_kind, cfg->ramp_u, cfg->ramp_steps)) return rc;
   // The 3D initial field is the equilibrium at the inlet speed of the first step (rest under a ramp
@@ -678,6 +691,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   HIP_TRY(hipStreamSynchronize(s->stream));
   a->J = cfg->scale * sum / double(count);
   a->have_forward = true;
+  a->forward_gen = s->tables_gen;
   *J = a->J;
   return LBM_OK;
 }
@@ -686,19 +700,19 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
   if (!s || !dJ_dsdf || n < s->N) return fail(LBM_ERR_INVALID, "dJ_dsdf must hold N doubles");
   lbm_adjoint* a = s->adj;
   if (!a || !a->have_forward) return fail(LBM_ERR_INVALID, "adjoint: call lbm_adjoint_forward first");
+  if (a->forward_gen != s->tables_gen)
+    return fail(LBM_ERR_INVALID, "adjoint: the geometry changed since the forward run; run lbm_adjoint_forward again");
   if (a->tables_gen != s->tables_gen)
     if (int rc = build_tables(s, a, a->cfg.direction)) return rc;
   const uint32_t N = s->N;
   const size_t fbytes = size_t(s->Q) * N * s->elem;
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
   for (float*& m : a->mu) {
-    if (m) HIP_TRY(hipFree(m));
-    m = nullptr;
-    if (int rc = alloc_zero(reinterpret_cast<void**>(&m), size_t(s->Q) * N * 4, s->stream)) return rc;
+    if (!m) HIP_TRY(hipMalloc(&m, size_t(s->Q) * N * 4));
+    HIP_TRY(hipMemsetAsync(m, 0, size_t(s->Q) * N * 4, s->stream));
   }
-  if (a->acc) HIP_TRY(hipFree(a->acc));
-  a->acc = nullptr;
-  if (int rc = alloc_zero(reinterpret_cast<void**>(&a->acc), nlk * 8, s->stream)) return rc;
+  if (!a->acc) HIP_TRY(hipMalloc(&a->acc, nlk * 8));
+  HIP_TRY(hipMemsetAsync(a->acc, 0, nlk * 8, s->stream));
   // weights scaled by the upstream gradient
   const std::vector<double> w0 = a->w;
   for (double& x : a->w) x *= seed;

@@ -140,8 +140,9 @@ class Tunnel2D:
         F_next = torch.where(self.fluid[None], out, F)
         return F_next, fx, fy
 
-    def objective(self, sdf, steps, window, u, ramp_steps, L, segment=100):
-        """J = mean of 2 F_x / (u^2 L) over the samples of steps (steps - window, steps]; ramp smoothstep."""
+    def objective(self, sdf, steps, window, u, ramp_steps, L, segment=100, ramp_kind="smoothstep"):
+        """J = mean of 2 F_x / (u^2 L) over the samples of steps (steps - window, steps]; inlet ramp
+        u r(min(1, k / ramp_steps)) with r(t) = t^2 (3 - 2t) (smoothstep) or t (linear)."""
         H, W = self.H, self.W
         F = _feq(torch.ones(H, W, dtype=torch.float64), torch.zeros(H, W, dtype=torch.float64),
                  torch.zeros(H, W, dtype=torch.float64))
@@ -150,7 +151,7 @@ class Tunnel2D:
             acc = torch.zeros((), dtype=torch.float64)
             for k in range(k0, k1):
                 r = min(1.0, k / ramp_steps) if ramp_steps > 0 else 1.0
-                uin = u * r * r * (3 - 2 * r)
+                uin = u * r * r * (3 - 2 * r) if ramp_kind == "smoothstep" else u * r
                 F, fx, _ = self.step(F, sdf, uin)
                 if k + 1 > steps - window:
                     acc = acc + fx
@@ -164,10 +165,42 @@ class Tunnel2D:
         return 2 * total / window / (u * u * L), F
 
 
-def reference_gradient(cfg, sdf, steps, window, u, ramp_steps, L, segment=100):
-    """(J, dJ/dsdf [N]) in float64 on the CPU."""
+def reference_gradient(cfg, sdf, steps, window, u, ramp_steps, L, segment=20, ramp_kind="smoothstep"):
+    """(J, dJ/dsdf [N]) in float64 on the CPU: a forward pass without a graph that keeps the state at
+    the start of every segment, then per segment in reverse a re-run with autograd and the
+    vector-Jacobian product of (its share of J + gF . F_end) for (F_start, sdf)."""
     t = Tunnel2D(cfg, sdf)
-    s = torch.tensor(np.asarray(sdf, dtype=np.float32).astype(np.float64).reshape(t.H, t.W), requires_grad=True)
-    J, _ = t.objective(s, steps, window, u, ramp_steps, L, segment)
-    J.backward()
-    return float(J), s.grad.numpy().reshape(-1).copy()
+    H, W = t.H, t.W
+    s = torch.tensor(np.asarray(sdf, dtype=np.float32).astype(np.float64).reshape(H, W), requires_grad=True)
+    scale = 2.0 / window / (u * u * L)
+
+    def u_in(k):
+        r = min(1.0, k / ramp_steps) if ramp_steps > 0 else 1.0
+        return u * r * r * (3 - 2 * r) if ramp_kind == "smoothstep" else u * r
+
+    F = _feq(torch.ones(H, W, dtype=torch.float64), torch.zeros(H, W, dtype=torch.float64),
+             torch.zeros(H, W, dtype=torch.float64))
+    starts, total = [], 0.0
+    with torch.no_grad():
+        for k0 in range(0, steps, segment):
+            starts.append(F.clone())
+            for k in range(k0, min(steps, k0 + segment)):
+                F, fx, _ = t.step(F, s, u_in(k))
+                if k + 1 > steps - window:
+                    total += float(fx)
+    gF = torch.zeros_like(F)
+    g_sdf = torch.zeros_like(s)
+    for i in reversed(range(len(starts))):
+        k0 = i * segment
+        Fs = starts[i].clone().requires_grad_(True)
+        F = Fs
+        acc = torch.zeros((), dtype=torch.float64)
+        for k in range(k0, min(steps, k0 + segment)):
+            F, fx, _ = t.step(F, s, u_in(k))
+            if k + 1 > steps - window:
+                acc = acc + fx
+        gFs, gs = torch.autograd.grad(scale * acc + (gF * F).sum(), [Fs, s], allow_unused=True)
+        gF = gFs
+        if gs is not None:
+            g_sdf += gs
+    return scale * total, g_sdf.numpy().reshape(-1).copy()
