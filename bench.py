@@ -69,7 +69,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     s.set_sdf(tunnel_sdf(W, H, D))
     s.init_field()
     s.set_inlet_ramp("smoothstep", 0.1, 3000)
-    block = s.autotune(40)
+    block = s.autotune(200)
     if warmup:
         s.time_steps(warmup)
     s.sync()
@@ -83,7 +83,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     out["roofline_frac"] = out["algorithmic_gbs"] / HBM_PEAK_GBS
     nb, nl, ns = s.node_counts()
     out["nodes"] = dict(bulk=nb, list=nl, solid=ns)
-    out["bulk_block"] = block
+    out["bulk_block"], out["alternating_rows"] = s.launch_info()
     if profile:
         # one kernel launch per step: HIP events after every step on the solver's stream
         tot, tsum, tmax = s.profile_steps(steps)
@@ -95,6 +95,41 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     out["last_force"] = list(forces[-1][1:]) if forces else None
     out["finite"] = bool(np.isfinite(s.read_populations()).all())
     s.destroy()
+    return out
+
+
+PRECISION_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                 "c1_sphere_192x96x96_fp64_forces.npy")
+
+
+def precision_study(steps=4000):
+    """C1 tunnel from the initial field for 4000 steps in FP32, FP16 x 2^15 and BF16 storage: C_D
+    (2 F_x / (u^2 pi 8^2)) against the FP64 run of the same tunnel (committed force trace of the
+    oracle in double precision, tests/golden/make_c1_fp64_forces.py) at identical steps."""
+    from cells_amd.lbm import Solver3D
+
+    W, H, D, u = 192, 96, 96, 0.1
+    scale = 2 / (u * u * np.pi * 8 * 8)
+    ref = np.load(PRECISION_FIXTURE) if os.path.exists(PRECISION_FIXTURE) else None
+    out = {}
+    for prec in ("fp32", "fp16", "bf16"):
+        s = Solver3D.create(tunnel_config(W, H, D, prec))
+        s.set_sdf(tunnel_sdf(W, H, D))
+        s.init_field()
+        s.set_inlet_ramp("smoothstep", u, 3000)
+        s.encode_steps(steps)
+        st, v = s.read_forces_raw()
+        s.destroy()
+        cd = scale * v[:, 0]
+        r = dict(cd_last=float(cd[-1]), cd_mean_last1000=float(cd[-1000:].mean()))
+        if ref is not None and len(ref) == steps and np.array_equal(st, np.arange(1, steps + 1)):
+            cd64 = scale * ref[:, 0]
+            r["cd64_last"] = float(cd64[-1])
+            r["cd64_mean_last1000"] = float(cd64[-1000:].mean())
+            r["rel_err_last"] = float(abs(cd[-1] - cd64[-1]) / abs(cd64[-1]))
+            r["rel_err_mean_last1000"] = float(abs(cd[-1000:].mean() - cd64[-1000:].mean()) / abs(cd64[-1000:].mean()))
+            r["max_abs_err_last1000_over_mean"] = float(np.abs(cd[-1000:] - cd64[-1000:]).max() / abs(cd64[-1000:].mean()))
+        out[prec] = r
     return out
 
 
@@ -200,6 +235,15 @@ def adjoint2d_case(fd=True):
     out = dict(grid=[W, H], steps=T, J=J, grad=grad.tolist(), info=info, plain_forward_s=t_plain,
                adjoint_forward_s=t1 - t0, backward_s=t2 - t1, ratio=(t2 - t0) / t_plain,
                backward_over_forward=(t2 - t1) / t_plain)
+    # the FP64 reverse-mode reference of the same case (committed fixture, CPU torch in float64)
+    try:
+        ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                          "adjoint_naca_384x128_fp64.json")))
+        g_ref = np.array(ref["grad_theta"])
+        out["J_fp64"] = ref["J"]
+        out["max_rel_err_vs_fp64"] = float(np.max(np.abs(grad - g_ref) / np.abs(g_ref)))
+    except (OSError, KeyError, ValueError):
+        pass
     if fd:
         # central differences at h = 1e-3 and, because the smallest components are then within a
         # few FP32 roundings of J, also at h = 1e-2
@@ -332,6 +376,7 @@ def main():
     copy = {"fp32_192x96x96_gbs": copy_roofline(W * H * D, 19, 4), "fp16_192x96x96_gbs": copy_roofline(W * H * D, 19, 2),
             "fp32_256x128x128_gbs": copy_roofline(256 * 128 * 128, 19, 4)}
     sweep_res = sweep_bench(args.sweep_steps) if (args.sweep_steps > 0 and not args.quick) else None
+    precision = None if args.quick else precision_study()
     adjoint = None
     if not (args.quick or args.no_adjoint):
         adjoint = {"naca_384x128": adjoint2d_case(), "sphere_256x128x128_fp16": adjoint3d_case()}
@@ -363,12 +408,13 @@ def main():
         "cpu_baseline": cpu,
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "step_kernel_us", "slowest_step_us", "kernel_achieved_gbs",
-                                               "bulk_block", "nodes", "finite") if kk in v}
+                                               "bulk_block", "alternating_rows", "nodes", "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],
         "bulk_block": main_run["bulk_block"],
         "copy_roofline": copy,
         "sweep": sweep_res,
+        "precision_vs_fp64": precision,
         "adjoint": adjoint,
         "adjoint_over_forward": adjoint["sphere_256x128x128_fp16"]["ratio"] if adjoint else None,
     }

@@ -81,7 +81,7 @@ void free_all(lbm_solver* s) {
   for (void* p : {s->f[0], s->f[1], (void*)s->flags, (void*)s->touches, (void*)s->sdf, (void*)s->mac,
                   (void*)s->list, (void*)s->partials, (void*)s->counter, (void*)s->ring, (void*)s->state,
                   (void*)s->xtab, (void*)s->gtab, (void*)s->bmask, (void*)s->xmask, (void*)s->lbase,
-                  (void*)s->links})
+                  (void*)s->links, (void*)s->row_clean})
     if (p) (void)hipFree(p);
   for (auto& kv : s->tickets) {
     if (kv.second.ring) (void)hipHostFree(kv.second.ring);
@@ -400,6 +400,27 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
     HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
   }
   if (list_marks) HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s->stream));
+  if (!d2) {
+    // rows whose flag words are exactly what their position implies: inlet at x = 0, plain fluid
+    // inside, the outlet at x = W - 1 (a bulk node off the faces, a list node on them)
+    const uint32_t W = uint32_t(s->W), H = uint32_t(s->H), D = uint32_t(s->D);
+    std::vector<uint8_t> clean(size_t(H) * D, 0);
+    for (uint32_t z = 0; z < D; z++) {
+      for (uint32_t y = 0; y < H; y++) {
+        const Row3 r = row3(s, y, z);
+        const bool face = r.loY || r.hiY || r.loZ || r.hiZ;
+        const uint32_t* row = fl.data() + size_t(W) * (y + size_t(H) * z);
+        bool ok = W >= 3 && row[0] == (FLAG_INLET3 | INLET_DIRS3 | r.mask);
+        for (uint32_t x = 1; x + 1 < W && ok; x++) ok = row[x] == r.mask;
+        if (ok) {
+          const uint32_t last = row[W - 1];
+          ok = face ? (bulk_kind3(r, last) == 0 && !(last & FLAG_SOLID3)) : last == (FLAG_OUTLET3 | OUTLET_DIRS3);
+        }
+        clean[y + size_t(H) * z] = ok ? 1 : 0;
+      }
+    }
+    HIP_TRY(hipMemcpyAsync(s->row_clean, clean.data(), clean.size(), hipMemcpyHostToDevice, s->stream));
+  }
   HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
@@ -466,6 +487,10 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
   HIP_TRY(hipMalloc(&s->state, 4));
   HIP_TRY(hipMalloc(&s->xtab, sizeof(float4) * size_t(s->W)));
+  if (s->nd == 3) {
+    HIP_TRY(hipMalloc(&s->row_clean, size_t(s->H) * size_t(s->D)));
+    HIP_TRY(hipMemsetAsync(s->row_clean, 0, size_t(s->H) * size_t(s->D), s->stream));
+  }
   HIP_TRY(hipMemsetAsync(s->f[0], 0, fbytes, s->stream));
   HIP_TRY(hipMemsetAsync(s->f[1], 0, fbytes, s->stream));
   HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * N, s->stream));
@@ -510,7 +535,11 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
     float fin[19];
     inlet_feq3(s->p3.uIn, fin);
     const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
-    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, s->stream, rec);
+    // row order of this step: which buffer is walked top-down matters (measured), so both
+    // alternations are candidates of the autotuner
+    const bool descend = s->sweep == 1 ? (src > dst) : s->sweep == 2 ? (src < dst) : false;
+    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, descend, s->row_clean,
+                 s->stream, rec);
   }
   s->parity ^= 1;
   s->step++;
@@ -940,23 +969,25 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
   return LBM_OK;
 }
 
-// Autotuner for the bulk kernel's block width: each distinct width the launcher can use for this W
-// is timed over trial_steps steps (after one warm-up step) on a copy of the solver's own state, the
-// fastest is kept, and the populations, step, parity and force ring are restored. Results are cached
-// per (lattice, precision, W, H, D) for the life of the process; the list kernel keeps 256 threads,
-// which fixes the order of the FP64 force summation.
+// Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
+// for this W (and, in 3D, both row orders: alternating or always bottom-up) is timed over
+// trial_steps steps (after one warm-up step) on a copy of the solver's own state, the fastest is
+// kept, and the populations, step, parity and force ring are restored. Results are cached per
+// (lattice, precision, W, H, D) for the life of the process. None of the choices changes a result:
+// cells are independent within a step and the force sum is ordered by node, not by block.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
   static std::mutex cache_mu;
-  static std::map<std::string, int> cache;
+  static std::map<std::string, std::pair<int, int>> cache;
   const std::string key = std::to_string(s->nd) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
                           std::to_string(s->H) + "x" + std::to_string(s->D);
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      s->bulk_block = it->second;
-      if (chosen) *chosen = it->second;
+      s->bulk_block = it->second.first;
+      s->sweep = it->second.second;
+      if (chosen) *chosen = s->bulk_block;
       return LBM_OK;
     }
   }
@@ -974,26 +1005,44 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (s->nd == 2) s->cfg2.force_every = 0;
   else s->cfg3.force_every = 0;
   std::vector<int> tried;
-  int best = s->bulk_block;
+  int best = s->bulk_block, best_sweep = s->sweep;
   double best_ms = 1e300;
   hipEvent_t a, b;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
+  // every candidate is timed twice, interleaved with the others, and keeps its faster time
+  std::vector<std::pair<int, int>> cand;
   for (int req = 64; req <= 1024; req += 64) {
     const int bx = int(bulk_block_x(s->W, req));
     if (std::find(tried.begin(), tried.end(), bx) != tried.end()) continue;
     tried.push_back(bx);
-    s->bulk_block = bx;
-    enqueue_one(s);
-    HIP_TRY(hipEventRecord(a, s->stream));
-    for (int k = 0; k < trial_steps; k++) enqueue_one(s);
-    HIP_TRY(hipEventRecord(b, s->stream));
-    HIP_TRY(hipEventSynchronize(b));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, a, b));
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = bx;
+    if (s->nd == 3) {
+      for (int sw : {1, 2, 0}) cand.push_back({bx, sw});
+    } else {
+      cand.push_back({bx, s->sweep});
+    }
+  }
+  std::vector<double> best_of(cand.size(), 1e300);
+  for (int rep = 0; rep < 2; rep++) {
+    for (size_t c = 0; c < cand.size(); c++) {
+      s->bulk_block = cand[c].first;
+      s->sweep = cand[c].second;
+      enqueue_one(s);
+      enqueue_one(s);
+      HIP_TRY(hipEventRecord(a, s->stream));
+      for (int k = 0; k < trial_steps; k++) enqueue_one(s);
+      HIP_TRY(hipEventRecord(b, s->stream));
+      HIP_TRY(hipEventSynchronize(b));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, a, b));
+      best_of[c] = std::min(best_of[c], double(ms));
+    }
+  }
+  for (size_t c = 0; c < cand.size(); c++) {
+    if (best_of[c] < best_ms) {
+      best_ms = best_of[c];
+      best = cand[c].first;
+      best_sweep = cand[c].second;
     }
   }
   (void)hipEventDestroy(a);
@@ -1009,11 +1058,19 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   s->parity = parity0;
   s->force_steps.resize(nsteps);
   s->bulk_block = best;
+  s->sweep = best_sweep;
   {
     std::lock_guard<std::mutex> lk(cache_mu);
-    cache[key] = best;
+    cache[key] = {best, best_sweep};
   }
   if (chosen) *chosen = best;
+  return LBM_OK;
+}
+
+int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (block) *block = int32_t(bulk_block_x(s->W, s->bulk_block));
+  if (sweep) *sweep = s->nd == 3 ? s->sweep : 0;
   return LBM_OK;
 }
 

@@ -27,23 +27,31 @@ __device__ __forceinline__ float feq3(int i, float rho, float ux, float uy, floa
 
 struct Mom3 {
   float drho, rho, invRho, ux, uy, uz, usq;
+  float s[9], d[9];  // f_a + f_b and f_a - f_b of the TRT pairs (a, b) = (2k + 1, 2k + 2)
 };
 
-// src/shaders/step3d.ts:38-45: sums in index order.
+// Moments of src/shaders/step3d.ts:38-45, summed through the pair sums and differences, which the
+// collision reuses: drho = f_0 + sum_k s_k, j = sum_k c_(2k+1) d_k.
 __device__ __forceinline__ Mom3 moments3(const float (&f)[19]) {
   Mom3 m;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    m.s[k] = f[2 * k + 1] + f[2 * k + 2];
+    m.d[k] = f[2 * k + 1] - f[2 * k + 2];
+  }
   float d = f[0];
 #pragma unroll
-  for (int i = 1; i < 19; i++) d = d + f[i];
+  for (int k = 0; k < 9; k++) d = d + m.s[k];
   float jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
-  for (int i = 1; i < 19; i++) {
-    if (CX3[i] > 0) jx = jx + f[i];
-    if (CX3[i] < 0) jx = jx - f[i];
-    if (CY3[i] > 0) jy = jy + f[i];
-    if (CY3[i] < 0) jy = jy - f[i];
-    if (CZ3[i] > 0) jz = jz + f[i];
-    if (CZ3[i] < 0) jz = jz - f[i];
+  for (int k = 0; k < 9; k++) {
+    const int a = 2 * k + 1;
+    if (CX3[a] > 0) jx = jx + m.d[k];
+    if (CX3[a] < 0) jx = jx - m.d[k];
+    if (CY3[a] > 0) jy = jy + m.d[k];
+    if (CY3[a] < 0) jy = jy - m.d[k];
+    if (CZ3[a] > 0) jz = jz + m.d[k];
+    if (CZ3[a] < 0) jz = jz - m.d[k];
   }
   m.drho = d;
   m.rho = 1.0f + d;
@@ -79,9 +87,9 @@ __device__ __forceinline__ Relax relax3(float x, const Params3& p) {
 }
 
 // TRT collision with sponge, Smagorinsky and the outlet absorbing layer: src/shaders/step3d.ts:47-102,
-// 210-222. Writes the post-collision populations to out.
-__device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, const Relax& r, const Params3& p,
-                                         float (&out)[19]) {
+// 210-222, term by term as the reference writes it. Writes the post-collision populations to out.
+__device__ __forceinline__ void collide3_general(const float (&f)[19], const Mom3& m, const Relax& r, const Params3& p,
+                                                 float (&out)[19]) {
   float cu[19], eqS[19], eqA[19];
 #pragma unroll
   for (int a = 1; a < 19; a += 2) {
@@ -152,6 +160,63 @@ __device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, co
   }
 }
 
+// The same collision without the subgrid model, regrouped per pair around the pair sums s and
+// differences d of the moments (f_a = (s + d) / 2, f_b = (s - d) / 2). With e = omega+ eqS and
+// m = omega- eqA (src/shaders/step3d.ts:93-101):
+//   out_a = f_a - sym - asym = A + B,  out_b = f_b - sym + asym = A - B,
+//   A = (1/2 - omega+/2) s + e,  B = (1/2 - omega-/2) d + m,
+//   e = P0 + P2 cu^2, P0 = omega+ w (drho - rho usq), P2 = omega+ w rho / 2,  m = Q cu, Q = omega- w rho,
+// the coefficients depending only on the pair's weight class. The absorbing-layer sources are added
+// where sigma > 0 with collide3_general's formulas. With the subgrid model (SMAG, chosen on the host
+// from smagC2 > 0, so that kernels without it do not carry its registers) collide3_general runs.
+template <bool SMAG>
+__device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, const Relax& r, const Params3& p,
+                                         float (&out)[19]) {
+  if constexpr (SMAG) {
+    collide3_general(f, m, r, p, out);
+    return;
+  }
+  const float omP = r.omBase, omM = r.omM;
+  const float base = m.drho - m.rho * m.usq;
+  const float cs = 0.5f - 0.5f * omP, cd = 0.5f - 0.5f * omM;
+  const float P0a = omP * W3[1] * base, P0d = omP * W3[7] * base;
+  const float P2a = 0.5f * omP * W3[1] * m.rho, P2d = 0.5f * omP * W3[7] * m.rho;
+  const float Qa = omM * W3[1] * m.rho, Qd = omM * W3[7] * m.rho;
+  const float u3x = 3.0f * m.ux, u3y = 3.0f * m.uy, u3z = 3.0f * m.uz;
+  out[0] = f[0] - omP * (f[0] - W3[0] * base);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int a = 2 * k + 1;
+    float c = 0.0f;
+    if (CX3[a] > 0) c = c + u3x;
+    if (CX3[a] < 0) c = c - u3x;
+    if (CY3[a] > 0) c = c + u3y;
+    if (CY3[a] < 0) c = c - u3y;
+    if (CZ3[a] > 0) c = c + u3z;
+    if (CZ3[a] < 0) c = c - u3z;
+    const bool axis = k < 3;
+    const float A = cs * m.s[k] + ((axis ? P2a : P2d) * (c * c) + (axis ? P0a : P0d));
+    const float B = cd * m.d[k] + (axis ? Qa : Qd) * c;
+    out[a] = A + B;
+    out[a + 1] = A - B;
+  }
+  if (r.sigma > 0.0f) {
+    const float sigma = r.sigma, ut = p.uIn;
+    const float dq = 1.5f * m.rho * (ut * ut - m.ux * m.ux - m.uy * m.uy - m.uz * m.uz);
+    out[0] += sigma * W3[0] * -dq;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int a = 2 * k + 1;
+      const float cu = 3.0f * (float(CX3[a]) * m.ux + float(CY3[a]) * m.uy + float(CZ3[a]) * m.uz);
+      const float cut = 3.0f * float(CX3[a]) * ut;
+      const float sS = sigma * W3[a] * (0.5f * m.rho * (cut * cut - cu * cu) - dq);
+      const float sA = sigma * W3[a] * m.rho * (cut - cu);
+      out[a] += sS + sA;
+      out[a + 1] += sS - sA;
+    }
+  }
+}
+
 // Inlet equilibrium feq(i, 1, (uIn, 0, 0)) for all i, computed on the host per step (it is uniform
 // over the inlet plane) and passed by value.
 struct FeqIn {
@@ -170,7 +235,7 @@ struct FeqIn {
 // node's flag word must be exactly the face bits of its row (the host moves nodes whose pulls
 // differ, e.g. where the mirror node is solid, to the boundary list and marks them FLAG_LIST3).
 // Outlet nodes of face rows go to the boundary list as well.
-template <class S, bool REC>
+template <class S, bool REC, bool SMAG>
 __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
                                            uint32_t nbb, const Params3& p, const FeqIn& fin,
@@ -180,22 +245,28 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
 // One time step in one launch. Grid (ceil(W / blockDim.x), H, zl + D): the first zl z-slices are
 // boundary-list blocks (dispatched first, so their dependent gathers overlap the streaming), the
 // rest are the bulk rows. REC additionally writes the adjoint's per-step record.
-template <class S, bool REC>
+//
+// descend walks the rows in reverse order (z and y from the top). Alternating the direction every
+// step makes a step start on the rows the previous step wrote last, which are still in the 256 MB
+// Infinity Cache when the two population buffers do not fit in it together.
+template <class S, bool REC, bool SMAG>
 __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
                                                      const float4* __restrict__ xtab, Params3 p, FeqIn fin,
                                                      ListTables lt, uint32_t zl, uint32_t nlb, uint32_t nbb,
-                                                     ForceSink fs, Record rc) {
+                                                     ForceSink fs, Record rc, uint32_t descend,
+                                                     const uint8_t* __restrict__ row_clean) {
   using E = typename S::elem;
   if (blockIdx.z < zl) {
     const uint32_t lb = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    if (lb < nlb + nbb) list_part3<S, REC>(src, dst, flags, lt, nlb, nbb, p, fin, xtab, fs, rc, lb);
+    if (lb < nlb + nbb) list_part3<S, REC, SMAG>(src, dst, flags, lt, nlb, nbb, p, fin, xtab, fs, rc, lb);
     return;
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  const uint32_t y = blockIdx.y, z = blockIdx.z - zl;
+  const uint32_t y = descend ? uint32_t(p.H) - 1u - blockIdx.y : blockIdx.y;
+  const uint32_t z = descend ? uint32_t(p.D) - 1u - (blockIdx.z - zl) : blockIdx.z - zl;
   const bool loY = y == 0u, hiY = y == uint32_t(p.H - 1), loZ = z == 0u, hiZ = z == uint32_t(p.D - 1);
   const bool face = loY || hiY || loZ || hiZ;
   uint32_t M = 0u;  // face bits of this row
@@ -206,7 +277,14 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     if (outY || outZ) M |= 1u << i;
   }
   const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
-  const uint32_t fl = flags[idx];
+  // A clean row (no solid or boundary-list node besides the outlet node of a face row) is known
+  // from its position alone, so its flag words are not read.
+  uint32_t fl;
+  if (row_clean[y + uint32_t(p.H) * z])
+    fl = x == 0u ? (FLAG_INLET3 | INLET_DIRS3 | M)
+                 : x == uint32_t(p.W - 1) ? (face ? FLAG_LIST3 : (FLAG_OUTLET3 | OUTLET_DIRS3)) : M;
+  else
+    fl = flags[idx];
   const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3 | M);
   const bool outlet = !face && fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
   if (fl != M && !inlet && !outlet) return;  // handled by the boundary list (or solid)
@@ -261,7 +339,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   r.sigma = t.z;
   r.t0 = t.w;
   float out[19];
-  collide3(f, m, r, p, out);
+  collide3<SMAG>(f, m, r, p, out);
 #pragma unroll
   for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, out[i]);
 }
@@ -280,7 +358,7 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
 // replaces the gathered f_j with the interpolation against its record's second operand and adds
 // c_j (f_j + f_i) to the force; a slip face mirroring a solid node (step3d.ts:139-146) keeps f_j and
 // adds c_j 2 f_j.
-template <class S, bool BODY, bool REC>
+template <class S, bool BODY, bool REC, bool SMAG>
 __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
                                            const Params3& p, const FeqIn& fin, const float4* __restrict__ xtab,
@@ -354,7 +432,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   r.sigma = t.z;
   r.t0 = t.w;
   float out[19];
-  collide3(f, m, r, p, out);
+  collide3<SMAG>(f, m, r, p, out);
 #pragma unroll
   for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
 }
@@ -362,7 +440,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
 // Boundary-list part of the step: list block lb of the launch. List blocks [0, nlb) take the nodes
 // without body links; the nbb blocks after them take the nodes with body links and carry the force
 // sample (an FP64 reduction over those blocks).
-template <class S, bool REC>
+template <class S, bool REC, bool SMAG>
 __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
                                            uint32_t nbb, const Params3& p, const FeqIn& fin,
@@ -371,11 +449,11 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
   float force[3] = {0.0f, 0.0f, 0.0f};
   if (lb < nlb) {
     const uint32_t k = lb * blockDim.x + threadIdx.x;
-    if (k < lt.n_link) list_node3<S, false, REC>(src, dst, flags, lt, k, p, fin, xtab, rc, force);
+    if (k < lt.n_link) list_node3<S, false, REC, SMAG>(src, dst, flags, lt, k, p, fin, xtab, rc, force);
     return;
   }
   const uint32_t kb = (lb - nlb) * blockDim.x + threadIdx.x;
-  if (kb < lt.n_body) list_node3<S, true, REC>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, rc, force);
+  if (kb < lt.n_body) list_node3<S, true, REC, SMAG>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, rc, force);
   if (fs.sample) force_sample<3>(force, fs, lb - nlb, nbb);
 }
 
@@ -510,8 +588,8 @@ uint32_t bulk_block_x(int W, int requested) {
 
 template <class S>
 static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt, const Params3& p,
-                    const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, hipStream_t st,
-                    const Record* rec) {
+                    const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, bool descend,
+                    const uint8_t* row_clean, hipStream_t st, const Record* rec) {
   using E = typename S::elem;
   FeqIn fin;
   for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
@@ -522,21 +600,27 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const Lis
   const uint32_t per_slice = gx * uint32_t(p.H);
   const uint32_t zl = (nlb + nbb + per_slice - 1) / per_slice;
   const dim3 grid(gx, uint32_t(p.H), zl + uint32_t(p.D));
-  if (rec)
-    step3_kernel<S, true><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin,
-                                                lt, zl, nlb, nbb, fs, *rec);
+  const uint32_t desc = descend ? 1u : 0u;
+  const E* s0 = static_cast<const E*>(src);
+  E* d0 = static_cast<E*>(dst);
+  if (rec)  // recording steps (the adjoint) never use the subgrid model
+    step3_kernel<S, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec, desc,
+                                                       row_clean);
+  else if (p.smagC2 > 0.0f)
+    step3_kernel<S, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                       Record{nullptr, nullptr}, desc, row_clean);
   else
-    step3_kernel<S, false><<<grid, bx, 0, st>>>(static_cast<const E*>(src), static_cast<E*>(dst), flags, xtab, p, fin,
-                                                 lt, zl, nlb, nbb, fs, Record{nullptr, nullptr});
+    step3_kernel<S, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                        Record{nullptr, nullptr}, desc, row_clean);
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  hipStream_t st, const Record* rec) {
+                  bool descend, const uint8_t* row_clean, hipStream_t st, const Record* rec) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
-    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, st, rec); break;
+    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
+    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
   }
 }
 

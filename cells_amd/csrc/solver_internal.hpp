@@ -64,6 +64,9 @@ struct lbm_solver {
   lbm::Params3 p3{};
   lbm::Ramp ramp;
   int bulk_block = 256;
+  // 3D row order (autotuned): 0 = always bottom-up; 1 / 2 = top-down on the steps that read the
+  // buffer at the higher / lower address, bottom-up on the others
+  int sweep = 1;
 
   void* f[2] = {nullptr, nullptr};
   uint32_t* flags = nullptr;
@@ -80,6 +83,7 @@ struct lbm_solver {
   lbm::BzLink* links = nullptr;     // Bouzidi records
   uint32_t n_links = 0;
   std::vector<float> sdf_host;      // the SDF as last set (q is computed from it on the host)
+  uint8_t* row_clean = nullptr;     // 3D: per row (y + H z), 1 when no flag word of it needs reading
   // Host copies of the boundary-list tables and the flag words as last built, from which the
   // discrete adjoint builds its transposed tables; bumped generation on every rebuild.
   std::vector<uint32_t> h_flags, h_list, h_gtab, h_bmask, h_xmask, h_lbase;

@@ -160,6 +160,7 @@ def load_library(path: str = LIB_PATH):
                               C.c_int),
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+        "lbm_launch_info": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
         "lbm_adjoint_backward": ([P, C.c_double, C.c_void_p, C.c_size_t], C.c_int),
@@ -303,6 +304,12 @@ class _Base:
         c = C.c_int32()
         _check(_lib.lbm_autotune(self._h, int(trial_steps), C.byref(c)))
         return int(c.value)
+
+    def launch_info(self):
+        """(threads per bulk block, 3D row order: 0 fixed, 1 / 2 alternating) in use."""
+        b, w = C.c_int32(), C.c_int32()
+        _check(_lib.lbm_launch_info(self._h, C.byref(b), C.byref(w)))
+        return int(b.value), int(w.value)
 
     def node_counts(self):
         """(bulk, list, solid) node counts."""

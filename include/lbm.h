@@ -167,10 +167,15 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n);
 int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
 int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms);
 /* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
- * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid on a copy of the
- * solver's state, keeps the fastest (cached per lattice, precision and grid size for the process),
- * and restores populations, step and force ring. chosen (may be NULL) receives the width. */
+ * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid (in 3D, each with
+ * the row order alternating between steps and fixed) on a copy of the solver's state, keeps the
+ * fastest (cached per lattice, precision and grid size for the process), and restores populations,
+ * step and force ring. chosen (may be NULL) receives the width. No choice changes results. */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
+/* No counterpart in the reference: the launch shape in use, threads per bulk block and (3D) the row
+ * order: 0 always bottom-up; 1 (2) top-down on the steps that read the population buffer at the
+ * higher (lower) address, bottom-up on the others. */
+int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
 /* Copy roofline (src/bench.ts:61-95): iters launches of a kernel that reads q structure-of-arrays
  * populations of elem_bytes each plus a u32 flag word per cell and writes the q populations, with
  * no arithmetic; returns their total milliseconds. The memory-only ceiling of a step. */

@@ -144,3 +144,21 @@ def test_autograd_function_semantics():
     np.testing.assert_array_equal(t.grad.numpy(), direct.astype(np.float32))
     assert float(J) == adj.forward(sdf)
     s.destroy()
+
+
+def test_adjoint2d_naca_384x128_matches_fp64_fixture():
+    """The 2D shape-gradient case at full size (384 x 128, 6000 steps, 9 parameters) against the FP64
+    reverse-mode reference computed on the CPU by tests/golden/make_adjoint_naca_fp64.py."""
+    import json
+    import os
+
+    import bench
+
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_naca_384x128_fp64.json")))
+    out = bench.adjoint2d_case(fd=False)
+    g, g_ref = np.array(out["grad"]), np.array(ref["grad_theta"])
+    err = np.abs(g - g_ref) / np.abs(g_ref)
+    print(f"J {out['J']:.9f} ref {ref['J']:.9f}; max relative gradient error {err.max():.2e}")
+    assert abs(out["J"] - ref["J"]) <= 1e-4 * abs(ref["J"])
+    assert err.max() < 1e-3
+
