@@ -83,7 +83,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     out["roofline_frac"] = out["algorithmic_gbs"] / HBM_PEAK_GBS
     nb, nl, ns = s.node_counts()
     out["nodes"] = dict(bulk=nb, list=nl, solid=ns)
-    out["bulk_block"], out["alternating_rows"] = s.launch_info()
+    out["bulk_block"], out["row_order"] = s.launch_info()
     if profile:
         # one kernel launch per step: HIP events after every step on the solver's stream
         tot, tsum, tmax = s.profile_steps(steps)
@@ -408,7 +408,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "step_kernel_us", "slowest_step_us", "kernel_achieved_gbs",
-                                               "bulk_block", "alternating_rows", "nodes", "finite") if kk in v}
+                                               "bulk_block", "row_order", "nodes", "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],
         "bulk_block": main_run["bulk_block"],
