@@ -98,6 +98,31 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     return out
 
 
+def d2q9_config0(steps=5000):
+    """D2Q9 512 x 128 cylinder tunnel (diameter 16 at (128, 64.5), tau 0.548, Zou-He inlet u = 0.1
+    ramped linearly over 3000 steps, Zou-He outlet, slip walls, 15% sponge, absorbing layer 0.02,
+    a force sample every step): MLUPS and algorithmic TB/s at 76 B per cell. At 65,536 cells one step
+    is a few microseconds, so this measures launch-bound throughput."""
+    from cells_amd import geometry as g
+    from cells_amd.lbm import X_OPEN, Y_SLIP, Solver
+
+    W, H = 512, 128
+    s = Solver.create(dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=0.548,
+                           uIn=0.1, spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=1))
+    sdf = g.empty_sdf(W, H)
+    g.add_circle(sdf, W, H, 128, 64.5, 8)
+    s.set_sdf(sdf)
+    s.init_field()
+    s.set_inlet_ramp("linear", 0.1, 3000)
+    block = s.autotune(200)
+    s.time_steps(500)
+    ms = s.time_steps(steps)
+    s.destroy()
+    mlups = W * H * steps / (ms * 1e3)
+    return dict(grid=[W, H], steps=steps, mlups=mlups, ms_per_step=ms / steps, algorithmic_gbs=mlups * 76e-3,
+                bulk_block=block)
+
+
 PRECISION_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
                                  "c1_sphere_192x96x96_fp64_forces.npy")
 
@@ -377,6 +402,7 @@ def main():
             "fp32_256x128x128_gbs": copy_roofline(256 * 128 * 128, 19, 4)}
     sweep_res = sweep_bench(args.sweep_steps) if (args.sweep_steps > 0 and not args.quick) else None
     precision = None if args.quick else precision_study()
+    d2q9 = None if args.quick else d2q9_config0()
     adjoint = None
     if not (args.quick or args.no_adjoint):
         adjoint = {"naca_384x128": adjoint2d_case(), "sphere_256x128x128_fp16": adjoint3d_case()}
@@ -415,6 +441,7 @@ def main():
         "copy_roofline": copy,
         "sweep": sweep_res,
         "precision_vs_fp64": precision,
+        "d2q9_512x128_cylinder": d2q9,
         "adjoint": adjoint,
         "adjoint_over_forward": adjoint["sphere_256x128x128_fp16"]["ratio"] if adjoint else None,
     }

@@ -565,6 +565,17 @@ extern "C" {
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
 const char* lbm_last_error(void) { return g_err.c_str(); }
 
+int lbm_device_name(int device, char* out, size_t cap) {
+  if (!out || cap == 0) return fail(LBM_ERR_INVALID, "lbm_device_name needs a buffer");
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (prop.name[0])
+    std::snprintf(out, cap, "AMD %s %s (%d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  else
+    std::snprintf(out, cap, "AMD %s (%d CUs)", prop.gcnArchName, prop.multiProcessorCount);
+  return LBM_OK;
+}
+
 void lbm_config2d_defaults(lbm_config2d* c) {
   std::memset(c, 0, sizeof(*c));
   c->lambda = MAGIC_LAMBDA;
@@ -728,6 +739,71 @@ int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n) {
   s->sdf_host.assign(sdf, sdf + s->N);
   HIP_TRY(hipMemcpyAsync(s->sdf, sdf, 4 * size_t(s->N), hipMemcpyHostToDevice, s->stream));
   return rebuild_flags_impl(s);
+}
+
+int lbm_rebuild_flags(lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  return rebuild_flags_impl(s);
+}
+
+namespace {
+
+// src/solver.ts:249-268 (the disc's box grown by 2 cells and clipped to the grid) and
+// src/shaders/aux.ts:176-196 over that box: d = |(x, y) - (cx, cy)| - r in float, with cx, cy and r
+// rounded to float; union sdf = min(sdf, d), erase sdf = max(sdf, -d). The stamp goes into the host
+// copy of the SDF, which the boundary tables are resolved from; the rows it touched widen [ylo, yhi].
+// Returns false for an empty box (the reference then skips the rebuild too).
+bool stamp_disc(lbm_solver* s, double cx, double cy, double r, bool erase, int& ylo, int& yhi) {
+#pragma clang fp contract(off)
+  const double pad = 2;
+  const double x0 = std::max(0.0, std::floor(cx - r - pad)), y0 = std::max(0.0, std::floor(cy - r - pad));
+  const double x1 = std::min(double(s->W) - 1, std::ceil(cx + r + pad));
+  const double y1 = std::min(double(s->H) - 1, std::ceil(cy + r + pad));
+  if (x1 < x0 || y1 < y0) return false;
+  const float fx = float(cx), fy = float(cy), fr = float(r);
+  for (int y = int(y0); y <= int(y1); y++) {
+    float* row = s->sdf_host.data() + size_t(y) * s->W;
+    for (int x = int(x0); x <= int(x1); x++) {
+      const float dx = float(x) - fx, dy = float(y) - fy;
+      const float d = std::sqrt(dx * dx + dy * dy) - fr;
+      row[x] = erase ? std::max(row[x], -d) : std::min(row[x], d);
+    }
+  }
+  ylo = std::min(ylo, int(y0));
+  yhi = std::max(yhi, int(y1));
+  return true;
+}
+
+enum class Rebuild { Never, IfStamped, Always };
+
+int brush_impl(lbm_solver* s, const double* xy, size_t n, double r, int erase, Rebuild rebuild) {
+  if (!s || s->nd != 2) return fail(LBM_ERR_INVALID, "the brush needs a 2D solver (src/solver.ts:244-268)");
+  if (n && !xy) return fail(LBM_ERR_INVALID, "null points");
+  if (!std::isfinite(r)) return fail(LBM_ERR_INVALID, "brush radius must be finite");
+  for (size_t k = 0; k < 2 * n; k++)
+    if (!std::isfinite(xy[k])) return fail(LBM_ERR_INVALID, "brush points must be finite");
+  int ylo = s->H, yhi = -1;
+  bool any = false;
+  for (size_t k = 0; k < n; k++) any |= stamp_disc(s, xy[2 * k], xy[2 * k + 1], r, erase != 0, ylo, yhi);
+  if (any) {
+    const size_t off = size_t(ylo) * s->W, len = size_t(yhi - ylo + 1) * s->W;
+    HIP_TRY(hipMemcpyAsync(s->sdf + off, s->sdf_host.data() + off, 4 * len, hipMemcpyHostToDevice, s->stream));
+  }
+  if (rebuild == Rebuild::Always || (rebuild == Rebuild::IfStamped && any)) return rebuild_flags_impl(s);
+  if (any) HIP_TRY(hipStreamSynchronize(s->stream));  // the host copy may change again before the copy ran
+  return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_brush(lbm_solver* s, double cx, double cy, double r, int erase, int rebuild) {
+  // brush() with an empty box returns before its rebuild (src/solver.ts:255)
+  const double xy[2] = {cx, cy};
+  return brush_impl(s, xy, 1, r, erase, rebuild ? Rebuild::IfStamped : Rebuild::Never);
+}
+
+int lbm_brush_stroke(lbm_solver* s, const double* xy, size_t n_points, double r, int erase) {
+  return brush_impl(s, xy, n_points, r, erase, Rebuild::Always);
 }
 
 int lbm_init_field(lbm_solver* s, const float* macro, size_t n) {

@@ -95,12 +95,8 @@ struct CodecF16 {
 struct CodecBF16 {
   using elem = uint16_t;
   __device__ static __forceinline__ float dec(elem e) { return __uint_as_float(uint32_t(e) << 16); }
-  __device__ static __forceinline__ elem enc(float v) {
-    const uint32_t x = __float_as_uint(v);
-    const uint32_t r =
-        ((x & 0x7fffffffu) > 0x7f800000u) ? ((x >> 16) | 0x40u) : ((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
-    return uint16_t(r);
-  }
+  // v_cvt_pk_bf16_f32 (gfx950) rounds to nearest even
+  __device__ static __forceinline__ elem enc(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
 };
 struct StoreF32 : StoreOps<CodecF32> {};
 struct StoreF16 : StoreOps<CodecF16> {};

@@ -1,6 +1,7 @@
 // Copy roofline: the memory traffic of one step with no arithmetic (the method of src/bench.ts:61-95:
 // read every population of a cell and its flag word, write every population). Q structure-of-arrays
-// streams of the given element size, ping-pong buffers, one thread per cell.
+// streams of the given element size, ping-pong buffers, one thread per cell, 50 untimed launches
+// before the timed ones as the reference's harness does.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -24,7 +25,7 @@ __global__ __launch_bounds__(256) void copy_kernel(const T* __restrict__ src, T*
 }
 
 template <class T, int Q>
-static int run_copy(int64_t n, int iters, double* ms_out) {
+static int run_copy(int64_t n, int block, int iters, double* ms_out) {
   const uint32_t N = uint32_t(n);
   T *a = nullptr, *b = nullptr;
   uint32_t* fl = nullptr;
@@ -37,13 +38,13 @@ static int run_copy(int64_t n, int iters, double* ms_out) {
   HIP_TRY(hipMemset(fl, 0, sizeof(uint32_t) * N));
   hipStream_t st;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  const uint32_t grid = (N + 255) / 256;
-  for (int k = 0; k < 20; k++) copy_kernel<T, Q><<<grid, 256, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, N);
+  const uint32_t grid = (N + uint32_t(block) - 1) / uint32_t(block);
+  for (int k = 0; k < 50; k++) copy_kernel<T, Q><<<grid, block, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, N);
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   HIP_TRY(hipEventRecord(e0, st));
-  for (int k = 0; k < iters; k++) copy_kernel<T, Q><<<grid, 256, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, N);
+  for (int k = 0; k < iters; k++) copy_kernel<T, Q><<<grid, block, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, N);
   HIP_TRY(hipEventRecord(e1, st));
   HIP_TRY(hipEventSynchronize(e1));
   float ms = 0;
@@ -60,14 +61,17 @@ static int run_copy(int64_t n, int iters, double* ms_out) {
 
 }  // namespace lbm
 
-extern "C" int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t iters,
-                                 double* ms_total) {
+extern "C" int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t workgroup,
+                                 int32_t iters, double* ms_total) {
   using namespace lbm;
   if (n_cells <= 0 || iters <= 0 || !ms_total || uint64_t(n_cells) * uint64_t(q) >= (1ull << 32))
     return fail(LBM_ERR_INVALID, "bad copy roofline arguments");
+  const int block = workgroup ? workgroup : 256;
+  if (block != 64 && block != 128 && block != 256)
+    return fail(LBM_ERR_INVALID, "copy roofline workgroup must be 64, 128 or 256 (0: 256)");
   HIP_TRY(hipSetDevice(device));
-  if (q == 19 && elem_bytes == 4) return run_copy<float, 19>(n_cells, iters, ms_total);
-  if (q == 19 && elem_bytes == 2) return run_copy<_Float16, 19>(n_cells, iters, ms_total);
-  if (q == 9 && elem_bytes == 4) return run_copy<float, 9>(n_cells, iters, ms_total);
+  if (q == 19 && elem_bytes == 4) return run_copy<float, 19>(n_cells, block, iters, ms_total);
+  if (q == 19 && elem_bytes == 2) return run_copy<_Float16, 19>(n_cells, block, iters, ms_total);
+  if (q == 9 && elem_bytes == 4) return run_copy<float, 9>(n_cells, block, iters, ms_total);
   return fail(LBM_ERR_UNSUPPORTED, "copy roofline: q must be 19 (4 or 2 bytes) or 9 (4 bytes)");
 }

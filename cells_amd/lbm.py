@@ -131,6 +131,7 @@ def load_library(path: str = LIB_PATH):
     sig = {
         "lbm_abi_version": ([], C.c_int),
         "lbm_last_error": ([], C.c_char_p),
+        "lbm_device_name": ([C.c_int, C.c_char_p, C.c_size_t], C.c_int),
         "lbm_config2d_defaults": ([C.POINTER(Config2D)], None),
         "lbm_config3d_defaults": ([C.POINTER(Config3D)], None),
         "lbm_create_2d": ([C.POINTER(Config2D), C.c_int, C.POINTER(P)], C.c_int),
@@ -140,6 +141,9 @@ def load_library(path: str = LIB_PATH):
         "lbm_update_3d": ([P, C.POINTER(Config3D)], C.c_int),
         "lbm_set_inlet_ramp": ([P, C.c_int, C.c_double, C.c_double], C.c_int),
         "lbm_set_sdf": ([P, C.c_void_p, C.c_size_t], C.c_int),
+        "lbm_rebuild_flags": ([P], C.c_int),
+        "lbm_brush": ([P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int], C.c_int),
+        "lbm_brush_stroke": ([P, C.c_void_p, C.c_size_t, C.c_double, C.c_int], C.c_int),
         "lbm_init_field": ([P, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_reset_forces": ([P], C.c_int),
         "lbm_enqueue_steps": ([P, C.c_int64], C.c_int),
@@ -161,7 +165,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "lbm_launch_info": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
-        "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
+                              C.c_int),
         "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
         "lbm_adjoint_backward": ([P, C.c_double, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_adjoint_info": ([P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
@@ -227,6 +232,12 @@ class _Base:
     def set_sdf(self, sdf):
         a = np.ascontiguousarray(sdf, dtype=np.float32)
         _check(_lib.lbm_set_sdf(self._h, _ptr(a), a.size))
+
+    def rebuild_flags(self):
+        """rebuildFlags (src/solver.ts:205-217): flags, boundary tables and refill from the current SDF."""
+        _check(_lib.lbm_rebuild_flags(self._h))
+
+    rebuildFlags = rebuild_flags
 
     def reset_forces(self):
         _check(_lib.lbm_reset_forces(self._h))
@@ -379,6 +390,17 @@ class Solver(_Base):
 
     initField = init_field
 
+    def brush(self, cx: float, cy: float, r: float, erase: bool = False, rebuild: bool = True):
+        """brush (src/solver.ts:249-268): union (or, erasing, subtract) a disc into the SDF."""
+        _check(_lib.lbm_brush(self._h, float(cx), float(cy), float(r), int(bool(erase)), int(bool(rebuild))))
+
+    def brush_stroke(self, points, r: float, erase: bool = False):
+        """brushStroke (src/solver.ts:243-247): a disc at every (x, y) point, then one rebuild."""
+        xy = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 2))
+        _check(_lib.lbm_brush_stroke(self._h, _ptr(xy), xy.shape[0], float(r), int(bool(erase))))
+
+    brushStroke = brush_stroke
+
 
 class Solver3D(_Base):
     """D3Q19 solver (src/solver3d.ts:42-319)."""
@@ -488,11 +510,19 @@ class Batch:
             pass
 
 
-def copy_roofline(n_cells: int, q: int = 19, elem_bytes: int = 4, iters: int = 200, device: int = 0) -> float:
+def device_name(device: int = 0) -> str:
+    buf = C.create_string_buffer(256)
+    _check(load_library().lbm_device_name(int(device), buf, len(buf)))
+    return buf.value.decode()
+
+
+def copy_roofline(n_cells: int, q: int = 19, elem_bytes: int = 4, iters: int = 200, device: int = 0,
+                  workgroup: int = 256) -> float:
     """GB/s of the copy-roofline kernel: q populations + one flag word read, q populations written."""
     lib = load_library()
     ms = C.c_double()
-    _check(lib.lbm_copy_roofline(int(device), int(n_cells), int(q), int(elem_bytes), int(iters), C.byref(ms)))
+    _check(lib.lbm_copy_roofline(int(device), int(n_cells), int(q), int(elem_bytes), int(workgroup), int(iters),
+                                 C.byref(ms)))
     return n_cells * (q * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
 
 

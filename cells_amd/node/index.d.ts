@@ -56,6 +56,7 @@ declare class SolverBase {
   readPopulations(): Float32Array | Uint16Array;
   readFlags(): Uint32Array;
   timeSteps(n: number): number;
+  autotune(trialSteps?: number): { block: number; sweep: number };
   destroy(): void;
 }
 
@@ -65,6 +66,9 @@ export declare class Solver extends SolverBase {
   readonly cfg: Required<SolverConfig>;
   update(changes: Partial<SolverConfig>): void;
   initField(macro?: Float32Array): void;
+  rebuildFlags(): void;
+  brush(cx: number, cy: number, r: number, erase: boolean, rebuild?: boolean): void;
+  brushStroke(points: [number, number][], r: number, erase: boolean): void;
   readForces(): Promise<ForceSample[]>;
 }
 
@@ -77,6 +81,22 @@ export declare class Solver3D extends SolverBase {
   setInlet(u: number): void;
   initField(): void;
   readForces(): Promise<ForceSample3[]>;
+}
+
+export interface ShapeAdjointOptions {
+  steps: number;
+  window: number;
+  direction?: number[];
+  scale?: number;
+  checkpointEvery?: number;
+  ramp?: { kind: RampKind | 'none'; uTarget: number; steps: number } | null;
+  recordBudgetBytes?: number;
+}
+
+export declare class ShapeAdjoint {
+  constructor(solver: Solver | Solver3D, opts: ShapeAdjointOptions);
+  forward(sdf?: Float32Array): Promise<number>;
+  backward(seed?: number): Promise<Float64Array>;
 }
 
 export declare function trtLambda3(tau: number): number;

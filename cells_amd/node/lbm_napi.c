@@ -246,6 +246,113 @@ static napi_value js_set_sdf(napi_env env, napi_callback_info info) {
   return rc ? throw_status(env, rc) : NULL;
 }
 
+/* deviceName(ordinal) -> string */
+static napi_value js_device_name(napi_env env, napi_callback_info info) {
+  napi_value argv[1], out;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  int32_t dev = 0;
+  napi_get_value_int32(env, argv[0], &dev);
+  char name[256];
+  int rc = lbm_device_name(dev, name, sizeof(name));
+  if (rc) return throw_status(env, rc);
+  CHECK_NAPI(env, napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &out));
+  return out;
+}
+
+/* copyRoofline(ordinal, nCells, q, elemBytes, workgroup, iters) -> total milliseconds */
+static napi_value js_copy_roofline(napi_env env, napi_callback_info info) {
+  napi_value argv[6], out;
+  if (!get_args(env, info, 6, argv)) return NULL;
+  double v[6];
+  for (int k = 0; k < 6; k++)
+    if (napi_get_value_double(env, argv[k], &v[k]) != napi_ok) {
+      napi_throw_type_error(env, NULL, "copyRoofline expects six numbers");
+      return NULL;
+    }
+  double ms = 0;
+  int rc = lbm_copy_roofline((int)v[0], (int64_t)v[1], (int32_t)v[2], (int32_t)v[3], (int32_t)v[4], (int32_t)v[5], &ms);
+  if (rc) return throw_status(env, rc);
+  CHECK_NAPI(env, napi_create_double(env, ms, &out));
+  return out;
+}
+
+/* autotune(handle, trialSteps) -> { block, sweep }: lbm_autotune then lbm_launch_info */
+static napi_value js_autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[2], out, v;
+  if (!get_args(env, info, 2, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  int32_t trial = 200, block = 0, sweep = 0;
+  napi_valuetype t;
+  napi_typeof(env, argv[1], &t);
+  if (t == napi_number) napi_get_value_int32(env, argv[1], &trial);
+  int rc = lbm_autotune(s, trial, NULL);
+  if (!rc) rc = lbm_launch_info(s, &block, &sweep);
+  if (rc) return throw_status(env, rc);
+  CHECK_NAPI(env, napi_create_object(env, &out));
+  napi_create_int32(env, block, &v);
+  napi_set_named_property(env, out, "block", v);
+  napi_create_int32(env, sweep, &v);
+  napi_set_named_property(env, out, "sweep", v);
+  return out;
+}
+
+static napi_value js_rebuild_flags(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  int rc = lbm_rebuild_flags(s);
+  return rc ? throw_status(env, rc) : NULL;
+}
+
+/* brush(handle, cx, cy, r, erase, rebuild) */
+static napi_value js_brush(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  double v[3];
+  bool erase = false, rebuild = true;
+  for (int k = 0; k < 3; k++)
+    if (napi_get_value_double(env, argv[1 + k], &v[k]) != napi_ok) {
+      napi_throw_type_error(env, NULL, "brush expects numbers cx, cy, r");
+      return NULL;
+    }
+  napi_get_value_bool(env, argv[4], &erase);
+  napi_get_value_bool(env, argv[5], &rebuild);
+  int rc = lbm_brush(s, v[0], v[1], v[2], erase, rebuild);
+  return rc ? throw_status(env, rc) : NULL;
+}
+
+/* brushStroke(handle, xy: Float64Array of (x, y) pairs, r, erase) */
+static napi_value js_brush_stroke(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  bool is_ta = false;
+  napi_is_typedarray(env, argv[1], &is_ta);
+  napi_typedarray_type type = napi_int8_array;
+  void* p = NULL;
+  size_t len = 0, off;
+  napi_value ab;
+  if (is_ta) napi_get_typedarray_info(env, argv[1], &type, &len, &p, &ab, &off);
+  if (!is_ta || type != napi_float64_array || len % 2) {
+    napi_throw_type_error(env, NULL, "brushStroke expects a Float64Array of (x, y) pairs");
+    return NULL;
+  }
+  double r = 0;
+  bool erase = false;
+  if (napi_get_value_double(env, argv[2], &r) != napi_ok) {
+    napi_throw_type_error(env, NULL, "brushStroke expects a number r");
+    return NULL;
+  }
+  napi_get_value_bool(env, argv[3], &erase);
+  int rc = lbm_brush_stroke(s, (const double*)p, len / 2, r, erase);
+  return rc ? throw_status(env, rc) : NULL;
+}
+
 static napi_value js_init_field(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return NULL;
@@ -336,7 +443,7 @@ static napi_value js_time_steps(napi_env env, napi_callback_info info) {
 }
 
 /* ---- asynchronous waits ---- */
-enum { JOB_SYNC = 0, JOB_MACRO = 1, JOB_FORCES = 2 };
+enum { JOB_SYNC = 0, JOB_MACRO = 1, JOB_FORCES = 2, JOB_ADJ_FWD = 3, JOB_ADJ_BWD = 4 };
 
 typedef struct {
   int kind;
@@ -348,6 +455,9 @@ typedef struct {
   float* out;
   size_t n;
   int64_t ticket;
+  lbm_adjoint_config acfg; /* JOB_ADJ_FWD */
+  double value;            /* J (JOB_ADJ_FWD), seed (JOB_ADJ_BWD) */
+  double* out64;           /* dJ/dsdf (JOB_ADJ_BWD) */
   int rc;
   char err[512];
 } Job;
@@ -357,6 +467,8 @@ static void job_execute(napi_env env, void* data) {
   Job* j = (Job*)data;
   if (j->kind == JOB_SYNC) j->rc = lbm_sync(j->s);
   else if (j->kind == JOB_MACRO) j->rc = lbm_read_macro(j->s, j->out, j->n);
+  else if (j->kind == JOB_ADJ_FWD) j->rc = lbm_adjoint_forward(j->s, &j->acfg, &j->value);
+  else if (j->kind == JOB_ADJ_BWD) j->rc = lbm_adjoint_backward(j->s, j->value, j->out64, j->n);
   else j->rc = lbm_read_forces_wait(j->s, j->ticket);
   if (j->rc) {
     strncpy(j->err, lbm_last_error(), sizeof(j->err) - 1);
@@ -375,8 +487,10 @@ static void job_complete(napi_env env, napi_status status, void* data) {
   if (rc == 0) {
     if (j->kind == JOB_SYNC) {
       napi_get_undefined(env, &result);
-    } else if (j->kind == JOB_MACRO) {
+    } else if (j->kind == JOB_MACRO || j->kind == JOB_ADJ_BWD) {
       napi_get_reference_value(env, j->out_ref, &result);
+    } else if (j->kind == JOB_ADJ_FWD) {
+      napi_create_double(env, j->value, &result);
     } else {
       /* Bookkeeping back on the JS thread (src/solver.ts:322-334). */
       int32_t dims[3];
@@ -525,6 +639,72 @@ static napi_value js_abi_version(napi_env env, napi_callback_info info) {
   return v;
 }
 
+/* adjointForwardAsync(handle, opts): lbm_adjoint_forward on the libuv pool, resolves to J. opts holds
+ * steps, windowStart, scale, checkpointEvery, direction ([dx, dy, dz]), rampKind, rampU,
+ * rampSteps and recordBudgetBytes (the fields of lbm_adjoint_config). */
+static napi_value js_adjoint_forward_async(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  Job* j = (Job*)calloc(1, sizeof(Job));
+  j->kind = JOB_ADJ_FWD;
+  j->s = s;
+  napi_value o = argv[1];
+  j->acfg.steps = (int64_t)num_prop(env, o, "steps", 0);
+  j->acfg.window_start = (int64_t)num_prop(env, o, "windowStart", 0);
+  j->acfg.scale = num_prop(env, o, "scale", 1.0);
+  j->acfg.checkpoint_every = (int32_t)num_prop(env, o, "checkpointEvery", 200);
+  j->acfg.ramp_kind = (int32_t)num_prop(env, o, "rampKind", 0);
+  j->acfg.ramp_u = num_prop(env, o, "rampU", 0);
+  j->acfg.ramp_steps = num_prop(env, o, "rampSteps", 0);
+  j->acfg.record_budget_bytes = (int64_t)num_prop(env, o, "recordBudgetBytes", 0);
+  j->acfg.direction[0] = 1.0;
+  napi_value dir;
+  bool is_array = false;
+  if (napi_get_named_property(env, argv[1], "direction", &dir) == napi_ok &&
+      napi_is_array(env, dir, &is_array) == napi_ok && is_array) {
+    for (uint32_t k = 0; k < 3; k++) {
+      napi_value e;
+      double d = 0;
+      if (napi_get_element(env, dir, k, &e) == napi_ok && napi_get_value_double(env, e, &d) == napi_ok)
+        j->acfg.direction[k] = d;
+      else
+        j->acfg.direction[k] = 0;
+    }
+  }
+  return start_job(env, argv[0], j, "lbm_adjoint_forward");
+}
+
+/* adjointBackwardAsync(handle, seed): lbm_adjoint_backward on the libuv pool, resolves to a
+ * Float64Array of N values (seed * dJ/dsdf). */
+static napi_value js_adjoint_backward_async(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  int32_t dims[3];
+  int64_t n, step;
+  int32_t q, prec;
+  if (lbm_info(s, dims, &n, &step, &q, &prec)) return throw_status(env, -1);
+  double seed = 1.0;
+  napi_valuetype t;
+  napi_typeof(env, argv[1], &t);
+  if (t == napi_number) napi_get_value_double(env, argv[1], &seed);
+  Job* j = (Job*)calloc(1, sizeof(Job));
+  j->kind = JOB_ADJ_BWD;
+  j->s = s;
+  j->value = seed;
+  j->n = (size_t)n;
+  napi_value ab, arr;
+  void* data = NULL;
+  CHECK_NAPI(env, napi_create_arraybuffer(env, sizeof(double) * (size_t)n, &data, &ab));
+  CHECK_NAPI(env, napi_create_typedarray(env, napi_float64_array, (size_t)n, ab, 0, &arr));
+  j->out64 = (double*)data;
+  napi_create_reference(env, arr, 1, &j->out_ref);
+  return start_job(env, argv[0], j, "lbm_adjoint_backward");
+}
+
 #define EXPORT(name, fn) {name, NULL, fn, NULL, NULL, NULL, napi_enumerable, NULL}
 
 static napi_value init(napi_env env, napi_value exports) {
@@ -537,6 +717,12 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("update3d", js_update3d),
       EXPORT("setInletRamp", js_set_inlet_ramp),
       EXPORT("setSdf", js_set_sdf),
+      EXPORT("rebuildFlags", js_rebuild_flags),
+      EXPORT("deviceName", js_device_name),
+      EXPORT("copyRoofline", js_copy_roofline),
+      EXPORT("autotune", js_autotune),
+      EXPORT("brush", js_brush),
+      EXPORT("brushStroke", js_brush_stroke),
       EXPORT("initField", js_init_field),
       EXPORT("resetForces", js_reset_forces),
       EXPORT("enqueueSteps", js_enqueue_steps),
@@ -548,6 +734,8 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("readForcesAsync", js_read_forces_async),
       EXPORT("readPopulations", js_read_populations),
       EXPORT("readFlags", js_read_flags),
+      EXPORT("adjointForwardAsync", js_adjoint_forward_async),
+      EXPORT("adjointBackwardAsync", js_adjoint_backward_async),
   };
   napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
   return exports;

@@ -64,10 +64,39 @@ function flat3(c) {
   return o;
 }
 
-/** Shared plumbing: async waits are tracked so destroy() can wait for them before freeing. */
+/**
+ * Shared plumbing: async waits are tracked so destroy() can wait for them before freeing, and while a
+ * ShapeAdjoint pass runs on the worker pool (it owns the solver's buffers and stream) every other
+ * method throws instead of racing it.
+ */
 class Base {
   constructor() {
     this._pending = new Set();
+    this._busy = null;
+    this._h = null;
+  }
+  get handle() {
+    if (this._busy) throw new Error(`${this._busy} is running on this solver`);
+    return this._h;
+  }
+  set handle(h) {
+    this._h = h;
+  }
+  /**
+   * Runs start(rawHandle) as the solver's only user: after the waits already pending settle, and until
+   * the promise start returns settles.
+   */
+  _exclusive(name, start) {
+    const h = this.handle;
+    if (!h) throw new Error('solver destroyed');
+    this._busy = name;
+    const before = Array.from(this._pending).map((q) => q.catch(() => undefined));
+    const p = Promise.all(before).then(() => start(h));
+    const clear = () => {
+      this._busy = null;
+    };
+    p.then(clear, clear);
+    return this._track(p);
   }
   _track(p) {
     this._pending.add(p);
@@ -127,10 +156,14 @@ class Base {
   timeSteps(n) {
     return addon().timeSteps(this.handle, n);
   }
+  /** Picks the fastest launch shape for this grid (results do not change); returns { block, sweep }. */
+  autotune(trialSteps) {
+    return addon().autotune(this.handle, trialSteps === undefined ? 200 : trialSteps);
+  }
   destroy() {
-    const h = this.handle;
+    const h = this._h;
     if (!h) return;
-    this.handle = null;
+    this._h = null;
     if (this._pending.size === 0) addon().destroy(h);
     else Promise.all(Array.from(this._pending).map((p) => p.catch(() => undefined))).then(() => addon().destroy(h));
   }
@@ -164,6 +197,23 @@ class Solver extends Base {
   /** initField (src/solver.ts:220-234): equilibrium at (rho, ux, uy) per cell, or at rest. */
   initField(macro) {
     addon().initField(this.handle, macro || null);
+  }
+  /** rebuildFlags (src/solver.ts:205-217). */
+  rebuildFlags() {
+    addon().rebuildFlags(this.handle);
+  }
+  /** brush (src/solver.ts:249-268): union (or, erasing, subtract) a disc into the SDF. */
+  brush(cx, cy, r, erase, rebuild) {
+    addon().brush(this.handle, cx, cy, r, !!erase, rebuild === undefined ? true : !!rebuild);
+  }
+  /** brushStroke (src/solver.ts:243-247): a disc at every [x, y] point, then one rebuild. */
+  brushStroke(points, r, erase) {
+    const xy = new Float64Array(points.length * 2);
+    points.forEach((p, k) => {
+      xy[2 * k] = p[0];
+      xy[2 * k + 1] = p[1];
+    });
+    addon().brushStroke(this.handle, xy, r, !!erase);
   }
 }
 
@@ -207,7 +257,40 @@ class Solver3D extends Base {
   }
 }
 
+const RAMP_CODES = Object.freeze({ off: 0, none: 0, linear: 1, smoothstep: 2 });
+
+/**
+ * Shape gradient of a drag objective through the discrete adjoint (no counterpart in the reference):
+ * forward(sdf) runs the tunnel from its initial field for `steps` steps and resolves to
+ * J = scale * mean over the last `window` force samples of direction . F; backward(seed) resolves to
+ * seed * dJ/dsdf at every node (Float64Array of N). Both run on the libuv pool over
+ * lbm_adjoint_forward / lbm_adjoint_backward.
+ */
+class ShapeAdjoint {
+  constructor(solver, opts) {
+    const o = opts || {};
+    const ramp = o.ramp || null;
+    const kind = ramp ? RAMP_CODES[ramp.kind] : 0;
+    if (kind === undefined) throw new Error(`unknown ramp ${ramp.kind}`);
+    if (!(o.steps > 0) || !(o.window > 0) || o.window > o.steps) throw new Error('need 0 < window <= steps');
+    this.solver = solver;
+    this.opts = {
+      steps: o.steps, windowStart: o.steps - o.window, direction: (o.direction || [1, 0, 0]).concat([0, 0, 0]).slice(0, 3),
+      scale: o.scale === undefined ? 1 : o.scale, checkpointEvery: o.checkpointEvery || 200, rampKind: kind,
+      rampU: ramp ? ramp.uTarget : 0, rampSteps: ramp ? ramp.steps : 0, recordBudgetBytes: o.recordBudgetBytes || 0,
+    };
+  }
+  forward(sdf) {
+    if (sdf !== undefined && sdf !== null) this.solver.setSdf(sdf);
+    return this.solver._exclusive('ShapeAdjoint.forward', (h) => addon().adjointForwardAsync(h, this.opts));
+  }
+  backward(seed) {
+    const k = seed === undefined ? 1 : seed;
+    return this.solver._exclusive('ShapeAdjoint.backward', (h) => addon().adjointBackwardAsync(h, k));
+  }
+}
+
 module.exports = {
-  Solver, Solver3D, XMode, YMode, MAGIC_LAMBDA, HISTORY_LEN, PRESETS3, trtLambda3, fitsLimits, bytesPerPopulation,
-  addon,
+  Solver, Solver3D, ShapeAdjoint, XMode, YMode, MAGIC_LAMBDA, HISTORY_LEN, PRESETS3, trtLambda3, fitsLimits,
+  bytesPerPopulation, addon,
 };

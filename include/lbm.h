@@ -89,6 +89,9 @@ typedef struct lbm_solver lbm_solver;
 /* Library-wide. */
 int lbm_abi_version(void);
 const char* lbm_last_error(void);
+/* The adapter line of the reference's benchmark page (src/bench.ts:100-102): vendor, architecture and
+ * product name of a GPU ordinal, NUL-terminated into out[cap]. */
+int lbm_device_name(int device, char* out, size_t cap);
 /* Fills the reference defaults; the caller then sets width/height/(depth)/tau (and modes in 2D). */
 void lbm_config2d_defaults(lbm_config2d* cfg);
 void lbm_config3d_defaults(lbm_config3d* cfg);
@@ -114,6 +117,17 @@ int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps);
 /* setSdf (src/solver.ts:200-203, src/solver3d.ts:211-214): n = N floats, negative inside solids.
  * Rebuilds flags, the boundary-cell list (in cell order) and refills cells the body uncovered. */
 int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n);
+
+/* rebuildFlags (src/solver.ts:205-217): flags, boundary tables and refill from the current SDF. */
+int lbm_rebuild_flags(lbm_solver* s);
+
+/* Solver.brush (src/solver.ts:249-268, src/shaders/aux.ts:176-196), 2D only: unions (erase = 0) or
+ * subtracts (erase = 1) the disc of radius r at (cx, cy) over its box grown by 2 cells, clipped to the
+ * grid; then rebuilds as lbm_rebuild_flags when rebuild != 0 and the box is not empty. */
+int lbm_brush(lbm_solver* s, double cx, double cy, double r, int erase, int rebuild);
+/* Solver.brushStroke (src/solver.ts:243-247), 2D only: the discs at the n_points (x, y) pairs of xy,
+ * in order, then one rebuild. */
+int lbm_brush_stroke(lbm_solver* s, const double* xy, size_t n_points, double r, int erase);
 
 /* initField (src/solver.ts:220-234, src/solver3d.ts:231-242). 2D: macro = N*4 floats (rho, ux, uy, -)
  * or NULL for rest; 3D: macro must be NULL (inflow equilibrium). Resets step, parity and forces. */
@@ -176,10 +190,12 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
  * order: 0 always bottom-up; 1 (2) top-down on the steps that read the population buffer at the
  * higher (lower) address, bottom-up on the others. */
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
-/* Copy roofline (src/bench.ts:61-95): iters launches of a kernel that reads q structure-of-arrays
- * populations of elem_bytes each plus a u32 flag word per cell and writes the q populations, with
- * no arithmetic; returns their total milliseconds. The memory-only ceiling of a step. */
-int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t iters, double* ms_total);
+/* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
+ * structure-of-arrays populations of elem_bytes each plus a u32 flag word per cell and writes the q
+ * populations, with no arithmetic, one thread per cell in blocks of workgroup (64, 128 or 256; 0 for
+ * 256); returns their total milliseconds. The memory-only ceiling of a step. */
+int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t workgroup, int32_t iters,
+                      double* ms_total);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 

@@ -36,6 +36,9 @@ def load():
         "orc_update_2d": ([P, C.POINTER(Config2D)], None),
         "orc_update_3d": ([P, C.POINTER(Config3D)], None),
         "orc_set_sdf": ([P, C.c_void_p], None),
+        "orc_brush": ([P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int], None),
+        "orc_rebuild_flags": ([P], None),
+        "orc_read_sdf": ([P, C.c_void_p], None),
         "orc_init_field": ([P, C.c_void_p], None),
         "orc_reset_forces": ([P], None),
         "orc_set_ramp": ([P, C.c_int, C.c_double, C.c_double], None),
@@ -102,6 +105,11 @@ class _Base:
         a = np.ascontiguousarray(sdf, dtype=np.float32)
         assert a.size == self.N
         _lib.orc_set_sdf(self._h, _ptr(a))
+
+    def read_sdf(self) -> np.ndarray:
+        out = np.empty(self.N, dtype=np.float32)
+        _lib.orc_read_sdf(self._h, _ptr(out))
+        return out
 
     def reset_forces(self):
         _lib.orc_reset_forces(self._h)
@@ -195,6 +203,18 @@ class OracleSolver(_Base):
             m = np.ascontiguousarray(macro, dtype=np.float32)
             assert m.size == self.N * 4
             _lib.orc_init_field(self._h, _ptr(m))
+
+    def rebuild_flags(self):
+        _lib.orc_rebuild_flags(self._h)
+
+    def brush(self, cx, cy, r, erase=False, rebuild=True):
+        _lib.orc_brush(self._h, float(cx), float(cy), float(r), int(bool(erase)), int(bool(rebuild)))
+
+    def brush_stroke(self, points, r, erase=False):
+        """src/solver.ts:244-247: every disc stamped, then one rebuild."""
+        for x, y in points:
+            self.brush(x, y, r, erase, False)
+        self.rebuild_flags()
 
 
 class OracleSolver3D(_Base):

@@ -22,6 +22,9 @@ struct Base {
   virtual void update2(const lbm_config2d*) {}
   virtual void update3(const lbm_config3d*) {}
   virtual void set_sdf(const float* s) = 0;
+  virtual void brush(double, double, double, bool, bool) {}
+  virtual void rebuild() = 0;
+  virtual void read_sdf(float* out) = 0;
   virtual void init_field(const float* m) = 0;
   virtual void reset_forces() = 0;
   virtual void set_ramp(int kind, double u, double steps) = 0;
@@ -45,6 +48,8 @@ struct Impl : Base {
     std::memcpy(s.sdf.data(), sd, sizeof(float) * size_t(s.N));
     s.rebuild_flags();
   }
+  void rebuild() override { s.rebuild_flags(); }
+  void read_sdf(float* out) override { std::memcpy(out, s.sdf.data(), sizeof(float) * size_t(s.N)); }
   void reset_forces() override { s.reset_forces(); }
   void set_ramp(int kind, double u, double steps) override {
     s.ramp.kind = kind;
@@ -85,6 +90,9 @@ struct Impl2 : Impl<orc::Solver2D<T, S>> {
     this->s.write_params();
   }
   void init_field(const float* m) override { this->s.init_field(m); }
+  void brush(double cx, double cy, double r, bool erase, bool rebuild) override {
+    this->s.brush(cx, cy, r, erase, rebuild);
+  }
   void read_slots(uint32_t* out) override {
     for (int64_t k = 0; k < this->s.N; k++) out[k] = this->s.flags[size_t(k)] >> orc::BIDX_SHIFT2;
   }
@@ -150,6 +158,11 @@ int orc_max_threads(void) { return omp_get_max_threads(); }
 void orc_update_2d(void* h, const lbm_config2d* c) { static_cast<Base*>(h)->update2(c); }
 void orc_update_3d(void* h, const lbm_config3d* c) { static_cast<Base*>(h)->update3(c); }
 void orc_set_sdf(void* h, const float* sdf) { static_cast<Base*>(h)->set_sdf(sdf); }
+void orc_brush(void* h, double cx, double cy, double r, int erase, int rebuild) {
+  static_cast<Base*>(h)->brush(cx, cy, r, erase != 0, rebuild != 0);
+}
+void orc_rebuild_flags(void* h) { static_cast<Base*>(h)->rebuild(); }
+void orc_read_sdf(void* h, float* out) { static_cast<Base*>(h)->read_sdf(out); }
 void orc_init_field(void* h, const float* macro) { static_cast<Base*>(h)->init_field(macro); }
 void orc_reset_forces(void* h) { static_cast<Base*>(h)->reset_forces(); }
 void orc_set_ramp(void* h, int kind, double u_target, double steps) {

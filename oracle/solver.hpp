@@ -364,6 +364,27 @@ struct Solver2D {
     return !open && sdf[size_t(int64_t(y) * W + x)] < 0.0f;
   }
 
+  // src/solver.ts:249-268: the disc's box grown by 2 cells, clipped to the grid (an empty box returns
+  // without a rebuild); src/shaders/aux.ts:176-196 over it: d = |(x, y) - (cx, cy)| - r in float with
+  // cx, cy, r rounded to float, union sdf = min(sdf, d), erase sdf = max(sdf, -d).
+  void brush(double cx, double cy, double r, bool erase, bool rebuild) {
+    const double pad = 2;
+    const double x0 = std::max(0.0, std::floor(cx - r - pad)), y0 = std::max(0.0, std::floor(cy - r - pad));
+    const double x1 = std::min(double(W - 1), std::ceil(cx + r + pad));
+    const double y1 = std::min(double(H - 1), std::ceil(cy + r + pad));
+    if (x1 < x0 || y1 < y0) return;
+    const float fx = float(cx), fy = float(cy), fr = float(r);
+    for (int y = int(y0); y <= int(y1); y++) {
+      for (int x = int(x0); x <= int(x1); x++) {
+        const float dx = float(x) - fx, dy = float(y) - fy;
+        const float d = std::sqrt(dx * dx + dy * dy) - fr;
+        float& v = sdf[size_t(int64_t(y) * W + x)];
+        v = erase ? std::max(v, -d) : std::min(v, d);
+      }
+    }
+    if (rebuild) rebuild_flags();
+  }
+
   // src/solver.ts:205-217 + src/shaders/aux.ts:39-87; slots in cell order.
   void rebuild_flags() {
     macro();

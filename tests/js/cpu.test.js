@@ -101,12 +101,42 @@ test('addon exports every entry point', () => {
   const a = m.addon();
   for (const k of ['create2d', 'create3d', 'destroy', 'update2d', 'update3d', 'setSdf', 'initField', 'resetForces',
     'enqueueSteps', 'enqueueMacro', 'syncAsync', 'readMacroAsync', 'readForcesAsync', 'readPopulations', 'readFlags',
-    'info', 'timeSteps', 'setInletRamp']) assert.strictEqual(typeof a[k], 'function', k);
+    'info', 'timeSteps', 'setInletRamp', 'adjointForwardAsync', 'adjointBackwardAsync', 'rebuildFlags', 'brush', 'brushStroke', 'deviceName', 'copyRoofline', 'autotune']) assert.strictEqual(typeof a[k], 'function', k);
   assert.strictEqual(a.abiVersion(), 1);
 });
 
-if (failures) {
-  console.log(`${failures} failed`);
-  process.exit(1);
+// An exclusive pass (ShapeAdjoint) starts after the pending waits and locks the solver until it settles.
+async function exclusiveGuard() {
+  const s = Object.create(m.Solver.prototype);
+  s._pending = new Set();
+  s._busy = null;
+  s._h = { fake: true };
+  const order = [];
+  s._track(new Promise((r) => setTimeout(() => { order.push('read'); r(); }, 20)));
+  const p = s._exclusive('ShapeAdjoint.forward', (h) => {
+    order.push('pass');
+    assert.strictEqual(h, s._h);
+    return Promise.resolve(1.5);
+  });
+  assert.throws(() => s.handle, /ShapeAdjoint.forward is running on this solver/);
+  assert.strictEqual(await p, 1.5);
+  assert.deepStrictEqual(order, ['read', 'pass']);
+  assert.strictEqual(s.handle, s._h);
+  const q = s._exclusive('ShapeAdjoint.backward', () => Promise.reject(new Error('boom')));
+  await assert.rejects(q, /boom/);
+  assert.strictEqual(s.handle, s._h);
 }
-console.log('all passed');
+
+exclusiveGuard().then(
+  () => console.log('ok exclusive adjoint pass'),
+  (e) => {
+    failures++;
+    console.log(`FAIL exclusive adjoint pass: ${e.message}`);
+  },
+).then(() => {
+  if (failures) {
+    console.log(`${failures} failed`);
+    process.exit(1);
+  }
+  console.log('all passed');
+});

@@ -54,6 +54,52 @@ async function main() {
     console.log(`ok ${dim}d force ring semantics`);
   }
 
+  // --- shape gradient through ShapeAdjoint: dJ/dcx and dJ/dr of a circle against central differences
+  {
+    const [AW, AH, u, cx, cy, r] = [64, 24, 0.05, 16.3, 11.5, 4];
+    const sa = await m.Solver.create(0, { width: AW, height: AH, left: m.XMode.Open, right: m.XMode.Open,
+      bottom: m.YMode.Slip, top: m.YMode.Slip, tau: 0.56, uIn: u, spongeFraction: 0.15, spongeTau: 1, absorb: 0.02,
+      forceEvery: 1 });
+    const adj = new m.ShapeAdjoint(sa, { steps: 200, window: 80, direction: [1, 0], scale: 2 / (u * u * 2 * r),
+      checkpointEvery: 50, ramp: { kind: 'smoothstep', uTarget: u, steps: 60 } });
+    const circle = (x, rad) => {
+      const g = m.emptySdf(AW, AH);
+      m.addCircle(g, AW, AH, x, cy, rad);
+      return g;
+    };
+    const sdf0 = circle(cx, r);
+    const J0 = await adj.forward(sdf0);
+    const grad = await adj.backward();
+    assert.strictEqual(grad.length, AW * AH);
+    assert.strictEqual(await adj.forward(circle(cx, r)), J0);
+    const g2 = await adj.backward(-2);
+    // dsdf/dcx = -(x - cx) / |x - c| and dsdf/dr = -1 where the circle wrote the SDF, 0 elsewhere
+    let dcx = 0;
+    let dr = 0;
+    const gmax = grad.reduce((a, v) => Math.max(a, Math.abs(v)), 0);
+    assert.ok(gmax > 0);
+    for (let y = 0; y < AH; y++) {
+      for (let x = 0; x < AW; x++) {
+        const k = y * AW + x;
+        assert.ok(Math.abs(g2[k] + 2 * grad[k]) <= 2e-5 * Math.abs(grad[k]) + 2e-6 * gmax, 'seed scaling');
+        if (sdf0[k] >= m.FAR) continue;
+        const dist = Math.hypot(x - cx, y - cy);
+        if (dist > 0) dcx -= grad[k] * (x - cx) / dist;
+        dr -= grad[k];
+      }
+    }
+    const h = 1e-3;
+    const fdx = ((await adj.forward(circle(cx + h, r))) - (await adj.forward(circle(cx - h, r)))) / (2 * h);
+    const fdr = ((await adj.forward(circle(cx, r + h))) - (await adj.forward(circle(cx, r - h)))) / (2 * h);
+    console.log(`adjoint J ${J0.toFixed(6)}: dJ/dcx ${dcx.toExponential(4)} (FD ${fdx.toExponential(4)}), ` +
+      `dJ/dr ${dr.toExponential(4)} (FD ${fdr.toExponential(4)})`);
+    const scale = Math.max(Math.abs(fdx), Math.abs(fdr));
+    assert.ok(Math.abs(dcx - fdx) <= 0.02 * Math.abs(fdx) + 2e-3 * scale, 'dJ/dcx');
+    assert.ok(Math.abs(dr - fdr) <= 0.02 * Math.abs(fdr) + 2e-3 * scale, 'dJ/dr');
+    sa.destroy();
+    console.log('ok shape adjoint');
+  }
+
   // --- allocation failure message keeps the reference's prefix
   let err = null;
   try {

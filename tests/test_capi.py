@@ -49,4 +49,5 @@ def test_host_only_calls():
     assert lib.lbm_create_2d(None, 0, C.byref(h)) != 0
     assert lib.lbm_last_error()
     assert lib.lbm_adjoint_forward(None, None, None) != 0
-    assert lib.lbm_copy_roofline(0, -1, 19, 4, 1, C.byref(C.c_double())) != 0
+    assert lib.lbm_copy_roofline(0, -1, 19, 4, 0, 1, C.byref(C.c_double())) != 0
+    assert lib.lbm_copy_roofline(0, 1024, 19, 4, 96, 1, C.byref(C.c_double())) != 0

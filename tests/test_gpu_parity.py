@@ -173,9 +173,11 @@ def test_d3q19_smagorinsky_and_setters():
     assert _err(gpu.read_populations(), o32.read_populations()) < 2e-5
 
 
-def test_d3q19_fp16_bits_match_oracle_codec():
-    """After init (no arithmetic differences yet) the stored FP16 words equal the oracle's codec."""
-    cfg, sdf = case3("fp16")
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_d3q19_fp16_bits_match_oracle_codec(precision):
+    """After init (no arithmetic differences yet) the stored FP16 / BF16 words equal the oracle's
+    codec (round to nearest even; a truncating or biased encoder would miss on about half)."""
+    cfg, sdf = case3(precision)
     gpu = Solver3D.create(dict(cfg))
     o32 = OracleSolver3D(dict(cfg), 32)
     for s in (gpu, o32):
@@ -203,3 +205,46 @@ def test_ring_and_reset_semantics_on_gpu():
     s.run(9)
     f = s.read_forces()
     assert [x.step for x in f] == [52, 56]
+
+
+def test_d2q9_brush_strokes_match_oracle():
+    """brush / brushStroke (src/solver.ts:243-268) during developed flow: a stroke that grows the body
+    and an erase that uncovers part of it. Flags bit-exact with the oracle, refilled populations and
+    20 more steps within the setSdf bounds, and the brushed field equal bit for bit to the same SDF
+    given through setSdf."""
+    name = "tunnel"
+    sims = _build2(name)
+    for s in sims.values():
+        s.run(40) if hasattr(s, "sync") else s.steps(40)
+    stroke = [(22.0, 8.0), (23.5, 9.0), (25.0, 10.0), (26.5, 11.0)]
+    for s in sims.values():
+        s.brush_stroke(stroke, 2.0)
+        s.brush(12.5, 11.5, 1.8, erase=True)
+    np.testing.assert_array_equal(sims["gpu"].read_flags(), sims["o32"].read_flags())
+    assert _err(sims["gpu"].read_populations(), sims["o32"].read_populations()) < 1e-5
+    for s in sims.values():
+        s.run(20) if hasattr(s, "sync") else s.steps(20)
+    assert _err(sims["gpu"].read_populations(), sims["o32"].read_populations()) < 2e-5
+    # the same field through setSdf on a second GPU solver that took the same path
+    cfg, sdf_fn = CASES2[name]
+    W, H = cfg["width"], cfg["height"]
+    twin = Solver.create(dict(cfg))
+    twin.set_sdf(sdf_fn(W, H))
+    twin.init_field()
+    twin.run(40)
+    brushed = sims["o32"].read_sdf()
+    twin.set_sdf(brushed)
+    twin.run(20)
+    np.testing.assert_array_equal(twin.read_flags(), sims["gpu"].read_flags())
+    np.testing.assert_array_equal(twin.read_populations(), sims["gpu"].read_populations())
+    # an empty box is a no-op; non-finite input and 3D solvers are refused
+    f0 = sims["gpu"].read_flags()
+    sims["gpu"].brush(-50.0, 5.0, 2.0)
+    np.testing.assert_array_equal(sims["gpu"].read_flags(), f0)
+    with pytest.raises(RuntimeError):
+        sims["gpu"].brush(float("nan"), 5.0, 2.0)
+    s3 = Solver3D.create(dict(width=16, height=8, depth=8, precision="fp32", tau=0.6))
+    with pytest.raises(RuntimeError):
+        Solver.brush(s3, 4.0, 4.0, 1.0)  # the brush is 2D only
+    for s in (twin, s3, sims["gpu"]):
+        s.destroy()
