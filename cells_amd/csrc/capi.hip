@@ -537,8 +537,10 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
     const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
     // row order of this step: which buffer is walked top-down matters (measured), so both
     // alternations are candidates of the autotuner
-    const bool descend = s->sweep == 1 ? (src > dst) : s->sweep == 2 ? (src < dst) : false;
-    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, descend, s->row_clean,
+    const int order = s->sweep & 3;
+    const bool descend = order == 1 ? (src > dst) : order == 2 ? (src < dst) : false;
+    launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, descend,
+                 (s->sweep & SWEEP_MERGE) != 0, s->row_clean,
                  s->stream, rec);
   }
   s->parity ^= 1;
@@ -1093,7 +1095,8 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     if (std::find(tried.begin(), tried.end(), bx) != tried.end()) continue;
     tried.push_back(bx);
     if (s->nd == 3) {
-      for (int sw : {1, 2, 0}) cand.push_back({bx, sw});
+      for (int sw : {1, 2, 0})
+        for (int mg : {0, SWEEP_MERGE}) cand.push_back({bx, sw | mg});
     } else {
       cand.push_back({bx, s->sweep});
     }
@@ -1147,6 +1150,17 @@ int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (block) *block = int32_t(bulk_block_x(s->W, s->bulk_block));
   if (sweep) *sweep = s->nd == 3 ? s->sweep : 0;
+  return LBM_OK;
+}
+
+int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  const int bb = bulk_block_for(block);
+  if (bb < 0 || block <= 0) return fail(LBM_ERR_INVALID, "block must be a multiple of 64 up to 1024");
+  if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE) || (sweep & 3) == 3) : sweep != 0)
+    return fail(LBM_ERR_INVALID, "variant must be 0-2 or 4-6 (3D), 0 (2D)");
+  s->bulk_block = bb;
+  if (s->nd == 3) s->sweep = sweep;
   return LBM_OK;
 }
 

@@ -44,11 +44,12 @@ struct Record {
 // ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 ----
 // One time step in one launch: bulk rows (plain fluid, slip-face rows and plain inlet/outlet nodes,
 // one thread per cell) and boundary-list blocks (every other non-solid node, plus the force sample).
-// descend: walk the rows top-down this step (alternated step by step for cache reuse). row_clean:
-// one byte per row (y + H z), 1 where the row's node kinds follow from x alone.
+// descend: walk the rows top-down this step (alternated step by step for cache reuse). merge: the
+// inlet and outlet nodes share their wave's stores (same values either way). row_clean: one byte per
+// row (y + H z), 1 where the row's node kinds follow from x alone.
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  bool descend, const uint8_t* row_clean, hipStream_t st, const Record* rec = nullptr);
+                  bool descend, bool merge, const uint8_t* row_clean, hipStream_t st, const Record* rec = nullptr);
 // Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
 void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
 uint32_t bulk_block_x(int W, int requested);

@@ -249,7 +249,14 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
 // descend walks the rows in reverse order (z and y from the top). Alternating the direction every
 // step makes a step start on the rows the previous step wrote last, which are still in the 256 MB
 // Infinity Cache when the two population buffers do not fit in it together.
-template <class S, bool REC, bool SMAG>
+//
+// MERGE selects how the inlet and outlet nodes of a row store: false, each stores its own values
+// in a branch of its own and the wave's other nodes store as soon as their collision is done (the
+// stores overlap the arithmetic); true, their values are chosen first and every node of the wave
+// shares one store per population (whole lines per store instead of a line with a one-node hole
+// plus a one-node store, which costs read-modify-writes where the lines go to HBM rather than stay
+// in cache). Both give the same values; the autotuner picks per grid.
+template <class S, bool REC, bool SMAG, bool MERGE>
 __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
@@ -291,12 +298,13 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   const size_t N = p.N;
   const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
-  if (inlet) {
+  if (!MERGE && inlet) {
     // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
 #pragma unroll
     for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, fin.v[i]);
     return;
   }
+  // (MERGE: the inlet node's pulls reach x = -1, the previous row's last node: in bounds, unused.)
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[19];
 #pragma unroll
@@ -325,8 +333,8 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     f[i] = S::load_at(base, o);
   }
   const Mom3 m = moments3(f);
-  if (REC) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
-  if (outlet) {
+  if (REC && !inlet) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
+  if (!MERGE && outlet) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
     for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz));
@@ -340,6 +348,13 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   r.t0 = t.w;
   float out[19];
   collide3<SMAG>(f, m, r, p, out);
+  if (MERGE && outlet) {
+#pragma unroll
+    for (int i = 0; i < 19; i++) out[i] = feq3(i, 1.0f, m.ux, m.uy, m.uz);
+  } else if (MERGE && inlet) {
+#pragma unroll
+    for (int i = 0; i < 19; i++) out[i] = fin.v[i];
+  }
 #pragma unroll
   for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, out[i]);
 }
@@ -588,7 +603,7 @@ uint32_t bulk_block_x(int W, int requested) {
 
 template <class S>
 static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt, const Params3& p,
-                    const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, bool descend,
+                    const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, bool descend, bool merge,
                     const uint8_t* row_clean, hipStream_t st, const Record* rec) {
   using E = typename S::elem;
   FeqIn fin;
@@ -603,24 +618,34 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const Lis
   const uint32_t desc = descend ? 1u : 0u;
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
+  const Record none{nullptr, nullptr};
   if (rec)  // recording steps (the adjoint) never use the subgrid model
-    step3_kernel<S, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec, desc,
-                                                       row_clean);
+    step3_kernel<S, true, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec,
+                                                              desc, row_clean);
   else if (p.smagC2 > 0.0f)
-    step3_kernel<S, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                       Record{nullptr, nullptr}, desc, row_clean);
+    step3_kernel<S, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
+                                                              desc, row_clean);
+  else if (merge)
+    step3_kernel<S, false, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
+                                                              desc, row_clean);
   else
-    step3_kernel<S, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                        Record{nullptr, nullptr}, desc, row_clean);
+    step3_kernel<S, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
+                                                               desc, row_clean);
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  bool descend, const uint8_t* row_clean, hipStream_t st, const Record* rec) {
+                  bool descend, bool merge, const uint8_t* row_clean, hipStream_t st, const Record* rec) {
   switch (prec) {
-    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
-    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
-    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, row_clean, st, rec); break;
+    case 1:
+      step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
+      break;
+    case 2:
+      step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
+      break;
+    default:
+      step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
+      break;
   }
 }
 

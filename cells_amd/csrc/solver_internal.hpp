@@ -48,6 +48,8 @@ struct Ticket {
 
 }  // namespace lbm
 
+constexpr int SWEEP_MERGE = 4;
+
 struct lbm_solver {
   int nd = 2;  // 2 or 3
   int device = 0;
@@ -64,8 +66,9 @@ struct lbm_solver {
   lbm::Params3 p3{};
   lbm::Ramp ramp;
   int bulk_block = 256;
-  // 3D row order (autotuned): 0 = always bottom-up; 1 / 2 = top-down on the steps that read the
-  // buffer at the higher / lower address, bottom-up on the others
+  // 3D launch variant (autotuned). Bits 0-1, row order: 0 = always bottom-up; 1 / 2 = top-down on
+  // the steps that read the buffer at the higher / lower address, bottom-up on the others. Bit 2
+  // (SWEEP_MERGE): inlet and outlet nodes share their wave's stores.
   int sweep = 1;
 
   void* f[2] = {nullptr, nullptr};

@@ -165,6 +165,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_node_counts": ([P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "lbm_launch_info": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "lbm_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
                               C.c_int),
         "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
@@ -317,10 +318,15 @@ class _Base:
         return int(c.value)
 
     def launch_info(self):
-        """(threads per bulk block, 3D row order: 0 fixed, 1 / 2 alternating) in use."""
+        """(threads per bulk block, 3D variant: bits 0-1 row order 0 fixed, 1 / 2 alternating; bit 2
+        inlet/outlet nodes share their wave's stores) in use."""
         b, w = C.c_int32(), C.c_int32()
         _check(_lib.lbm_launch_info(self._h, C.byref(b), C.byref(w)))
         return int(b.value), int(w.value)
+
+    def set_launch(self, block: int, variant: int = 0):
+        """Pins the launch shape instead of autotune (same meaning as launch_info)."""
+        _check(_lib.lbm_set_launch(self._h, int(block), int(variant)))
 
     def node_counts(self):
         """(bulk, list, solid) node counts."""

@@ -111,7 +111,7 @@ async function bench3D(say, o) {
         let shape = '';
         if (wg === 'auto') {
           const t = s.autotune(200);
-          shape = `, block ${t.block} rows ${['up', 'alt-hi', 'alt-lo'][t.sweep] || t.sweep}`;
+          shape = `, block ${t.block} rows ${['up', 'alt-hi', 'alt-lo'][t.sweep & 3]}${t.sweep & 4 ? ' merged stores' : ''}`;
         }
         s.timeSteps(50);
         const runs = [];

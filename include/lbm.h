@@ -186,10 +186,14 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
  * fastest (cached per lattice, precision and grid size for the process), and restores populations,
  * step and force ring. chosen (may be NULL) receives the width. No choice changes results. */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
-/* No counterpart in the reference: the launch shape in use, threads per bulk block and (3D) the row
- * order: 0 always bottom-up; 1 (2) top-down on the steps that read the population buffer at the
- * higher (lower) address, bottom-up on the others. */
+/* No counterpart in the reference: the launch shape in use, threads per bulk block and (3D) the
+ * variant: bits 0-1 the row order, 0 always bottom-up, 1 (2) top-down on the steps that read the
+ * population buffer at the higher (lower) address, bottom-up on the others; bit 2 set when the
+ * inlet and outlet nodes share their wave's stores. */
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
+/* No counterpart in the reference: sets the launch shape by hand instead of lbm_autotune (block as
+ * workgroup_size, variant as lbm_launch_info reports it; 0 in 2D). No choice changes results. */
+int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
 /* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
  * structure-of-arrays populations of elem_bytes each plus a u32 flag word per cell and writes the q
  * populations, with no arithmetic, one thread per cell in blocks of workgroup (64, 128 or 256; 0 for
