@@ -198,12 +198,24 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
 #pragma unroll
   for (int a = 0; a < Q; a++) lam[a] = 0.0f;
   const uint32_t e1 = at.ae0[k + 1];
-  for (uint32_t e = at.ae0[k]; e < e1; e++) {
-    const AdjEntry en = at.ent[e];
-    const float val = en.w * as.mu_next[en.src] + as.w_next * en.v;
+  // Entries four at a time: their table loads, then their gathers, are issued together instead of
+  // one dependent pair per entry; the sums still run in entry order.
+  for (uint32_t e = at.ae0[k]; e < e1; e += 4) {
+    AdjEntry en[4];
+    float mv[4];
 #pragma unroll
-    for (int a = 0; a < Q; a++)
-      if (en.a == uint32_t(a)) lam[a] += val;
+    for (int t = 0; t < 4; t++)
+      if (e + t < e1) en[t] = at.ent[e + t];
+#pragma unroll
+    for (int t = 0; t < 4; t++) mv[t] = e + t < e1 ? as.mu_next[en[t].src] : 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (e + t >= e1) break;
+      const float val = en[t].w * mv[t] + as.w_next * en[t].v;
+#pragma unroll
+      for (int a = 0; a < Q; a++)
+        if (en[t].a == uint32_t(a)) lam[a] += val;
+    }
   }
   float mu[Q];
   const float4 m = as.mom[idx];
@@ -240,9 +252,11 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
 }
 
 // ---------------------------------------------------------------------------- adjoint step kernels
-// Grid (gx, H, zl + D) in 3D, (gx, yl + H) in 2D: list blocks first, then the bulk rows.
+// Grid (gx, H, zl + D) in 3D, (gx, yl + H) in 2D: list blocks first, then the bulk rows. In 3D the
+// rows are walked top-down on every other reverse step (descend), so that a step starts on the rows
+// the previous one wrote last, as the forward step does.
 __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, const float4* __restrict__ xtab,
-                                                    Params3 p, uint32_t zl) {
+                                                    Params3 p, uint32_t zl, uint32_t descend) {
   const uint32_t N = p.N;
   if (blockIdx.z < zl) {
     const uint32_t k = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
@@ -251,9 +265,14 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  const uint32_t y = blockIdx.y, z = blockIdx.z - zl;
+  const uint32_t y = descend ? uint32_t(p.H) - 1u - blockIdx.y : blockIdx.y;
+  const uint32_t z = descend ? uint32_t(p.D) - 1u - (blockIdx.z - zl) : blockIdx.z - zl;
   const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
-  if (at.aflag[idx]) return;
+  // Rows off the y and z faces: every node's 19 downstream neighbours lie in the buffer (x = 0 and
+  // x = W - 1 reach into the adjacent rows), so the loads are issued before the node's flag is
+  // known and its latency overlaps theirs. Face rows test the flag first.
+  const bool inner = y > 0u && y + 1u < uint32_t(p.H) && z > 0u && z + 1u < uint32_t(p.D);
+  if (!inner && at.aflag[idx]) return;
   const uint32_t off = idx * 4u;
   const char* const mb = reinterpret_cast<const char*>(as.mu_next);
   float lam[19];
@@ -264,8 +283,9 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
     lam[a] = *reinterpret_cast<const float*>(base + off);
   }
   const float4 m = as.mom[idx];
-  const float u[3] = {m.y, m.z, m.w};
   const float4 t = xtab[x];
+  if (inner && at.aflag[idx]) return;
+  const float u[3] = {m.y, m.z, m.w};
   float mu[19];
   trt_vjp<19>(lam, m.x, u, t.x, t.y, t.z, as.ut, mu);
   char* const d = reinterpret_cast<char*>(as.mu);
@@ -569,13 +589,15 @@ int adjoint_step(lbm_solver* s, lbm_adjoint* a, int64_t n, int slot, const float
   as.dx = float(a->cfg.direction[0]);
   as.dy = float(a->cfg.direction[1]);
   as.dz = float(a->cfg.direction[2]);
-  const uint32_t bx = bulk_block_x(s->W, s->bulk_block);
+  // 3D: whole rows per block (up to 1024 threads), the shape the reverse step runs fastest in
+  const uint32_t bx = bulk_block_x(s->W, s->nd == 3 ? 1024 : s->bulk_block);
   const uint32_t gx = (uint32_t(s->W) + bx - 1) / bx;
   const uint32_t lblocks = (a->n_list + bx - 1) / bx;
   if (s->nd == 3) {
     const uint32_t per = gx * uint32_t(s->H);
     const uint32_t zl = (lblocks + per - 1) / per;
-    adj3_kernel<<<dim3(gx, uint32_t(s->H), zl + uint32_t(s->D)), bx, 0, s->stream>>>(at, as, s->xtab, s->p3, zl);
+    adj3_kernel<<<dim3(gx, uint32_t(s->H), zl + uint32_t(s->D)), bx, 0, s->stream>>>(at, as, s->xtab, s->p3, zl,
+                                                                                      uint32_t(n & 1));
   } else {
     const uint32_t yl = (lblocks + gx - 1) / gx;
     adj2_kernel<<<dim3(gx, yl + uint32_t(s->H)), bx, 0, s->stream>>>(at, as, s->xtab, s->p2, yl);

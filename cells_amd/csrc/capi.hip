@@ -535,6 +535,7 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
     float fin[19];
     inlet_feq3(s->p3.uIn, fin);
     const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    s->p3.seed = lowbias32(uint32_t(s->step));  // the stochastic codec's seed for this step
     // row order of this step: which buffer is walked top-down matters (measured), so both
     // alternations are candidates of the autotuner
     const int order = s->sweep & 3;
@@ -643,7 +644,7 @@ extern "C" {
 int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   if (!cfg || !out) return fail(LBM_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (cfg->precision < LBM_FP32 || cfg->precision > LBM_BF16) return fail(LBM_ERR_UNSUPPORTED, "unknown precision");
+  if (cfg->precision < LBM_FP32 || cfg->precision > LBM_FP16_SR) return fail(LBM_ERR_UNSUPPORTED, "unknown precision");
   if (cfg->width < 3 || cfg->height < 1 || cfg->depth < 1)
     return fail(LBM_ERR_INVALID, "width must be at least 3, height and depth positive");
   if (cfg->height > 65535 || cfg->depth > 65535)
@@ -1048,11 +1049,12 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 }
 
 // Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
-// for this W (and, in 3D, both row orders: alternating or always bottom-up) is timed over
-// trial_steps steps (after one warm-up step) on a copy of the solver's own state, the fastest is
-// kept, and the populations, step, parity and force ring are restored. Results are cached per
-// (lattice, precision, W, H, D) for the life of the process. None of the choices changes a result:
-// cells are independent within a step and the force sum is ordered by node, not by block.
+// for this W (and, in 3D, the three row orders and two store variants) is timed over trial_steps
+// steps (after two warm-up steps) on a copy of the solver's own state, the fastest is kept, and the
+// populations, step, parity and force ring are restored. Results are cached per (lattice,
+// precision, W, H, D) for the life of the process. Block width and row order do not change a result
+// (cells are independent within a step and the force sum is ordered by node, not by block); the
+// store variants are separate compilations and may differ in the last bits.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
   static std::mutex cache_mu;
@@ -1077,11 +1079,16 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   HIP_TRY(hipMemcpyAsync(save[1], s->f[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
   const int64_t step0 = s->step;
   const int parity0 = s->parity;
-  const int every = s->force_every();
   const size_t nsteps = s->force_steps.size();
-  // trials run without force samples so the ring is untouched
-  if (s->nd == 2) s->cfg2.force_every = 0;
-  else s->cfg3.force_every = 0;
+  // Trials sample forces as the solver is configured to (the sampling changes which launch shape
+  // wins); the force ring and its counter are saved here and restored afterwards.
+  const size_t ring_bytes = sizeof(double) * 3 * HISTORY_LEN;
+  void* save_ring = nullptr;
+  void* save_state = nullptr;
+  HIP_TRY(hipMalloc(&save_ring, ring_bytes));
+  HIP_TRY(hipMalloc(&save_state, 4));
+  HIP_TRY(hipMemcpyAsync(save_ring, s->ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(save_state, s->state, 4, hipMemcpyDeviceToDevice, s->stream));
   std::vector<int> tried;
   int best = s->bulk_block, best_sweep = s->sweep;
   double best_ms = 1e300;
@@ -1126,13 +1133,15 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  if (s->nd == 2) s->cfg2.force_every = every;
-  else s->cfg3.force_every = every;
   HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->ring, save_ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->state, save_state, 4, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   (void)hipFree(save[0]);
   (void)hipFree(save[1]);
+  (void)hipFree(save_ring);
+  (void)hipFree(save_state);
   s->step = step0;
   s->parity = parity0;
   s->force_steps.resize(nsteps);

@@ -41,7 +41,7 @@ struct Record {
   float* dq;
 };
 
-// ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 ----
+// ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 / LBM_FP16_SR ----
 // One time step in one launch: bulk rows (plain fluid, slip-face rows and plain inlet/outlet nodes,
 // one thread per cell) and boundary-list blocks (every other non-solid node, plus the force sample).
 // descend: walk the rows top-down this step (alternated step by step for cache reuse). merge: the

@@ -20,9 +20,24 @@
 namespace lbm {
 
 __device__ __forceinline__ float feq3(int i, float rho, float ux, float uy, float uz) {
-  // src/shaders/common3d.ts:78-81
+  // src/shaders/common3d.ts:78-81. Evaluated without contraction, so that every kernel that inlines it
+  // (whatever code surrounds it) rounds the same way.
+#pragma clang fp contract(off)
   const float cu = 3.0f * (float(CX3[i]) * ux + float(CY3[i]) * uy + float(CZ3[i]) * uz);
   return W3[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (ux * ux + uy * uy + uz * uz)));
+}
+
+// Stores of the time step: with a stochastic codec every value gets its random word (node word nw =
+// lowbias32(idx ^ seed), population i); otherwise round to nearest.
+template <class S>
+__device__ __forceinline__ void put_at(char* base, uint32_t off, float v, uint32_t nw, int i) {
+  if constexpr (S::stochastic) S::store_at_r(base, off, v, step_random(nw, i));
+  else S::store_at(base, off, v);
+}
+template <class S>
+__device__ __forceinline__ void put(typename S::elem* p, uint32_t k, float v, uint32_t nw, int i) {
+  if constexpr (S::stochastic) S::store_r(p, k, v, step_random(nw, i));
+  else S::store(p, k, v);
 }
 
 struct Mom3 {
@@ -298,10 +313,11 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   const size_t N = p.N;
   const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
+  const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
   if (!MERGE && inlet) {
     // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, fin.v[i]);
+    for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, fin.v[i], nw, i);
     return;
   }
   // (MERGE: the inlet node's pulls reach x = -1, the previous row's last node: in bounds, unused.)
@@ -337,7 +353,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   if (!MERGE && outlet) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+    for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
     return;
   }
   const float4 t = xtab[x];
@@ -356,7 +372,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     for (int i = 0; i < 19; i++) out[i] = fin.v[i];
   }
 #pragma unroll
-  for (int i = 0; i < 19; i++) S::store_at(d + i * N * sizeof(E), off, out[i]);
+  for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, out[i], nw, i);
 }
 
 __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restrict__ tab) {
@@ -382,9 +398,10 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
   const uint32_t fl = flags[idx];
+  const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
   if (fl & FLAG_INLET3) {
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, fin.v[i]);
+    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, fin.v[i], nw, i);
     return;
   }
   float f[19];
@@ -437,7 +454,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   if (REC) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
   if (fl & FLAG_OUTLET3) {
 #pragma unroll
-    for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz));
+    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
     return;
   }
   const float4 t = xtab[idx % uint32_t(p.W)];
@@ -449,7 +466,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   float out[19];
   collide3<SMAG>(f, m, r, p, out);
 #pragma unroll
-  for (int i = 0; i < 19; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+  for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, out[i], nw, i);
 }
 
 // Boundary-list part of the step: list block lb of the launch. List blocks [0, nlb) take the nodes
@@ -619,7 +636,10 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const Lis
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   const Record none{nullptr, nullptr};
-  if (rec)  // recording steps (the adjoint) never use the subgrid model
+  if (rec && merge)  // recording steps (the adjoint) never use the subgrid model
+    step3_kernel<S, true, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec,
+                                                             desc, row_clean);
+  else if (rec)
     step3_kernel<S, true, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec,
                                                               desc, row_clean);
   else if (p.smagC2 > 0.0f)
@@ -643,6 +663,9 @@ void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, c
     case 2:
       step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
       break;
+    case 3:
+      step3_t<StoreF16SR>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
+      break;
     default:
       step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
       break;
@@ -659,7 +682,8 @@ static void macro3_t(const void* f, const uint32_t* flags, float4* mac, const Pa
 }
 void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st) {
   switch (prec) {
-    case 1: macro3_t<StoreF16>(f, flags, mac, p, st); break;
+    case 1:  // FP16 with stochastic rounding: same format, initial and refilled values round to nearest
+    case 3: macro3_t<StoreF16>(f, flags, mac, p, st); break;
     case 2: macro3_t<StoreBF16>(f, flags, mac, p, st); break;
     default: macro3_t<StoreF32>(f, flags, mac, p, st); break;
   }
@@ -672,7 +696,8 @@ static void init3_t(void* fa, void* fb, const uint32_t* flags, const Params3& p,
 }
 void launch_init3(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st) {
   switch (prec) {
-    case 1: init3_t<StoreF16>(fa, fb, flags, p, st); break;
+    case 1:  // FP16 with stochastic rounding: same format, initial and refilled values round to nearest
+    case 3: init3_t<StoreF16>(fa, fb, flags, p, st); break;
     case 2: init3_t<StoreBF16>(fa, fb, flags, p, st); break;
     default: init3_t<StoreF32>(fa, fb, flags, p, st); break;
   }
@@ -687,7 +712,8 @@ static void flags3_t(const float* sdf, uint32_t* flags, void* fa, void* fb, cons
 void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
                    const Params3& p, hipStream_t st) {
   switch (prec) {
-    case 1: flags3_t<StoreF16>(sdf, flags, fa, fb, mac, p, st); break;
+    case 1:  // FP16 with stochastic rounding: same format, initial and refilled values round to nearest
+    case 3: flags3_t<StoreF16>(sdf, flags, fa, fb, mac, p, st); break;
     case 2: flags3_t<StoreBF16>(sdf, flags, fa, fb, mac, p, st); break;
     default: flags3_t<StoreF32>(sdf, flags, fa, fb, mac, p, st); break;
   }

@@ -69,38 +69,83 @@ __device__ __forceinline__ float bouzidi_dq(float q, float fj, float b, bool has
 // decode / encode one stored element; load(p, k) and store(p, k, v) address element k of p, while
 // load_at(base, off) and store_at(base, off, v) take a 32-bit byte offset from a wave-uniform base,
 // which lets the compiler use the SGPR-base + VGPR-offset addressing of global loads and stores.
+// A stochastic codec (FP16 with stochastic rounding) takes a 32-bit random word per stored value in
+// the time-step kernels (store_r / store_at_r, see step_random); every other store rounds to nearest.
 template <class Codec>
 struct StoreOps {
   using elem = typename Codec::elem;
+  static constexpr bool stochastic = Codec::stochastic;
   __device__ static __forceinline__ float load(const elem* p, uint32_t k) { return Codec::dec(p[k]); }
   __device__ static __forceinline__ void store(elem* p, uint32_t k, float v) { p[k] = Codec::enc(v); }
+  __device__ static __forceinline__ void store_r(elem* p, uint32_t k, float v, uint32_t r) { p[k] = Codec::enc_r(v, r); }
   __device__ static __forceinline__ float load_at(const char* base, uint32_t off) {
     return Codec::dec(*reinterpret_cast<const elem*>(base + off));
   }
   __device__ static __forceinline__ void store_at(char* base, uint32_t off, float v) {
     *reinterpret_cast<elem*>(base + off) = Codec::enc(v);
   }
+  __device__ static __forceinline__ void store_at_r(char* base, uint32_t off, float v, uint32_t r) {
+    *reinterpret_cast<elem*>(base + off) = Codec::enc_r(v, r);
+  }
 };
 struct CodecF32 {
   using elem = float;
+  static constexpr bool stochastic = false;
   __device__ static __forceinline__ float dec(elem e) { return e; }
   __device__ static __forceinline__ elem enc(float v) { return v; }
+  __device__ static __forceinline__ elem enc_r(float v, uint32_t) { return v; }
 };
 struct CodecF16 {
   using elem = _Float16;
+  static constexpr bool stochastic = false;
   __device__ static __forceinline__ float dec(elem e) { return float(e) * (1.0f / 32768.0f); }
   // v_cvt_f16_f32 rounds to nearest even
   __device__ static __forceinline__ elem enc(float v) { return _Float16(wclamp(v, -1.99f, 1.99f) * 32768.0f); }
+  __device__ static __forceinline__ elem enc_r(float v, uint32_t) { return enc(v); }
+};
+// The FP16 x 2^15 format of CodecF16 with stochastic rounding in the time step: a scaled value in the
+// f16 normal range (|x| >= 2^-14) becomes the f16 value below |x| after adding r >> 19 to the 13
+// mantissa bits f16 drops (v_cvt_sr_f16_f32; the rounding is up with probability equal to the dropped
+// fraction, so it carries no bias); smaller values round to nearest even.
+struct CodecF16SR {
+  using elem = _Float16;
+  static constexpr bool stochastic = true;
+  typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+  __device__ static __forceinline__ float dec(elem e) { return CodecF16::dec(e); }
+  __device__ static __forceinline__ elem enc(float v) { return CodecF16::enc(v); }
+  __device__ static __forceinline__ elem enc_r(float v, uint32_t r) {
+    const float x = wclamp(v, -1.99f, 1.99f) * 32768.0f;
+    const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{0, 0}, x, r, false);
+    return fabsf(x) < 6.103515625e-05f ? _Float16(x) : h[0];
+  }
 };
 struct CodecBF16 {
   using elem = uint16_t;
+  static constexpr bool stochastic = false;
   __device__ static __forceinline__ float dec(elem e) { return __uint_as_float(uint32_t(e) << 16); }
   // v_cvt_pk_bf16_f32 (gfx950) rounds to nearest even
   __device__ static __forceinline__ elem enc(float v) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v)); }
+  __device__ static __forceinline__ elem enc_r(float v, uint32_t) { return enc(v); }
 };
 struct StoreF32 : StoreOps<CodecF32> {};
 struct StoreF16 : StoreOps<CodecF16> {};
+struct StoreF16SR : StoreOps<CodecF16SR> {};
 struct StoreBF16 : StoreOps<CodecBF16> {};
+
+// Random words of the stochastic codec: lowbias32 (a 32-bit integer hash: xor-shift, multiply,
+// xor-shift, multiply, xor-shift) of the cell index xor the step's seed gives one word per node and
+// step; population i uses that word plus i times 0x9E3779B9. The seed of step n is lowbias32(n).
+__host__ __device__ __forceinline__ uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ uint32_t step_random(uint32_t node_word, int i) {
+  return node_word + uint32_t(i) * 0x9E3779B9u;
+}
 
 // ---------------------------------------------------------------- parameter blocks
 // Values of the reference's Params uniform (src/shaders/common.ts:55-63) after writeParams
@@ -116,6 +161,7 @@ struct Params3 {
   int32_t W, H, D;
   uint32_t N, WH;
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
+  uint32_t seed;  // lowbias32(step index): the stochastic codec's per-step seed
 };
 
 // Where the step kernels put a force sample: per-block FP64 partials, a completion counter,

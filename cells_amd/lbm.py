@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
 LBM_OK = 0
 X_PERIODIC, X_WALL, X_OPEN = 0, 1, 2  # XMode (src/lattice.ts:17-21)
 Y_PERIODIC, Y_NOSLIP, Y_SLIP, Y_MOVING = 0, 1, 2, 3  # YMode (src/lattice.ts:23-28)
-PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2, "fp16sr": 3}
 HISTORY_LEN = 8192
 MAGIC_LAMBDA = 3 / 16
 
@@ -357,7 +357,7 @@ class _Base:
 
 def decode(raw: np.ndarray, precision: int) -> np.ndarray:
     """Storage codec decode (src/shaders/common3d.ts:95): FP16 x 2^-15, BF16 as the top half of a float32."""
-    if precision == 1:
+    if precision in (1, 3):
         return raw.view(np.float16).astype(np.float32) * np.float32(1.0 / 32768.0)
     if precision == 2:
         return (raw.astype(np.uint32) << 16).view(np.float32)

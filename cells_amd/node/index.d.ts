@@ -3,7 +3,7 @@
 export declare const enum XModeValues { Periodic = 0, Wall = 1, Open = 2 }
 export declare const XMode: { readonly Periodic: 0; readonly Wall: 1; readonly Open: 2 };
 export declare const YMode: { readonly Periodic: 0; readonly NoSlip: 1; readonly Slip: 2; readonly MovingWall: 3 };
-export type Precision = 'fp16' | 'fp32' | 'bf16';
+export type Precision = 'fp16' | 'fp32' | 'bf16' | 'fp16sr';
 export declare const MAGIC_LAMBDA: number;
 export declare const HISTORY_LEN: number;
 export declare const PRESETS3: { readonly low: [number, number, number]; readonly medium: [number, number, number]; readonly high: [number, number, number] };

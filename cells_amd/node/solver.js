@@ -22,7 +22,7 @@ const XMode = Object.freeze({ Periodic: 0, Wall: 1, Open: 2 });
 const YMode = Object.freeze({ Periodic: 0, NoSlip: 1, Slip: 2, MovingWall: 3 });
 const MAGIC_LAMBDA = 3 / 16;
 const HISTORY_LEN = 8192;
-const PRECISION_CODES = Object.freeze({ fp32: 0, fp16: 1, bf16: 2 });
+const PRECISION_CODES = Object.freeze({ fp32: 0, fp16: 1, bf16: 2, fp16sr: 3 });
 const PRESETS3 = Object.freeze({ low: [128, 64, 64], medium: [192, 96, 96], high: [256, 128, 128] });
 
 function deviceOrdinal(device) {

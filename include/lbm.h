@@ -45,6 +45,10 @@ extern "C" {
 #define LBM_FP32 0
 #define LBM_FP16 1 /* f16 of (f - w) * 2^15, clamped to +-1.99 before scaling */
 #define LBM_BF16 2 /* bf16 of (f - w), round to nearest even, no clamp */
+/* The LBM_FP16 format, stochastically rounded in the time step: a value in the f16 normal range is
+ * rounded up with probability equal to the fraction f16 drops, using a random word that is a hash of
+ * (cell, population, step), so runs repeat exactly; initial and refilled values round to nearest. */
+#define LBM_FP16_SR 3
 
 /* Length of the on-device force history ring (src/shaders/aux.ts:4, aux3d.ts:5). */
 #define LBM_HISTORY_LEN 8192
@@ -182,9 +186,11 @@ int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
 int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms);
 /* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
  * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid (in 3D, each with
- * the row order alternating between steps and fixed) on a copy of the solver's state, keeps the
- * fastest (cached per lattice, precision and grid size for the process), and restores populations,
- * step and force ring. chosen (may be NULL) receives the width. No choice changes results. */
+ * the three row orders and the two store variants of lbm_launch_info) on a copy of the solver's
+ * state, with force samples as configured, keeps the fastest (cached per lattice, precision and grid
+ * size for the process), and restores populations, step and force ring. chosen (may be NULL)
+ * receives the width. Block width and row order never change a result; the two store variants agree
+ * to floating-point rounding (their compiled code may fuse different multiply-adds). */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
 /* No counterpart in the reference: the launch shape in use, threads per bulk block and (3D) the
  * variant: bits 0-1 the row order, 0 always bottom-up, 1 (2) top-down on the steps that read the
@@ -192,7 +198,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
  * inlet and outlet nodes share their wave's stores. */
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
 /* No counterpart in the reference: sets the launch shape by hand instead of lbm_autotune (block as
- * workgroup_size, variant as lbm_launch_info reports it; 0 in 2D). No choice changes results. */
+ * workgroup_size, variant as lbm_launch_info reports it; 0 in 2D), e.g. to pin a run's shape. */
 int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
 /* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
  * structure-of-arrays populations of elem_bytes each plus a u32 flag word per cell and writes the q

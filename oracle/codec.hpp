@@ -96,12 +96,14 @@ inline float bf16_to_f32(uint16_t b) { return bits_f32(uint32_t(b) << 16); }
 struct StoreF64 {
   using elem = double;
   static constexpr int bytes = 8;
+  static constexpr bool stochastic = false;
   template <class T> static T load(elem e) { return T(e); }
   template <class T> static elem store(T v) { return double(v); }
 };
 struct StoreF32 {
   using elem = float;
   static constexpr int bytes = 4;
+  static constexpr bool stochastic = false;
   template <class T> static T load(elem e) { return T(e); }
   template <class T> static elem store(T v) { return float(v); }
 };
@@ -109,6 +111,7 @@ struct StoreF32 {
 struct StoreF16 {
   using elem = uint16_t;
   static constexpr int bytes = 2;
+  static constexpr bool stochastic = false;
   template <class T> static T load(elem e) { return T(f16_to_f32(e) * (1.0f / 32768.0f)); }
   template <class T> static elem store(T v) {
     float c = float(v);
@@ -119,8 +122,53 @@ struct StoreF16 {
 struct StoreBF16 {
   using elem = uint16_t;
   static constexpr int bytes = 2;
+  static constexpr bool stochastic = false;
   template <class T> static T load(elem e) { return T(bf16_to_f32(e)); }
   template <class T> static elem store(T v) { return f32_to_bf16_rne(float(v)); }
+};
+
+// lowbias32, a 32-bit integer hash (xor-shift 16, multiply 0x7feb352d, xor-shift 15, multiply
+// 0x846ca68b, xor-shift 16). The stochastic codec's random word for population i of a node in the step
+// with index n is lowbias32(node ^ lowbias32(n)) + i * 0x9E3779B9 (mod 2^32).
+inline uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+inline uint32_t step_random(uint32_t node, uint32_t step, int i) {
+  return lowbias32(node ^ lowbias32(step)) + uint32_t(i) * 0x9E3779B9u;
+}
+
+// binary32 -> binary16 with stochastic rounding by the random word r, for magnitudes in the binary16
+// normal range [2^-14, 65504]: the top 13 bits of r are added to the 13 mantissa bits binary16 drops
+// and the sum is truncated (towards zero), so the magnitude rounds up with probability equal to the
+// dropped fraction; smaller magnitudes round to nearest even. (The gfx950 conversion the kernels use,
+// v_cvt_sr_f16_f32, behaves this way: checked bit for bit on 65,536 random values and words.)
+inline uint16_t f32_to_f16_sr(float f, uint32_t r) {
+  const uint32_t x = f32_bits(f);
+  const uint32_t ax = x & 0x7fffffffu;
+  if (ax < 0x38800000u) return f32_to_f16_rne(f);  // below 2^-14
+  const uint32_t sum = ax + (r >> 19);
+  const uint32_t e = (sum >> 23) - 112u;  // binary16 exponent field (no overflow below 65504 + 2^-13 ulp)
+  return uint16_t(((x >> 16) & 0x8000u) | (e << 10) | ((sum >> 13) & 0x3ffu));
+}
+
+// LBM_FP16 with stochastic rounding in the time step (store_r); store() rounds to nearest (initial and
+// refilled values).
+struct StoreF16SR {
+  using elem = uint16_t;
+  static constexpr int bytes = 2;
+  static constexpr bool stochastic = true;
+  template <class T> static T load(elem e) { return StoreF16::load<T>(e); }
+  template <class T> static elem store(T v) { return StoreF16::store<T>(v); }
+  template <class T> static elem store_r(T v, uint32_t r) {
+    float c = float(v);
+    c = std::fmin(std::fmax(c, -1.99f), 1.99f);
+    return f32_to_f16_sr(c * 32768.0f, r);
+  }
 };
 
 }  // namespace orc

@@ -59,6 +59,8 @@ def load():
         "orc_feq2": ([C.c_int, C.c_double, C.c_double, C.c_double], C.c_double),
         "orc_feq3": ([C.c_int, C.c_double, C.c_double, C.c_double, C.c_double], C.c_double),
         "orc_f32_to_f16": ([C.c_float], C.c_uint16),
+        "orc_f32_to_f16_sr": ([C.c_float, C.c_uint32], C.c_uint16),
+        "orc_step_random": ([C.c_uint32, C.c_uint32, C.c_int], C.c_uint32),
         "orc_f16_to_f32": ([C.c_uint16], C.c_float),
         "orc_f32_to_bf16": ([C.c_float], C.c_uint16),
         "orc_bf16_to_f32": ([C.c_uint16], C.c_float),
@@ -284,6 +286,14 @@ def feq3(i, rho, ux, uy, uz):
 
 def f32_to_f16(x: float) -> int:
     return int(load().orc_f32_to_f16(x))
+
+
+def f32_to_f16_sr(x: float, r: int) -> int:
+    return int(load().orc_f32_to_f16_sr(x, r))
+
+
+def step_random(node: int, step: int, i: int) -> int:
+    return int(load().orc_step_random(node, step, i))
 
 
 def f32_to_bf16(x: float) -> int:

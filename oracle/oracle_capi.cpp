@@ -146,6 +146,7 @@ void* orc_create_3d(const lbm_config3d* c, int real_bits) {
   if (real_bits == 64) return static_cast<Base*>(new Impl3<double, orc::StoreF64>(*c));
   switch (c->precision) {
     case LBM_FP16: return static_cast<Base*>(new Impl3<float, orc::StoreF16>(*c));
+    case LBM_FP16_SR: return static_cast<Base*>(new Impl3<float, orc::StoreF16SR>(*c));
     case LBM_BF16: return static_cast<Base*>(new Impl3<float, orc::StoreBF16>(*c));
     default: return static_cast<Base*>(new Impl3<float, orc::StoreF32>(*c));
   }
@@ -212,6 +213,8 @@ double orc_feq3(int i, double rho, double ux, double uy, double uz) {
 
 // Codec helpers, exported so tests can pin them against closed forms.
 uint16_t orc_f32_to_f16(float f) { return orc::f32_to_f16_rne(f); }
+uint16_t orc_f32_to_f16_sr(float f, uint32_t r) { return orc::f32_to_f16_sr(f, r); }
+uint32_t orc_step_random(uint32_t node, uint32_t step, int i) { return orc::step_random(node, step, i); }
 float orc_f16_to_f32(uint16_t h) { return orc::f16_to_f32(h); }
 uint16_t orc_f32_to_bf16(float f) { return orc::f32_to_bf16_rne(f); }
 float orc_bf16_to_f32(uint16_t b) { return orc::bf16_to_f32(b); }

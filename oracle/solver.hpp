@@ -601,6 +601,15 @@ struct Solver3D {
     return acc;
   }
 
+  // A value the time step stores (population i of node idx): the stochastic codec (LBM_FP16_SR) rounds
+  // it with the random word of (idx, step, i); the others round to nearest.
+  E put(T v, int64_t idx, int i) const {
+    if constexpr (S::stochastic)
+      return S::template store_r<T>(v, step_random(uint32_t(idx), uint32_t(step), i));
+    else
+      return S::template store<T>(v);
+  }
+
   // src/shaders/step3d.ts:170-223 for one cell.
   void step_cell(const std::vector<E>& src, std::vector<E>& dst, int64_t idx) {
     const uint32_t fl = flags[size_t(idx)];
@@ -616,7 +625,7 @@ struct Solver3D {
     } else {
       if (fl & FLAG_SOLID3) return;
       if (fl & FLAG_INLET3) {
-        for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = S::template store<T>(feq(i, T(1), P.uIn, T(0), T(0)));
+        for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), P.uIn, T(0), T(0)), idx, i);
         return;
       }
       outlet = (fl & FLAG_OUTLET3) != 0;
@@ -639,7 +648,7 @@ struct Solver3D {
     T uz = csum(CZ3, f) * invRho;
     T usq = T(1.5) * (ux * ux + uy * uy + uz * uz);
     if (outlet) {
-      for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = S::template store<T>(feq(i, T(1), ux, uy, uz));
+      for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), ux, uy, uz), idx, i);
       return;
     }
     // c_i . u in x, y, z order (step3d.ts:14-17, 47)
@@ -706,13 +715,13 @@ struct Solver3D {
       }
     }
     // collide (step3d.ts:92-102)
-    dst[size_t(idx)] = S::template store<T>(f[0] - omP * (f[0] - T(W3[0]) * (drho - rho * usq)) + s0);
+    dst[size_t(idx)] = put(f[0] - omP * (f[0] - T(W3[0]) * (drho - rho * usq)) + s0, idx, 0);
     for (int a = 1; a < 19; a += 2) {
       int b = a + 1;
       T sym = omP * (T(0.5) * (f[a] + f[b]) - eqS[a]);
       T asym = omM * (T(0.5) * (f[a] - f[b]) - eqA[a]);
-      dst[size_t(a * N + idx)] = S::template store<T>(f[a] - sym - asym + sS[a] + sA[a]);
-      dst[size_t(b * N + idx)] = S::template store<T>(f[b] - sym + asym + sS[a] - sA[a]);
+      dst[size_t(a * N + idx)] = put(f[a] - sym - asym + sS[a] + sA[a], idx, a);
+      dst[size_t(b * N + idx)] = put(f[b] - sym + asym + sS[a] - sA[a], idx, b);
     }
   }
 

@@ -163,3 +163,34 @@ def test_fp16_storage_roundtrip_error_bound():
     scaled = np.abs(v.astype(np.float64) * 32768.0)
     ulp = np.where(scaled < 2 ** -14, 2.0 ** -24, 2.0 ** (np.floor(np.log2(np.maximum(scaled, 2 ** -14))) - 10))
     assert np.all(np.abs(dec - v) <= 0.5 * ulp / 32768.0 + 1e-12)
+
+
+def test_f16_stochastic_rounding_closed_forms():
+    """The stochastic conversion (LBM_FP16_SR) against what rounding with a uniform 13-bit dither fixes:
+    every result is one of the two binary16 neighbours of x (RNE's candidates), a representable x is
+    returned for every word, the mean over all 8192 dither values is x exactly (no bias), the result
+    is odd in x, and below 2^-14 it is round-to-nearest-even."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.standard_normal(40).astype(np.float32) * 1000,
+                         (rng.standard_normal(40) * 0.01).astype(np.float32),
+                         np.array([6.103515625e-05, 65216.0, 1.0 + 2 ** -12, 2048.0 - 2 ** -3], dtype=np.float32)])
+    for x in xs:
+        if abs(x) < 2 ** -14:
+            continue
+        lo = np.float16(x)  # RNE result: one of the neighbours
+        outs = np.array([O.f32_to_f16_sr(float(x), int(d) << 19) for d in range(8192)], dtype=np.uint16)
+        vals = outs.view(np.float16).astype(np.float64)
+        assert set(np.unique(vals)) <= {float(lo), float(np.nextafter(lo, np.float16(np.inf))),
+                                        float(np.nextafter(lo, np.float16(-np.inf)))}
+        assert abs(vals.mean() - np.float64(x)) <= 1e-12 * abs(np.float64(x)), (x, vals.mean())
+        assert O.f32_to_f16_sr(float(-x), 12345 << 19) == O.f32_to_f16_sr(float(x), 12345 << 19) ^ 0x8000
+    for x in np.array([1.0, 0.5, 1024.0, -3.25, 65504.0 / 2], dtype=np.float32):
+        assert {O.f32_to_f16_sr(float(x), int(r)) for r in rng.integers(0, 2 ** 32, 64)} == {O.f32_to_f16(float(x))}
+    for x in (1e-6, -3e-7, 5.9e-8, 0.0):
+        for r in (0, 2 ** 31, 2 ** 32 - 1):
+            assert O.f32_to_f16_sr(x, r) == O.f32_to_f16(x)
+    # the random words of distinct (node, step, population) differ and use all of their top 13 bits
+    words = np.array([O.step_random(n, s, i) for n in range(40) for s in range(5) for i in range(19)], dtype=np.uint64)
+    assert len(np.unique(words)) == words.size
+    top = words >> np.uint64(19)
+    assert abs(top.mean() / 4095.5 - 1) < 0.05 and top.min() < 400 and top.max() > 7800
