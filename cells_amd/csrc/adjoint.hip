@@ -52,8 +52,8 @@ struct AdjTables {
   const uint8_t* aflag;   // per cell: 0 = adjoint bulk node
   const uint32_t* aidx;   // adjoint-list cells
   const uint32_t* akind;  // their kinds
-  const uint32_t* ae0;    // entries of node k are [ae0[k], ae0[k + 1])
-  const AdjEntry* ent;
+  const uint32_t* ae0;    // node k has ae0[k + 1] - ae0[k] entries
+  const AdjEntry* ent;    // entry j of node k at ent[j * n_list + k]
   const uint32_t* alk;    // forward list position of the node when it has Bouzidi links, else NO_OPERAND
   const uint32_t* bmask;  // forward tables (Bouzidi directions, first record)
   const uint32_t* lbase;
@@ -197,15 +197,15 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
   float lam[Q];
 #pragma unroll
   for (int a = 0; a < Q; a++) lam[a] = 0.0f;
-  const uint32_t e1 = at.ae0[k + 1];
+  const uint32_t e1 = at.ae0[k + 1] - at.ae0[k];  // entry count; entry j is at.ent[j * n_list + k]
   // Entries four at a time: their table loads, then their gathers, are issued together instead of
   // one dependent pair per entry; the sums still run in entry order.
-  for (uint32_t e = at.ae0[k]; e < e1; e += 4) {
+  for (uint32_t e = 0; e < e1; e += 4) {
     AdjEntry en[4];
     float mv[4];
 #pragma unroll
     for (int t = 0; t < 4; t++)
-      if (e + t < e1) en[t] = at.ent[e + t];
+      if (e + t < e1) en[t] = at.ent[size_t(e + t) * at.n_list + k];
 #pragma unroll
     for (int t = 0; t < 4; t++) mv[t] = e + t < e1 ? as.mu_next[en[t].src] : 0.0f;
 #pragma unroll
@@ -553,12 +553,19 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
     if (v != NO_OPERAND && v >= s->h_list.size()) return fail(LBM_ERR_DEVICE, "adjoint tables: link node out of range");
   free_tables(a);
   const size_t nl = alist.size();
+  // Entries column-major ("ELL"): entry j of list node k at j * n_list + k, so the threads of a wave
+  // (consecutive list nodes) read their j-th entries from consecutive addresses.
+  uint32_t max_e = 0;
+  for (size_t k = 0; k < nl; k++) max_e = std::max(max_e, ne[k + 1] - ne[k]);
+  std::vector<AdjEntry> ell(size_t(max_e) * nl, AdjEntry{0u, 0u, 0.0f, 0.0f});
+  for (size_t k = 0; k < nl; k++)
+    for (uint32_t j = 0; j < ne[k + 1] - ne[k]; j++) ell[size_t(j) * nl + k] = ent[ne[k] + j];
   HIP_TRY(hipMalloc(&a->aflag, N));
   HIP_TRY(hipMalloc(&a->aidx, 4 * std::max<size_t>(1, nl)));
   HIP_TRY(hipMalloc(&a->akind, 4 * std::max<size_t>(1, nl)));
   HIP_TRY(hipMalloc(&a->alk, 4 * std::max<size_t>(1, nl)));
   HIP_TRY(hipMalloc(&a->ae0, 4 * (nl + 1)));
-  HIP_TRY(hipMalloc(&a->ent, sizeof(AdjEntry) * std::max<size_t>(1, ent.size())));
+  HIP_TRY(hipMalloc(&a->ent, sizeof(AdjEntry) * std::max<size_t>(1, ell.size())));
   HIP_TRY(hipMemcpy(a->aflag, aflag.data(), N, hipMemcpyHostToDevice));
   if (nl) {
     HIP_TRY(hipMemcpy(a->aidx, alist.data(), 4 * nl, hipMemcpyHostToDevice));
@@ -566,7 +573,7 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
     HIP_TRY(hipMemcpy(a->alk, lk.data(), 4 * nl, hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemcpy(a->ae0, ne.data(), 4 * (nl + 1), hipMemcpyHostToDevice));
-  if (!ent.empty()) HIP_TRY(hipMemcpy(a->ent, ent.data(), sizeof(AdjEntry) * ent.size(), hipMemcpyHostToDevice));
+  if (!ell.empty()) HIP_TRY(hipMemcpy(a->ent, ell.data(), sizeof(AdjEntry) * ell.size(), hipMemcpyHostToDevice));
   a->n_list = uint32_t(nl);
   a->n_ent = ent.size();
   a->tables_gen = s->tables_gen;
