@@ -275,11 +275,28 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
   if (!inner && at.aflag[idx]) return;
   const uint32_t off = idx * 4u;
   const char* const mb = reinterpret_cast<const char*>(as.mu_next);
+  // On a y / z face row a population that would stream off the grid is read back from the node and
+  // direction the forward step's mirrored (one face) or bounce-back (two faces) pull takes it to;
+  // the pattern is uniform per row (the host admits only nodes that follow it).
+  const bool loY = y == 0u, hiY = y + 1u == uint32_t(p.H), loZ = z == 0u, hiZ = z + 1u == uint32_t(p.D);
   float lam[19];
 #pragma unroll
   for (int a = 0; a < 19; a++) {
-    const ptrdiff_t delta = ptrdiff_t(CZ3[a]) * p.WH + ptrdiff_t(CY3[a]) * p.W + CX3[a];
-    const char* base = mb + (ptrdiff_t(a) * ptrdiff_t(N) + delta) * 4;
+    const bool oY = (loY && CY3[a] < 0) || (hiY && CY3[a] > 0);
+    const bool oZ = (loZ && CZ3[a] < 0) || (hiZ && CZ3[a] > 0);
+    int pa = a;
+    ptrdiff_t delta = ptrdiff_t(CZ3[a]) * p.WH + ptrdiff_t(CY3[a]) * p.W + CX3[a];
+    if (oY && oZ) {
+      pa = opp3(a);
+      delta = 0;
+    } else if (oY) {
+      pa = mirror_y3(a);
+      delta = ptrdiff_t(CZ3[a]) * p.WH + CX3[a];
+    } else if (oZ) {
+      pa = mirror_z3(a);
+      delta = ptrdiff_t(CY3[a]) * p.W + CX3[a];
+    }
+    const char* base = mb + (ptrdiff_t(pa) * ptrdiff_t(N) + delta) * 4;
     lam[a] = *reinterpret_cast<const float*>(base + off);
   }
   const float4 m = as.mom[idx];
@@ -454,6 +471,42 @@ int pull_sources(const lbm_solver* s, uint32_t idx, uint32_t pos, int i, const d
   return 1;
 }
 
+// Where the reverse step's bulk rows take lambda_a(u) from: the node and direction whose pull reads
+// the post-collision f_a(u). Off the faces that is the plain pull of u + c_a in direction a. On a 3D
+// slip-face row (y = 0 or H-1, z = 0 or D-1) a population that would stream off the grid across one
+// face is read by the forward step's mirrored pull (src/shaders/step3d.ts:132-145) of the node
+// u + c_a reflected back onto the row, in the mirrored direction; across two faces, by u's own
+// bounce-back pull in the opposite direction. The same rule, per row, drives adj3_kernel; the host
+// admits a node to the bulk rows only if every population of it is read exactly this way.
+void expected_reader(const lbm_solver* s, uint32_t u, int a, uint32_t* x, int* i) {
+  const int64_t W = s->W, H = s->H, D = s->D;
+  const int64_t ux = u % W, uy = (u / W) % H, uz = u / (W * H);
+  if (s->nd == 2) {
+    const int64_t vx = ux + CX2[a], vy = uy + CY2[a];
+    if (vx < 0 || vx >= W || vy < 0 || vy >= H) return;
+    *x = uint32_t(vx + W * vy);
+    *i = a;
+    return;
+  }
+  const bool oY = (uy == 0 && CY3[a] < 0) || (uy == H - 1 && CY3[a] > 0);
+  const bool oZ = (uz == 0 && CZ3[a] < 0) || (uz == D - 1 && CZ3[a] > 0);
+  int64_t vx = ux + CX3[a], vy = uy + CY3[a], vz = uz + CZ3[a];
+  int ia = a;
+  if (oY && oZ) {
+    vx = ux, vy = uy, vz = uz;
+    ia = opp3(a);
+  } else if (oY) {
+    vy = uy;
+    ia = mirror_y3(a);
+  } else if (oZ) {
+    vz = uz;
+    ia = mirror_z3(a);
+  }
+  if (vx < 0 || vx >= W || vy < 0 || vy >= H || vz < 0 || vz >= D) return;
+  *x = uint32_t(vx + W * (vy + H * vz));
+  *i = ia;
+}
+
 // Builds the adjoint tables for the solver's current geometry (after every rebuild of the list).
 int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   const uint32_t N = s->N;
@@ -464,23 +517,24 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   std::vector<uint32_t> pos(N, NO_OPERAND);
   for (uint32_t k = 0; k < s->h_list.size(); k++) pos[s->h_list[k]] = k;
   auto pulls = [&](uint32_t x) { return !(s->h_flags[x] & solid) && !(!d2 && (s->h_flags[x] & FLAG_INLET3)); };
-  // pass 1: per target population, how many sources read it and whether the only one is the plain
-  // pull of the downstream neighbour (bit 7 marks any other kind)
+  // pass 1: per target population, how many sources read it and whether the only one is the read
+  // the reverse step's bulk rows assume (bit 7 marks any other kind): the plain pull of the
+  // downstream neighbour, or in 3D on a y / z face row the forward step's row pattern read backwards
+  // (expected_reader)
   std::vector<uint8_t> cnt(size_t(Q) * N, 0);
   Src src[2];
   for (uint32_t x = 0; x < N; x++) {
     if (!pulls(x)) continue;
-    const uint32_t cx = x % W, cy = (x / W) % H, cz = x / WH;
     for (int i = 0; i < Q; i++) {
       const int ns = pull_sources(s, x, pos[x], i, dir, src);
-      const int ci = d2 ? CX2[i] : CX3[i], cj = d2 ? CY2[i] : CY3[i], ck = d2 ? 0 : CZ3[i];
-      const int64_t ux = int64_t(cx) - ci, uy = int64_t(cy) - cj, uz = int64_t(cz) - ck;
-      const bool in = ux >= 0 && ux < W && uy >= 0 && uy < H && uz >= 0 && uz < D;
-      const uint32_t plain = in ? uint32_t(i) * N + uint32_t(ux + W * (uy + H * uz)) : NO_OPERAND;
       for (int e = 0; e < ns; e++) {
         uint8_t& c = cnt[src[e].src];
         if ((c & 0x7f) < 0x7f) c++;
-        if (!(src[e].src == plain && src[e].w == 1.0f && src[e].v == 0.0f)) c |= 0x80;
+        const uint32_t a = src[e].src / N, u = src[e].src % N;
+        uint32_t ex = NO_OPERAND;
+        int ei = -1;
+        expected_reader(s, u, int(a), &ex, &ei);
+        if (!(ex == x && ei == i && src[e].w == 1.0f && src[e].v == 0.0f)) c |= 0x80;
       }
     }
   }
@@ -540,11 +594,11 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   // and every entry's source and target.
   for (uint32_t y = 0; y < N; y++) {
     if (aflag[y]) continue;
-    const int64_t cx = y % W, cy = (y / W) % H, cz = y / WH;
     for (int i = 0; i < Q; i++) {
-      const int64_t nx = cx + (d2 ? CX2[i] : CX3[i]), ny = cy + (d2 ? CY2[i] : CY3[i]), nz = cz + (d2 ? 0 : CZ3[i]);
-      if (nx < 0 || nx >= W || ny < 0 || ny >= H || nz < 0 || nz >= D)
-        return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node has a neighbour off the grid");
+      uint32_t ex = NO_OPERAND;
+      int ei = -1;
+      expected_reader(s, y, i, &ex, &ei);
+      if (ex == NO_OPERAND) return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node has a reader off the grid");
     }
   }
   for (const AdjEntry& en : ent)
