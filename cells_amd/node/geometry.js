@@ -185,6 +185,35 @@ function mean(v) {
   return s / v.length;
 }
 
+/** streamFunction (src/analysis.ts:28-37): psi[y * W + x] from the velocity components. */
+function streamFunction(ux, uy, W, H) {
+  const psi = new Float64Array(W * H);
+  for (let y = 1; y < H; y++) psi[y * W] = psi[(y - 1) * W] + 0.5 * (ux[y * W] + ux[(y - 1) * W]);
+  for (let y = 0; y < H; y++) {
+    for (let x = 1; x < W; x++) {
+      const i = y * W + x;
+      psi[i] = psi[i - 1] - 0.5 * (uy[i] + uy[i - 1]);
+    }
+  }
+  return psi;
+}
+
+/** primaryVortex (src/analysis.ts:40-55): extremum of |psi| refined by a quadratic fit, [x, y, psi]. */
+function primaryVortex(psi, W, H) {
+  let best = 1;
+  for (let y = 1; y < H - 1; y++) {
+    for (let x = 1; x < W - 1; x++) if (Math.abs(psi[y * W + x]) > Math.abs(psi[best])) best = y * W + x;
+  }
+  const bx = best % W;
+  const by = Math.floor(best / W);
+  const refine = (m, c, p) => {
+    const den = m - 2 * c + p;
+    return den === 0 ? 0 : (0.5 * (m - p)) / den;
+  };
+  return [bx + refine(psi[best - 1], psi[best], psi[best + 1]), by + refine(psi[best - W], psi[best], psi[best + W]),
+    psi[best]];
+}
+
 /** Period from linearly interpolated upward crossings of the mean; null with fewer than 3 crossings. */
 function periodFromCrossings(t, v) {
   const m = mean(v);
@@ -213,6 +242,7 @@ function fieldDiverged(values) {
 
 module.exports = {
   FAR, TAU_MIN, emptySdf, addCircle, addBox, nacaPolygon, addPolygon, polygonDistance, emptySdf3, addSphere, addBox3,
+  streamFunction, primaryVortex,
   addCylinderZ, addWing, referenceArea, deriveTau, maxReynolds, mean, periodFromCrossings, coefficientScale,
   hasDiverged, fieldDiverged,
 };

@@ -119,6 +119,8 @@ export declare function referenceArea(obstacle: 'sphere' | 'cube' | 'cylinder' |
 export declare function deriveTau(re: number, uLat: number, length: number): { nu: number; tau: number; clamped: boolean; effectiveRe: number; mach: number };
 export declare function maxReynolds(uLat: number, length: number): number;
 export declare function mean(v: number[]): number;
+export declare function streamFunction(ux: ArrayLike<number>, uy: ArrayLike<number>, W: number, H: number): Float64Array;
+export declare function primaryVortex(psi: ArrayLike<number>, W: number, H: number): [number, number, number];
 export declare function periodFromCrossings(t: number[], v: number[]): { period: number; crossings: number } | null;
 export declare function coefficientScale(uTarget: number, area: number): number;
 export declare function hasDiverged(samples: ReadonlyArray<{ fx: number; fy: number; fz?: number }>): boolean;
