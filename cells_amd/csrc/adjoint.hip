@@ -512,7 +512,6 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   const uint32_t N = s->N;
   const int Q = s->Q;
   const bool d2 = s->nd == 2;
-  const uint32_t W = uint32_t(s->W), H = uint32_t(s->H), D = uint32_t(s->D), WH = W * H;
   const uint32_t solid = d2 ? FLAG_SOLID2 : FLAG_SOLID3;
   std::vector<uint32_t> pos(N, NO_OPERAND);
   for (uint32_t k = 0; k < s->h_list.size(); k++) pos[s->h_list[k]] = k;
@@ -674,6 +673,8 @@ int check_supported(const lbm_solver* s) {
       return fail(LBM_ERR_UNSUPPORTED, "adjoint: body force is not supported");
   } else if (s->cfg3.smagorinsky > 0) {
     return fail(LBM_ERR_UNSUPPORTED, "adjoint: Smagorinsky is not supported");
+  } else if (s->faces != LBM_FACES_EQUILIBRIUM) {
+    return fail(LBM_ERR_UNSUPPORTED, "adjoint: only the equilibrium inlet and outlet planes are supported");
   }
   return LBM_OK;
 }

@@ -136,6 +136,7 @@ void write_params3(lbm_solver* s) {
   p.spongeStart = float(c.sponge_fraction > 0 ? (s->W - 1) * (1 - c.sponge_fraction) : 1e9);
   p.tauSponge = float(std::max(c.sponge_tau, c.tau));
   p.absorb = float(c.absorb);
+  p.faces = uint32_t(s->faces);
   if (s->xtab) launch_xtab3(p, s->xtab, s->stream);
 }
 
@@ -181,7 +182,10 @@ int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
   }
   const bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
   const bool outlet = (f & FLAG_OUTLET3) != 0u;
-  if (outX && outlet) {
+  if (outX && (f & FLAG_INLET3)) {
+    // entering from x = -1: a Zou-He inlet face rebuilds it (the equilibrium inlet pulls nothing)
+    *g = uint32_t(j) * N + idx;
+  } else if (outX && outlet) {
     *g = uint32_t(i) * N + at(W - 2, y, z);
   } else if (outY && outZ) {
     *g = uint32_t(j) * N + idx;
@@ -301,9 +305,13 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
       // pattern is the one the reference's rules give; otherwise it is marked for the list.
       const uint32_t WH = uint32_t(s->W) * uint32_t(s->H);
       const Row3 r = row3(s, (k % WH) / uint32_t(s->W), k / WH);
+      // (With Zou-He faces the inlet node pulls too; the directions entering from x = -1 are rebuilt,
+      // so only the others are compared.)
       const int bk = bulk_kind3(r, v);
       bool ok = bk != 0;
-      for (int i = 1; i < Q && bk == 1 && ok; i++) {
+      const bool check = bk == 1 || (bk == 2 && s->faces != LBM_FACES_EQUILIBRIUM);
+      for (int i = 1; i < Q && check && ok; i++) {
+        if (bk == 2 && CX3[i] > 0) continue;
         uint32_t gi;
         BzLink rec;
         ok = resolve(k, i, &gi, &rec) == PULL_GATHER && gi == row_gather3(s, r, k, (v & outlet_flag) != 0u, i);
@@ -319,7 +327,8 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
     }
     const int kind = (v & inlet_flag) ? 0 : (v & outlet_flag) ? 1 : 2;
     bool has_body = false;
-    if (!(!d2 && kind == 0)) {  // the 3D inlet takes no pulls
+    const bool pulls = d2 || kind != 0 || s->faces != LBM_FACES_EQUILIBRIUM;  // the 3D equilibrium inlet takes none
+    if (pulls) {
       for (int i = 1; i < Q && !has_body; i++) {
         uint32_t gi;
         BzLink rec;
@@ -342,7 +351,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
   for (uint32_t k = 0; k < nl; k++) {
     const uint32_t idx = lst[k];
     lb[k] = uint32_t(recs.size());
-    if (!d2 && (fl[idx] & inlet_flag)) {
+    if (!d2 && (fl[idx] & inlet_flag) && s->faces == LBM_FACES_EQUILIBRIUM) {
       for (int i = 1; i < Q; i++) gt[size_t(i - 1) * nl + k] = idx;  // never read
       continue;
     }
@@ -746,6 +755,17 @@ int lbm_set_sdf(lbm_solver* s, const float* sdf, size_t n) {
 
 int lbm_rebuild_flags(lbm_solver* s) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  return rebuild_flags_impl(s);
+}
+
+int lbm_set_open_faces(lbm_solver* s, int kind) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (s->nd != 3) return fail(LBM_ERR_INVALID, "open faces: 3D solvers only (the 2D inlet and outlet are Zou-He already)");
+  if (kind != LBM_FACES_EQUILIBRIUM && kind != LBM_FACES_ZOU_HE) return fail(LBM_ERR_INVALID, "open faces: unknown kind");
+  if (kind == s->faces) return LBM_OK;
+  s->faces = kind;
+  write_params3(s);
+  // which inlet nodes the bulk pass can take, and the inlet list nodes' gathers, depend on the kind
   return rebuild_flags_impl(s);
 }
 

@@ -27,6 +27,45 @@ __device__ __forceinline__ float feq3(int i, float rho, float ux, float uy, floa
   return W3[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (ux * ux + uy * uy + uz * uz)));
 }
 
+// Zou-He velocity (x = 0, u = (uIn, 0, 0)) and pressure (x = W-1, rho = 1, no transverse velocity)
+// faces after Hecht and Harting, J. Stat. Mech. (2010) P01018 (arXiv:0811.4593), in the f - w
+// variables. The populations entering from outside the grid (c_x = +1 at the inlet, -1 at the
+// outlet) are rebuilt from their opposites,
+//   f_i = f_opp(i) + 6 w_i rho (c_i . u) - c_iy N_y - c_iz N_z,  N_a = 1/2 sum_{c_x = 0} c_a f,
+// with rho (1 - uIn) = 1 + sum_{c_x = 0} f + 2 sum_{c_x = -1} f at the inlet and
+// u_x = sum_{c_x = 0} f + 2 sum_{c_x = +1} f at the outlet, so that the node's density and
+// momentum are exactly the prescribed ones; the node then collides. On slip-face rows the mirrored
+// populations count as known, so the planes' edges and corners take the same rule.
+template <bool IN>
+__device__ __forceinline__ void zou_he_face3(float (&f)[19], float uIn) {
+  constexpr int in = IN ? 1 : -1;
+  float s0 = 0.0f, sk = 0.0f, ny = 0.0f, nz = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 19; i++) {
+    if (CX3[i] == 0) {
+      s0 = s0 + f[i];
+      if (CY3[i]) ny = CY3[i] > 0 ? ny + f[i] : ny - f[i];
+      if (CZ3[i]) nz = CZ3[i] > 0 ? nz + f[i] : nz - f[i];
+    } else if (CX3[i] == -in) {
+      sk = sk + f[i];
+    }
+  }
+  ny = 0.5f * ny;
+  nz = 0.5f * nz;
+  float ru;
+  if (IN) ru = (1.0f + s0 + 2.0f * sk) / (1.0f - uIn) * uIn;
+  else ru = s0 + 2.0f * sk;
+#pragma unroll
+  for (int i = 1; i < 19; i++) {
+    if (CX3[i] != in) continue;
+    const float k = (CY3[i] || CZ3[i]) ? 1.0f / 6.0f : 1.0f / 3.0f;  // 6 w_i
+    float v = IN ? f[opp3(i)] + k * ru : f[opp3(i)] - k * ru;
+    if (CY3[i]) v = CY3[i] > 0 ? v - ny : v + ny;
+    if (CZ3[i]) v = CZ3[i] > 0 ? v - nz : v + nz;
+    f[i] = v;
+  }
+}
+
 // Stores of the time step: with a stochastic codec every value gets its random word (node word nw =
 // lowbias32(idx ^ seed), population i); otherwise round to nearest.
 template <class S>
@@ -314,13 +353,15 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
-  if (!MERGE && inlet) {
+  const bool zh = p.faces != 0u;
+  if (!MERGE && inlet && !zh) {
     // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
 #pragma unroll
     for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, fin.v[i], nw, i);
     return;
   }
-  // (MERGE: the inlet node's pulls reach x = -1, the previous row's last node: in bounds, unused.)
+  // (MERGE or Zou-He faces: the inlet node's c_x = +1 pulls reach x = -1, the previous row's last
+  // node: in bounds, never used.)
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[19];
 #pragma unroll
@@ -348,9 +389,11 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     if (CX3[i] < 0 && outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
     f[i] = S::load_at(base, o);
   }
+  if (zh && inlet) zou_he_face3<true>(f, p.uIn);
+  else if (zh && outlet) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
   if (REC && !inlet) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
-  if (!MERGE && outlet) {
+  if (!MERGE && outlet && !zh) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
     for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
@@ -364,10 +407,10 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   r.t0 = t.w;
   float out[19];
   collide3<SMAG>(f, m, r, p, out);
-  if (MERGE && outlet) {
+  if (MERGE && outlet && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) out[i] = feq3(i, 1.0f, m.ux, m.uy, m.uz);
-  } else if (MERGE && inlet) {
+  } else if (MERGE && inlet && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) out[i] = fin.v[i];
   }
@@ -399,7 +442,8 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   const uint32_t idx = lt.list[k];
   const uint32_t fl = flags[idx];
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
-  if (fl & FLAG_INLET3) {
+  const bool zh = p.faces != 0u;
+  if ((fl & FLAG_INLET3) && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, fin.v[i], nw, i);
     return;
@@ -450,9 +494,11 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
       }
     }
   }
+  if (zh && (fl & FLAG_INLET3)) zou_he_face3<true>(f, p.uIn);
+  else if (zh && (fl & FLAG_OUTLET3)) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
   if (REC) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
-  if (fl & FLAG_OUTLET3) {
+  if ((fl & FLAG_OUTLET3) && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
     return;

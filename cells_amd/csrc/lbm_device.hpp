@@ -161,7 +161,8 @@ struct Params3 {
   int32_t W, H, D;
   uint32_t N, WH;
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
-  uint32_t seed;  // lowbias32(step index): the stochastic codec's per-step seed
+  uint32_t seed;   // lowbias32(step index): the stochastic codec's per-step seed
+  uint32_t faces;  // open x planes: 0 equilibrium (the reference's), 1 Zou-He faces (Hecht-Harting)
 };
 
 // Where the step kernels put a force sample: per-block FP64 partials, a completion counter,

@@ -65,6 +65,7 @@ struct lbm_solver {
   lbm::Params2 p2{};
   lbm::Params3 p3{};
   lbm::Ramp ramp;
+  int faces = LBM_FACES_EQUILIBRIUM;  // 3D open x planes (lbm_set_open_faces)
   int bulk_block = 256;
   // 3D launch variant (autotuned). Bits 0-1, row order: 0 = always bottom-up; 1 / 2 = top-down on
   // the steps that read the buffer at the higher / lower address, bottom-up on the others. Bit 2

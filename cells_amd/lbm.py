@@ -140,6 +140,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_update_2d": ([P, C.POINTER(Config2D)], C.c_int),
         "lbm_update_3d": ([P, C.POINTER(Config3D)], C.c_int),
         "lbm_set_inlet_ramp": ([P, C.c_int, C.c_double, C.c_double], C.c_int),
+        "lbm_set_open_faces": ([P, C.c_int], C.c_int),
         "lbm_set_sdf": ([P, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_rebuild_flags": ([P], C.c_int),
         "lbm_brush": ([P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int], C.c_int),
@@ -441,6 +442,11 @@ class Solver3D(_Base):
         """setInlet (src/solver3d.ts:192-195)."""
         self.cfg["u_in"] = u
         self._push()
+
+    def set_open_faces(self, kind: str):
+        """Open x planes: 'equilibrium' (the reference's, src/shaders/step3d.ts:190-209) or 'zou-he'
+        (Hecht-Harting velocity inlet and rho = 1 pressure outlet, the nodes then collide)."""
+        _check(_lib.lbm_set_open_faces(self._h, {"equilibrium": 0, "zou-he": 1}[kind]))
 
     def init_field(self):
         """initField (src/solver3d.ts:231-242): inflow equilibrium in fluid, zero in solids."""

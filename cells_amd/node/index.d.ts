@@ -79,6 +79,8 @@ export declare class Solver3D extends SolverBase {
   setTau(tau: number): void;
   setSmagorinsky(cs: number): void;
   setInlet(u: number): void;
+  /** Open x planes: the reference's equilibrium planes (default) or Hecht-Harting Zou-He faces. */
+  setOpenFaces(kind: 'equilibrium' | 'zouHe'): void;
   initField(): void;
   readForces(): Promise<ForceSample3[]>;
 }

@@ -306,6 +306,18 @@ static napi_value js_rebuild_flags(napi_env env, napi_callback_info info) {
   return rc ? throw_status(env, rc) : NULL;
 }
 
+/* setOpenFaces(handle, kind) */
+static napi_value js_set_open_faces(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  int32_t kind = -1;
+  napi_get_value_int32(env, argv[1], &kind);
+  int rc = lbm_set_open_faces(s, kind);
+  return rc ? throw_status(env, rc) : NULL;
+}
+
 /* brush(handle, cx, cy, r, erase, rebuild) */
 static napi_value js_brush(napi_env env, napi_callback_info info) {
   napi_value argv[6];
@@ -718,6 +730,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("setInletRamp", js_set_inlet_ramp),
       EXPORT("setSdf", js_set_sdf),
       EXPORT("rebuildFlags", js_rebuild_flags),
+      EXPORT("setOpenFaces", js_set_open_faces),
       EXPORT("deviceName", js_device_name),
       EXPORT("copyRoofline", js_copy_roofline),
       EXPORT("autotune", js_autotune),

@@ -251,6 +251,15 @@ class Solver3D extends Base {
     this.cfg.uIn = u;
     this._push();
   }
+  /**
+   * Open x planes (an extension): 'equilibrium', the reference's (src/shaders/step3d.ts:190-209), or
+   * 'zouHe', Hecht-Harting velocity inlet and rho = 1 pressure outlet whose nodes then collide.
+   */
+  setOpenFaces(kind) {
+    const k = { equilibrium: 0, zouHe: 1 }[kind];
+    if (k === undefined) throw new Error(`unknown open faces ${kind}`);
+    addon().setOpenFaces(this.handle, k);
+  }
   /** initField (src/solver3d.ts:231-242): inflow equilibrium in fluid cells, rest in solids. */
   initField() {
     addon().initField(this.handle, null);

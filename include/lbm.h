@@ -41,6 +41,10 @@ extern "C" {
 #define LBM_Y_SLIP 2
 #define LBM_Y_MOVING 3
 
+/* Open x planes of a 3D tunnel (lbm_set_open_faces). */
+#define LBM_FACES_EQUILIBRIUM 0 /* the reference's: inlet stores feq(1, uIn), outlet feq(1, u) */
+#define LBM_FACES_ZOU_HE 1      /* Hecht-Harting Zou-He velocity inlet and rho = 1 pressure outlet */
+
 /* Storage precision of the distribution buffers (src/shaders/common3d.ts:4, plus bf16). */
 #define LBM_FP32 0
 #define LBM_FP16 1 /* f16 of (f - w) * 2^15, clamped to +-1.99 before scaling */
@@ -117,6 +121,14 @@ int lbm_update_3d(lbm_solver* s, const lbm_config3d* cfg);
  * u_target * r(min(1, step / steps)), r = t (kind 1) or t^2 (3 - 2t) (kind 2); kind 0 turns it off and
  * restores the configured u_in. The step index is that of the step being taken (0 for the first). */
 int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps);
+
+/* 3D only. The reference's planes (LBM_FACES_EQUILIBRIUM, the default, src/shaders/step3d.ts:190-209)
+ * or the Zou-He faces its 3D spec lists as the follow-up for validation runs (LBM_FACES_ZOU_HE; spec
+ * docs/dietpowers/2026-09-27-3d-tunnel-spec.md:287, Hecht and Harting, arXiv:0811.4593): the
+ * populations entering from outside are rebuilt so the inlet node carries exactly (uIn, 0, 0) and
+ * the outlet node rho = 1 with no transverse velocity, and both nodes then collide. Takes effect
+ * from the next enqueued step; rebuilds the boundary tables. The adjoint accepts only the default. */
+int lbm_set_open_faces(lbm_solver* s, int kind);
 
 /* setSdf (src/solver.ts:200-203, src/solver3d.ts:211-214): n = N floats, negative inside solids.
  * Rebuilds flags, the boundary-cell list (in cell order) and refills cells the body uncovered. */

@@ -42,6 +42,7 @@ def load():
         "orc_init_field": ([P, C.c_void_p], None),
         "orc_reset_forces": ([P], None),
         "orc_set_ramp": ([P, C.c_int, C.c_double, C.c_double], None),
+        "orc_set_open_faces": ([P, C.c_int], None),
         "orc_steps": ([P, C.c_int64], None),
         "orc_macro": ([P, C.c_void_p], None),
         "orc_read_forces": ([P, C.c_void_p, C.c_void_p, C.c_int64], C.c_int64),
@@ -249,6 +250,10 @@ class OracleSolver3D(_Base):
 
     def init_field(self):
         _lib.orc_init_field(self._h, None)
+
+    def set_open_faces(self, kind):
+        """'equilibrium' (the reference's inlet and outlet planes) or 'zou-he' (Hecht-Harting faces)."""
+        _lib.orc_set_open_faces(self._h, {"equilibrium": 0, "zou-he": 1}[kind])
 
 
 def tables2():

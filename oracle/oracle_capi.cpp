@@ -37,6 +37,7 @@ struct Base {
   virtual void read_flags(uint32_t* out) = 0;
   virtual void read_slots(uint32_t* out) = 0;
   virtual void params(double* out) const = 0;
+  virtual void set_faces(int) {}
 };
 
 template <class Sv>
@@ -121,6 +122,7 @@ struct Impl3 : Impl<orc::Solver3D<T, S>> {
     this->s.write_params();
   }
   void init_field(const float*) override { this->s.init_field(); }
+  void set_faces(int kind) override { this->s.faces = kind; }
   void read_slots(uint32_t* out) override { std::memcpy(out, this->s.slot.data(), sizeof(uint32_t) * size_t(this->s.N)); }
   // Params3 order of src/shaders/common3d.ts:53-54: tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb.
   void params(double* o) const override {
@@ -169,6 +171,7 @@ void orc_reset_forces(void* h) { static_cast<Base*>(h)->reset_forces(); }
 void orc_set_ramp(void* h, int kind, double u_target, double steps) {
   static_cast<Base*>(h)->set_ramp(kind, u_target, steps);
 }
+void orc_set_open_faces(void* h, int kind) { static_cast<Base*>(h)->set_faces(kind); }
 void orc_steps(void* h, int64_t n) { static_cast<Base*>(h)->steps(n); }
 void orc_macro(void* h, double* out) { static_cast<Base*>(h)->macro(out); }
 int64_t orc_read_forces(void* h, int64_t* steps, double* values, int64_t cap) {

@@ -494,6 +494,9 @@ struct Solver3D {
   ForceBook book;
   Ramp ramp;
   int OPP[19], MY[19], MZ[19];
+  // Open x planes: 0 = the reference's equilibrium inlet and outlet (src/shaders/step3d.ts:190-209),
+  // 1 = Zou-He faces after Hecht and Harting (zou_he_face).
+  int faces = 0;
 
   explicit Solver3D(const lbm_config3d& c)
       : W(c.width), H(c.height), D(c.depth), N(int64_t(c.width) * c.height * c.depth),
@@ -610,6 +613,48 @@ struct Solver3D {
       return S::template store<T>(v);
   }
 
+  // Zou-He velocity and pressure faces of Hecht and Harting (J. Stat. Mech. (2010) P01018,
+  // arXiv:0811.4593), on the x = 0 plane (u = (uIn, 0, 0)) and the x = W-1 plane (rho = 1, no
+  // transverse velocity), in the f - w variables (the weights of a plane and twice those entering
+  // it sum to 1). The populations entering from outside the grid (c_x = +1 at the inlet, -1 at the
+  // outlet) are rebuilt from their opposites:
+  //   f_i = f_opp(i) + 6 w_i rho (c_i . u) - c_iy N_y - c_iz N_z,  N_a = 1/2 sum_{c_x = 0} c_a f,
+  // with rho (1 - uIn) = 1 + sum_{c_x = 0} f + 2 sum_{c_x = -1} f at the inlet and
+  // u_x = sum_{c_x = 0} f + 2 sum_{c_x = +1} f at the outlet, so that the node's moments are exactly
+  // the prescribed ones. Slip-face rows use the mirrored populations as known ones, so edges and
+  // corners of the planes take the same rule.
+  void zou_he_face(T f[19], bool inlet) const {
+    const int in = inlet ? 1 : -1;
+    T s0 = T(0), sk = T(0), ny = T(0), nz = T(0);
+    for (int i = 0; i < 19; i++) {
+      if (CX3[i] == 0) {
+        s0 = s0 + f[i];
+        if (CY3[i]) ny = CY3[i] > 0 ? ny + f[i] : ny - f[i];
+        if (CZ3[i]) nz = CZ3[i] > 0 ? nz + f[i] : nz - f[i];
+      } else if (CX3[i] == -in) {
+        sk = sk + f[i];
+      }
+    }
+    ny = T(0.5) * ny;
+    nz = T(0.5) * nz;
+    T rho = T(1), ux;
+    if (inlet) {
+      ux = P.uIn;
+      rho = (T(1) + s0 + T(2) * sk) / (T(1) - ux);
+    } else {
+      ux = s0 + T(2) * sk;
+    }
+    const T ru = rho * ux;
+    for (int i = 1; i < 19; i++) {
+      if (CX3[i] != in) continue;
+      const T k = (CY3[i] || CZ3[i]) ? T(1) / T(6) : T(1) / T(3);  // 6 w_i
+      T v = in > 0 ? f[OPP[i]] + k * ru : f[OPP[i]] - k * ru;
+      if (CY3[i]) v = CY3[i] > 0 ? v - ny : v + ny;
+      if (CZ3[i]) v = CZ3[i] > 0 ? v - nz : v + nz;
+      f[i] = v;
+    }
+  }
+
   // src/shaders/step3d.ts:170-223 for one cell.
   void step_cell(const std::vector<E>& src, std::vector<E>& dst, int64_t idx) {
     const uint32_t fl = flags[size_t(idx)];
@@ -624,18 +669,24 @@ struct Solver3D {
       for (int i = 1; i < 19; i++) f[i] = ld(src, i * N + idx - CZ3[i] * WH - int64_t(CY3[i]) * W - CX3[i]);
     } else {
       if (fl & FLAG_SOLID3) return;
-      if (fl & FLAG_INLET3) {
+      const bool inlet = (fl & FLAG_INLET3) != 0;
+      if (inlet && !faces) {
         for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), P.uIn, T(0), T(0)), idx, i);
         return;
       }
       outlet = (fl & FLAG_OUTLET3) != 0;
       T force[3] = {T(0), T(0), T(0)};
       f[0] = ld(src, idx);
-      for (int i = 1; i < 19; i++) f[i] = pull_slow(src, i, x, y, z, idx, fl, outlet, force);
+      // (a Zou-He inlet rebuilds the populations entering from x = -1 instead of pulling them)
+      for (int i = 1; i < 19; i++) f[i] = inlet && CX3[i] > 0 ? T(0) : pull_slow(src, i, x, y, z, idx, fl, outlet, force);
       uint32_t sl = slot[size_t(idx)];
       if (sl != 0) {
         T* cf = &cellForce[size_t(sl - 1) * 4];
         cf[0] = force[0], cf[1] = force[1], cf[2] = force[2], cf[3] = T(0);
+      }
+      if (faces && (inlet || outlet)) {
+        zou_he_face(f, inlet);
+        outlet = false;  // the face node collides
       }
     }
     // moments (step3d.ts:38-45)

@@ -26,6 +26,22 @@ async function main() {
   assert.ok(!m.fieldDiverged(mac));
   console.log(`ok 3d run: C_D-like fx ${f[49].fx.toExponential(4)}`);
 
+  // --- Zou-He faces: the inlet plane carries u = (uIn, 0, 0) (no absorbing source at x = 0)
+  s3.setOpenFaces('zouHe');
+  await s3.run(50, 25);
+  const mz = await s3.readMacro();
+  let du = 0;
+  let drho = 0;
+  for (let k = 0; k < H * D; k++) {
+    const c = k * W * 4;
+    du = Math.max(du, Math.abs(mz[c] - 0.05), Math.abs(mz[c + 1]), Math.abs(mz[c + 2]));
+    drho = Math.max(drho, Math.abs(mz[c + 3] - 1));
+  }
+  assert.ok(du < 1e-6 && drho > 1e-6, `${du} ${drho}`);
+  assert.throws(() => s3.setOpenFaces('open'), /unknown open faces/);
+  s3.setOpenFaces('equilibrium');
+  console.log(`ok zou-he faces: inlet |u - (uIn, 0, 0)| ${du.toExponential(2)}, |rho - 1| ${drho.toExponential(2)}`);
+
   // --- stale read across resetForces (2D and 3D)
   for (const dim of [2, 3]) {
     const s = dim === 2

@@ -53,6 +53,7 @@ def test_host_only_calls():
     assert lib.lbm_copy_roofline(0, 1024, 19, 4, 96, 1, C.byref(C.c_double())) != 0
     assert lib.lbm_set_launch(None, 64, 0) != 0
     assert lib.lbm_rebuild_flags(None) != 0
+    assert lib.lbm_set_open_faces(None, 1) != 0
     assert lib.lbm_brush(None, 1.0, 1.0, 1.0, 0, 1) != 0
     assert lib.lbm_brush_stroke(None, None, 0, 1.0, 0) != 0
     assert lib.lbm_device_name(0, None, 0) != 0

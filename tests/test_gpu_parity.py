@@ -303,3 +303,52 @@ def test_d3q19_launch_shapes_give_identical_bits(precision):
     tol = TOL32[100] if precision == "fp32" else TOL16[100]
     assert _err(a[2], b[2]) <= tol
     assert np.abs(a[1] - b[1]).max() <= 1e-4 * np.abs(a[1]).max()
+
+
+@pytest.mark.parametrize("precision,variant", [("fp32", 0), ("fp32", 4), ("fp16", 1)])
+def test_d3q19_zou_he_faces_match_oracle(precision, variant):
+    """Hecht-Harting Zou-He inlet and outlet faces (lbm_set_open_faces): GPU against the oracle to the
+    bounds of the default planes, in both store variants, with a body two cells from the inlet plane
+    (inlet nodes with Bouzidi links go through the boundary list) and one touching the z = 0 face;
+    switching back gives the reference's planes bit for bit."""
+    cfg, sdf = case3(precision)
+    W, H, D = cfg["width"], cfg["height"], cfg["depth"]
+    g.add_sphere(sdf, W, H, D, 2.6, 8.0, 6.5, 1.9)
+    gpu = Solver3D.create(dict(cfg))
+    o32 = OracleSolver3D(dict(cfg), 32)
+    o64 = OracleSolver3D(dict(cfg), 64)
+    for s in (gpu, o32, o64):
+        s.set_sdf(sdf)
+        s.init_field()
+        s.set_open_faces("zou-he")
+    gpu.set_launch(64, variant)
+    np.testing.assert_array_equal(gpu.read_flags(), o32.read_flags())
+    tol = {"fp32": TOL32, "fp16": TOL16}[precision]
+    done = 0
+    for n in (1, 10, 100):
+        gpu.run(n - done)
+        o32.steps(n - done)
+        o64.steps(n - done)
+        done = n
+        e = _err(gpu.read_populations(), o32.read_populations())
+        print(f"zou-he {precision} v{variant} step {n}: |gpu-o32| {e:.3e} |gpu-o64| "
+              f"{_err(gpu.read_populations(), o64.read_populations()):.3e}")
+        assert e <= tol[n], (n, e)
+    m = gpu.read_macro().reshape(D, H, W, 4)
+    assert np.abs(m[:, :, 0, 3] - 1).max() > 1e-5  # the inlet density floats: not the equilibrium plane
+    sg, vg = gpu.read_forces_raw()
+    so, vo = o32.read_forces_raw()
+    np.testing.assert_array_equal(sg, so)
+    assert np.abs(vg - vo).max() / np.abs(vo).max() < {"fp32": 1e-4, "fp16": 2e-3}[precision]
+    # back to the reference's planes: the same bits as a solver that never switched
+    ref = Solver3D.create(dict(cfg))
+    for s in (gpu, ref):
+        s.set_sdf(sdf)
+        s.init_field()
+        s.set_launch(64, variant)
+    gpu.set_open_faces("equilibrium")
+    for s in (gpu, ref):
+        s.run(20)
+    np.testing.assert_array_equal(gpu.read_populations_raw(), ref.read_populations_raw())
+    for s in (gpu, ref, o32, o64):
+        s.destroy()

@@ -14,3 +14,37 @@ def test_validation_case(case):
     for m in r["metrics"]:
         print(f"  {m['name']}: {m['value']:.6g} accept {m['accept']} {'ok' if m['pass_'] else 'MISS'}")
     assert r["pass"], [m for m in r["metrics"] if not m["pass_"]]
+
+
+def test_sphere_drag_with_zou_he_faces():
+    """The sphere of src/cases3d.ts:11-81 (Re 100, 192 x 96 x 96, FP32) with the Hecht-Harting Zou-He
+    inlet and outlet faces instead of the equilibrium planes: the same inflow and outflow conditions,
+    so the converged C_D agrees with the equilibrium planes' to 1%."""
+    import math
+
+    from cells_amd import geometry as g
+    from cells_amd.analysis import derive_tau, mean
+    from cells_amd.lbm import Solver3D
+
+    W, H, D, d, u = 192, 96, 96, 16, 0.1
+    sdf = g.empty_sdf3(W, H, D)
+    g.add_sphere(sdf, W, H, D, W / 4, H / 2, D / 2, d / 2)
+    scale = 2 / (u * u * g.reference_area("sphere", d, D))
+    cd = {}
+    for faces in ("equilibrium", "zou-he"):
+        s = Solver3D.create(dict(width=W, height=H, depth=D, precision="fp32", tau=derive_tau(100, u, d).tau,
+                                 spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=4))
+        s.set_sdf(sdf)
+        s.init_field()
+        s.set_open_faces(faces)
+        s.set_inlet_ramp("smoothstep", u, 3000)
+        prev = math.nan
+        for k in range(1, 16):
+            s.run(2000)
+            cd[faces] = mean(f.fx for f in s.read_forces()) * scale
+            if k >= 5 and abs(cd[faces] - prev) < 1e-4 * abs(prev):
+                break
+            prev = cd[faces]
+        s.destroy()
+    print(cd)
+    assert abs(cd["zou-he"] / cd["equilibrium"] - 1) < 1e-2
