@@ -62,10 +62,10 @@ def tunnel_sdf(W, H, D):
     return sdf
 
 
-def run_gpu(W, H, D, precision, steps, warmup, profile=True):
+def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
     from cells_amd.lbm import Solver3D, bytes_per_cell
 
-    s = Solver3D.create(tunnel_config(W, H, D, precision))
+    s = Solver3D.create(dict(tunnel_config(W, H, D, precision), velocitySet=velocity_set))
     s.set_sdf(tunnel_sdf(W, H, D))
     s.init_field()
     s.set_inlet_ramp("smoothstep", 0.1, 3000)
@@ -78,7 +78,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True):
     N = s.N
     mlups = N * steps / (ms * 1e3)
     out = dict(precision=precision, grid=[W, H, D], cells=N, steps=steps, ms=ms, ms_per_step=ms / steps,
-               mlups=mlups, bytes_per_cell=bytes_per_cell(19, s.precision))
+               mlups=mlups, bytes_per_cell=bytes_per_cell(s.Q, s.precision), velocity_set=s.Q)
     out["algorithmic_gbs"] = mlups * 1e6 * out["bytes_per_cell"] / 1e9
     out["roofline_frac"] = out["algorithmic_gbs"] / HBM_PEAK_GBS
     nb, nl, ns = s.node_counts()
@@ -426,6 +426,8 @@ def main():
         for prec in ("fp32", "fp16"):
             variants[f"{prec}_256x128x128"] = run_gpu(256, 128, 128, prec, max(1, args.steps // 2),
                                                       args.warmup // 2)
+        for prec in ("fp32", "fp16"):  # the D3Q27 velocity set on the same tunnel
+            variants[f"d3q27_{prec}_192x96x96"] = run_gpu(W, H, D, prec, args.steps, args.warmup, velocity_set=27)
     from cells_amd.lbm import copy_roofline
 
     copy = {"fp32_192x96x96_gbs": copy_roofline(W * H * D, 19, 4), "fp16_192x96x96_gbs": copy_roofline(W * H * D, 19, 2),
@@ -464,7 +466,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "step_kernel_us", "evented_step_us", "slowest_step_us",
-                                               "kernel_achieved_gbs",
+                                               "kernel_achieved_gbs", "bytes_per_cell",
                                                "bulk_block", "row_order", "nodes", "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],

@@ -673,6 +673,8 @@ int check_supported(const lbm_solver* s) {
       return fail(LBM_ERR_UNSUPPORTED, "adjoint: body force is not supported");
   } else if (s->cfg3.smagorinsky > 0) {
     return fail(LBM_ERR_UNSUPPORTED, "adjoint: Smagorinsky is not supported");
+  } else if (s->Q != 19) {
+    return fail(LBM_ERR_UNSUPPORTED, "adjoint: D3Q19 only");
   } else if (s->faces != LBM_FACES_EQUILIBRIUM) {
     return fail(LBM_ERR_UNSUPPORTED, "adjoint: only the equilibrium inlet and outlet planes are supported");
   }

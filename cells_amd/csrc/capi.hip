@@ -443,6 +443,9 @@ int rebuild_flags_impl(lbm_solver* s) {
     launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
     launch_flags2(s->sdf, s->flags, s->touches, static_cast<float*>(s->f[0]), static_cast<float*>(s->f[1]), s->mac,
                   s->p2, s->stream);
+  } else if (s->Q == 27) {
+    launch_macro27(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
+    launch_flags27(s->prec, s->sdf, s->flags, s->f[0], s->f[1], s->mac, s->p3, s->stream);
   } else {
     launch_macro3(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
     launch_flags3(s->prec, s->sdf, s->flags, s->f[0], s->f[1], s->mac, s->p3, s->stream);
@@ -457,6 +460,30 @@ int rebuild_flags_impl(lbm_solver* s) {
   }
   HIP_TRY(hipStreamSynchronize(s->stream));
 
+  if (s->Q == 27) {  // the D3Q27 step resolves its boundary rules in place: only the node counts
+    s->n_bulk = s->n_solid = s->nlist = 0;
+    const uint32_t W = uint32_t(s->W), bx = step27_block_x(s->W);
+    const uint32_t gx = (W + bx - 1) / bx;
+    std::vector<uint32_t> waves;  // ascending wave indices holding a FLAG_SLOW27 node
+    for (uint32_t k = 0; k < N; k++) {
+      if (fl[k] & FLAG_SOLID27) {
+        s->n_solid++;
+      } else if (fl[k] & FLAG_SLOW27) {
+        s->nlist++;
+        const uint32_t x = k % W, row = k / W;  // row = z H + y
+        const uint32_t wave = (row * gx + x / bx) * (bx / 64u) + (x % bx) / 64u;
+        if (waves.empty() || waves.back() != wave) waves.push_back(wave);
+      } else {
+        s->n_bulk++;
+      }
+    }
+    if (!waves.empty())
+      HIP_TRY(hipMemcpyAsync(s->list, waves.data(), 4 * waves.size(), hipMemcpyHostToDevice, s->stream));
+    s->n_link = uint32_t(waves.size());
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->tables_gen++;
+    return LBM_OK;
+  }
   if (s->nd == 2) {
     // Force slots in cell order (the reference's atomicAdd order is arbitrary, src/shaders/aux.ts:63-66).
     uint32_t slot = 0;
@@ -490,7 +517,8 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMalloc(&s->sdf, 4 * N));
   HIP_TRY(hipMalloc(&s->mac, 16 * N));
   HIP_TRY(hipMalloc(&s->list, 4 * N));
-  const size_t maxwaves = (N + 63) / 64 + 16;  // one force partial per wave of body-link nodes
+  size_t maxwaves = (N + 63) / 64 + 16;  // one force partial per wave of body-link nodes
+  if (s->Q == 27) maxwaves = std::max(maxwaves, step27_partials(s->W, s->H, s->D));  // per wave of the grid
   HIP_TRY(hipMalloc(&s->partials, sizeof(double) * 3 * maxwaves));
   HIP_TRY(hipMalloc(&s->counter, 4));
   HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
@@ -539,6 +567,9 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
     const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
     launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, lt, s->p2, s->xtab, fs,
                  s->bulk_block, s->stream, rec);
+  } else if (s->Q == 27) {
+    if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
+    launch_step27(s->prec, src, dst, s->flags, s->sdf, s->xtab, s->p3, fs, s->list, s->n_link, s->stream);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];
@@ -602,6 +633,7 @@ void lbm_config3d_defaults(lbm_config3d* c) {
   c->sponge_tau = 1.0;
   c->workgroup_size = 128;
   c->force_every = 1;
+  c->velocity_set = 19;
 }
 
 int lbm_create_2d(const lbm_config2d* cfg, int device, lbm_solver** out) {
@@ -658,13 +690,18 @@ int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
     return fail(LBM_ERR_INVALID, "width must be at least 3, height and depth positive");
   if (cfg->height > 65535 || cfg->depth > 65535)
     return fail(LBM_ERR_INVALID, "height and depth must be at most 65535 (one grid row per launch row)");
+  const int Q = cfg->velocity_set == 27 ? 27 : 19;
+  if (cfg->velocity_set != 0 && cfg->velocity_set != 19 && cfg->velocity_set != 27)
+    return fail(LBM_ERR_UNSUPPORTED, "velocity_set must be 19 or 27");
+  if (Q == 27 && cfg->precision == LBM_FP16_SR)
+    return fail(LBM_ERR_UNSUPPORTED, "D3Q27: precision fp32, fp16 or bf16");
   const uint64_t N = uint64_t(cfg->width) * uint64_t(cfg->height) * uint64_t(cfg->depth);
   const size_t elem = cfg->precision == LBM_FP32 ? 4 : 2;
-  if (19 * N >= (1ull << 32)) {
+  if (Q * N >= (1ull << 32)) {
     return fail(LBM_ERR_ALLOC, "Solver3D allocation failed: one distribution buffer needs " +
-                                   std::to_string(19 * elem * N) + " bytes; this device allows " +
-                                   std::to_string((1ull << 32) / 19 * elem) + " per storage binding and " +
-                                   std::to_string((1ull << 32) / 19 * elem) + " per buffer");
+                                   std::to_string(Q * elem * N) + " bytes; this device allows " +
+                                   std::to_string((1ull << 32) / Q * elem) + " per storage binding and " +
+                                   std::to_string((1ull << 32) / Q * elem) + " per buffer");
   }
   const int bb = bulk_block_for(cfg->workgroup_size);
   if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
@@ -675,7 +712,7 @@ int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   s->H = cfg->height;
   s->D = cfg->depth;
   s->N = uint32_t(N);
-  s->Q = 19;
+  s->Q = Q;
   s->prec = cfg->precision;
   s->elem = elem;
   s->cfg3 = *cfg;
@@ -724,8 +761,9 @@ int lbm_update_2d(lbm_solver* s, const lbm_config2d* cfg) {
 
 int lbm_update_3d(lbm_solver* s, const lbm_config3d* cfg) {
   if (!s || !cfg || s->nd != 3) return fail(LBM_ERR_INVALID, "lbm_update_3d needs a 3D solver and a config");
-  if (cfg->width != s->W || cfg->height != s->H || cfg->depth != s->D || cfg->precision != s->prec)
-    return fail(LBM_ERR_INVALID, "grid size and precision cannot change");
+  if (cfg->width != s->W || cfg->height != s->H || cfg->depth != s->D || cfg->precision != s->prec ||
+      (cfg->velocity_set == 27 ? 27 : 19) != s->Q)
+    return fail(LBM_ERR_INVALID, "grid size, precision and velocity set cannot change");
   const int bb = bulk_block_for(cfg->workgroup_size);
   if (bb < 0) return fail(LBM_ERR_INVALID, "workgroup_size must be a multiple of 64 up to 1024");
   s->cfg3 = *cfg;
@@ -761,6 +799,7 @@ int lbm_rebuild_flags(lbm_solver* s) {
 int lbm_set_open_faces(lbm_solver* s, int kind) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (s->nd != 3) return fail(LBM_ERR_INVALID, "open faces: 3D solvers only (the 2D inlet and outlet are Zou-He already)");
+  if (s->Q == 27 && kind != LBM_FACES_EQUILIBRIUM) return fail(LBM_ERR_UNSUPPORTED, "open faces: Zou-He faces are D3Q19 only");
   if (kind != LBM_FACES_EQUILIBRIUM && kind != LBM_FACES_ZOU_HE) return fail(LBM_ERR_INVALID, "open faces: unknown kind");
   if (kind == s->faces) return LBM_OK;
   s->faces = kind;
@@ -846,7 +885,8 @@ int lbm_init_field(lbm_solver* s, const float* macro, size_t n) {
     HIP_TRY(hipStreamSynchronize(s->stream));
   } else {
     if (macro) return fail(LBM_ERR_INVALID, "the 3D initField takes no macro field");
-    launch_init3(s->prec, s->f[0], s->f[1], s->flags, s->p3, s->stream);
+    if (s->Q == 27) launch_init27(s->prec, s->f[0], s->f[1], s->flags, s->p3, s->stream);
+    else launch_init3(s->prec, s->f[0], s->f[1], s->flags, s->p3, s->stream);
     HIP_TRY(hipGetLastError());
   }
   s->parity = 0;
@@ -875,6 +915,7 @@ int lbm_enqueue_steps(lbm_solver* s, int64_t n) {
 int lbm_enqueue_macro(lbm_solver* s) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (s->nd == 2) launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
+  else if (s->Q == 27) launch_macro27(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
   else launch_macro3(s->prec, s->f[s->parity], s->flags, s->mac, s->p3, s->stream);
   HIP_TRY(hipGetLastError());
   return LBM_OK;
@@ -1019,7 +1060,7 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n) {
   HIP_TRY(hipMemcpyAsync(out, s->flags, 4 * size_t(s->N), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (s->nd == 3)
-    for (uint32_t k = 0; k < s->N; k++) out[k] &= ~FLAG_LIST3;
+    for (uint32_t k = 0; k < s->N; k++) out[k] &= s->Q == 27 ? ~FLAG_SLOW27 : ~FLAG_LIST3;
   return LBM_OK;
 }
 
@@ -1077,6 +1118,10 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 // store variants are separate compilations and may differ in the last bits.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (s->Q == 27) {  // one launch shape (whole rows of at most 256 threads)
+    if (chosen) *chosen = int32_t(step27_block_x(s->W));
+    return LBM_OK;
+  }
   static std::mutex cache_mu;
   static std::map<std::string, std::pair<int, int>> cache;
   const std::string key = std::to_string(s->nd) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
@@ -1177,7 +1222,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
 
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
-  if (block) *block = int32_t(bulk_block_x(s->W, s->bulk_block));
+  if (block) *block = int32_t(s->Q == 27 ? step27_block_x(s->W) : bulk_block_x(s->W, s->bulk_block));
   if (sweep) *sweep = s->nd == 3 ? s->sweep : 0;
   return LBM_OK;
 }
@@ -1188,6 +1233,7 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
   if (bb < 0 || block <= 0) return fail(LBM_ERR_INVALID, "block must be a multiple of 64 up to 1024");
   if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE) || (sweep & 3) == 3) : sweep != 0)
     return fail(LBM_ERR_INVALID, "variant must be 0-2 or 4-6 (3D), 0 (2D)");
+  if (s->Q == 27) return fail(LBM_ERR_UNSUPPORTED, "D3Q27 has one launch shape");
   s->bulk_block = bb;
   if (s->nd == 3) s->sweep = sweep;
   return LBM_OK;

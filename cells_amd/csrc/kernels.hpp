@@ -62,6 +62,21 @@ void launch_flags3(int prec, const float* sdf, uint32_t* flags, void* fa, void* 
 uint32_t list_blocks(uint32_t n, uint32_t bx);
 uint32_t body_list_blocks(uint32_t n_body, uint32_t bx, bool sample);
 
+// ---- D3Q27 (prec LBM_FP32 / LBM_FP16 / LBM_BF16), same parameter block and relaxation table ----
+// One launch per step (plus a one-wave reduction when a force sample is due). Partials: 3 doubles per
+// wave index, up to step27_partials(W, H, D) waves; force_waves lists, ascending, the n_force_waves
+// wave indices holding a FLAG_SLOW27 node (wave = (block (z H + y) gx + x / bx) (bx / 64) + x % bx / 64,
+// bx = step27_block_x(W)).
+void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab,
+                   const Params3& p, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
+                   hipStream_t st);
+uint32_t step27_block_x(int W);
+size_t step27_partials(int W, int H, int D);
+void launch_macro27(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);
+void launch_init27(int prec, void* fa, void* fb, const uint32_t* flags, const Params3& p, hipStream_t st);
+void launch_flags27(int prec, const float* sdf, uint32_t* flags, void* fa, void* fb, const float4* mac,
+                    const Params3& p, hipStream_t st);
+
 // ---- D2Q9 (FP32 storage) ----
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
                   const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block, hipStream_t st,

@@ -49,6 +49,32 @@ __host__ __device__ constexpr int find3(int cx, int cy, int cz) {
 __host__ __device__ constexpr int mirror_y3(int i) { return find3(CX3[i], -CY3[i], CZ3[i]); }
 __host__ __device__ constexpr int mirror_z3(int i) { return find3(CX3[i], CY3[i], -CZ3[i]); }
 
+// ---------------------------------------------------------------- D3Q27
+// The velocity-set switch of the reference's 3D roadmap (docs/research/3d-lattice-boltzmann-webgpu.md:74):
+// the D3Q19 directions in the same order, then the eight corners, so odd i and i + 1 stay opposites.
+// Weights 8/27, 2/27, 1/54, 1/216. Flag words: direction bits 1..26, then SOLID, INLET, OUTLET.
+constexpr int CX27[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+constexpr int CY27[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+constexpr int CZ27[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+constexpr float W27[27] = {8.0f / 27,  2.0f / 27,  2.0f / 27,  2.0f / 27,  2.0f / 27,  2.0f / 27,  2.0f / 27,
+                           1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,
+                           1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 54,  1.0f / 216, 1.0f / 216,
+                           1.0f / 216, 1.0f / 216, 1.0f / 216, 1.0f / 216, 1.0f / 216, 1.0f / 216};
+constexpr uint32_t FLAG_SOLID27 = 1u << 27;
+constexpr uint32_t FLAG_INLET27 = 1u << 28;
+constexpr uint32_t FLAG_OUTLET27 = 1u << 29;
+// Set by the flag kernel on nodes whose pulls meet a solid node (a Bouzidi link, or a slip face or
+// edge reflecting into a solid): the step resolves those per direction. Never reported.
+constexpr uint32_t FLAG_SLOW27 = 1u << 30;
+__host__ __device__ constexpr int find27(int cx, int cy, int cz) {
+  int r = -1;
+  for (int i = 0; i < 27; i++)
+    if (CX27[i] == cx && CY27[i] == cy && CZ27[i] == cz) r = i;
+  return r;
+}
+__host__ __device__ constexpr int mirror_y27(int i) { return find27(CX27[i], -CY27[i], CZ27[i]); }
+__host__ __device__ constexpr int mirror_z27(int i) { return find27(CX27[i], CY27[i], -CZ27[i]); }
+
 // ---------------------------------------------------------------- WGSL built-ins
 __device__ __forceinline__ float wclamp(float e, float lo, float hi) { return fminf(fmaxf(e, lo), hi); }
 __device__ __forceinline__ float smoothstep(float lo, float hi, float x) {

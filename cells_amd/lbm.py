@@ -46,6 +46,7 @@ class Config3D(C.Structure):
         ("workgroup_size", C.c_int32), ("force_every", C.c_int32),
         ("tau", C.c_double), ("lambda_", C.c_double), ("smagorinsky", C.c_double), ("u_in", C.c_double),
         ("sponge_fraction", C.c_double), ("sponge_tau", C.c_double), ("absorb", C.c_double),
+        ("velocity_set", C.c_int32), ("_pad1", C.c_int32),
     ]
 
 
@@ -70,13 +71,14 @@ KEYS3 = {
     "width": "width", "height": "height", "depth": "depth", "precision": "precision", "tau": "tau",
     "lambda": "lambda_", "smagorinsky": "smagorinsky", "uIn": "u_in", "spongeFraction": "sponge_fraction",
     "spongeTau": "sponge_tau", "absorb": "absorb", "workgroupSize": "workgroup_size", "forceEvery": "force_every",
+    "velocitySet": "velocity_set",
 }
 
 DEFAULTS2 = dict(lambda_=MAGIC_LAMBDA, smagorinsky=0.0, u_in=0.0, parabolic=0, u_lid=0.0, sponge_fraction=0.0,
                  sponge_tau=1.0, body_force=(0.0, 0.0), absorb=0.0, inlet_layer_fraction=0.0, workgroup_size=128,
                  force_every=1)
 DEFAULTS3 = dict(lambda_=float("nan"), smagorinsky=0.0, u_in=0.0, sponge_fraction=0.0, sponge_tau=1.0, absorb=0.0,
-                 workgroup_size=128, force_every=1)
+                 workgroup_size=128, force_every=1, velocity_set=19)
 
 
 def make_config2d(cfg: dict) -> Config2D:
@@ -222,6 +224,7 @@ class _Base:
         _check(_lib.lbm_info(self._h, _ptr(dims), C.byref(n), C.byref(step), C.byref(q), C.byref(prec)))
         self.W, self.H, self.D = (int(v) for v in dims)
         self.N = int(n.value)
+        self.Q = int(q.value)
         self.precision = int(prec.value)
 
     # -- readonly step counter (src/solver.ts:57)

@@ -38,6 +38,8 @@ export interface Solver3DConfig {
   absorb?: number;
   workgroupSize?: number;
   forceEvery?: number;
+  /** 19 (default, the reference's D3Q19) or 27 (D3Q27). */
+  velocitySet?: 19 | 27;
 }
 
 export interface ForceSample { step: number; fx: number; fy: number }

@@ -140,6 +140,7 @@ static void read_cfg3(napi_env env, napi_value o, lbm_config3d* c) {
   c->sponge_fraction = num_prop(env, o, "spongeFraction", 0);
   c->sponge_tau = num_prop(env, o, "spongeTau", 1);
   c->absorb = num_prop(env, o, "absorb", 0);
+  c->velocity_set = (int32_t)num_prop(env, o, "velocitySet", 19);
 }
 
 static napi_value js_create2d(napi_env env, napi_callback_info info) {

@@ -59,6 +59,7 @@ function flat3(c) {
     width: c.width, height: c.height, depth: c.depth, precisionCode: PRECISION_CODES[c.precision],
     workgroupSize: c.workgroupSize, forceEvery: c.forceEvery, tau: c.tau, smagorinsky: c.smagorinsky,
     uIn: c.uIn, spongeFraction: c.spongeFraction, spongeTau: c.spongeTau, absorb: c.absorb,
+    velocitySet: c.velocitySet === undefined ? 19 : c.velocitySet,
   };
   if (c.lambda !== undefined && c.lambda !== null) o.lambda = c.lambda;
   return o;

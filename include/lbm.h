@@ -90,6 +90,8 @@ typedef struct lbm_config3d {
   double sponge_fraction;
   double sponge_tau;
   double absorb;
+  int32_t velocity_set; /* 19 (D3Q19, the reference's; 0 means 19) or 27 (D3Q27, its roadmap switch) */
+  int32_t _pad1;
 } lbm_config3d;
 
 typedef struct lbm_solver lbm_solver;

@@ -48,6 +48,38 @@ inline int find3(int cx, int cy, int cz) {
 inline int mirror_y3(int i) { return find3(CX3[i], -CY3[i], CZ3[i]); }
 inline int mirror_z3(int i) { return find3(CX3[i], CY3[i], -CZ3[i]); }
 
+// ---- D3Q27 (the reference's roadmap switch: docs/research/3d-lattice-boltzmann-webgpu.md:74) ----
+// The D3Q19 set in the same order, then the eight corners, so that odd i and i + 1 stay opposites
+// and the TRT pairs are (1, 2), (3, 4), ..., (25, 26). Weights 8/27, 2/27, 1/54, 1/216 (the
+// D3Q27 standard set). Flag words use bits 1..26 for directions, then SOLID, INLET, OUTLET.
+constexpr int Q27 = 27;
+constexpr int CX27[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+constexpr int CY27[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+constexpr int CZ27[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+constexpr double W27[27] = {8.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,
+                            1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,
+                            1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 216, 1.0 / 216,
+                            1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216};
+
+// The tables of a 3D velocity set by its size (19 or 27); flag bits SOLID, INLET, OUTLET follow the
+// Q - 1 direction bits.
+template <int Q>
+struct Lattice3 {
+  static constexpr const int* CX = Q == 19 ? CX3 : CX27;
+  static constexpr const int* CY = Q == 19 ? CY3 : CY27;
+  static constexpr const int* CZ = Q == 19 ? CZ3 : CZ27;
+  static constexpr const double* W = Q == 19 ? W3 : W27;
+  static constexpr uint32_t SOLID = 1u << Q, INLET = 1u << (Q + 1), OUTLET = 1u << (Q + 2);
+  static int find(int cx, int cy, int cz) {
+    for (int i = 0; i < Q; i++)
+      if (CX[i] == cx && CY[i] == cy && CZ[i] == cz) return i;
+    return -1;
+  }
+  static int opp(int i) { return i == 0 ? 0 : (i % 2 ? i + 1 : i - 1); }
+  static int mirror_y(int i) { return find(CX[i], -CY[i], CZ[i]); }
+  static int mirror_z(int i) { return find(CX[i], CY[i], -CZ[i]); }
+};
+
 // src/lattice3d.ts:29-31.
 inline double trt_lambda3(double tau) { return std::min(MAGIC_LAMBDA, 50.0 * (tau - 0.5) * (tau - 0.5)); }
 

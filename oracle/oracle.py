@@ -55,6 +55,7 @@ def load():
         "orc_params": ([P, C.c_void_p], None),
         "orc_tables2": ([C.c_void_p] * 5, None),
         "orc_tables3": ([C.c_void_p] * 7, None),
+        "orc_tables27": ([C.c_void_p] * 7, None),
         "orc_flag_bits": ([C.c_int, C.c_int], C.c_uint32),
         "orc_trt_lambda3": ([C.c_double], C.c_double),
         "orc_feq2": ([C.c_int, C.c_double, C.c_double, C.c_double], C.c_double),
@@ -96,6 +97,7 @@ class _Base:
         _lib.orc_info(self._h, _ptr(dims), C.byref(n), C.byref(st), C.byref(q))
         self.W, self.H, self.D = (int(v) for v in dims)
         self.N = int(n.value)
+        self.Q = int(q.value)
 
     @property
     def step(self):
@@ -269,6 +271,14 @@ def tables3():
     cx, cy, cz, opp, my, mz = (np.zeros(19, np.int32) for _ in range(6))
     w = np.zeros(19)
     lib.orc_tables3(_ptr(cx), _ptr(cy), _ptr(cz), _ptr(w), _ptr(opp), _ptr(my), _ptr(mz))
+    return dict(cx=cx, cy=cy, cz=cz, w=w, opp=opp, mirror_y=my, mirror_z=mz)
+
+
+def tables27():
+    lib = load()
+    cx, cy, cz, opp, my, mz = (np.zeros(27, np.int32) for _ in range(6))
+    w = np.zeros(27)
+    lib.orc_tables27(_ptr(cx), _ptr(cy), _ptr(cz), _ptr(w), _ptr(opp), _ptr(my), _ptr(mz))
     return dict(cx=cx, cy=cy, cz=cz, w=w, opp=opp, mirror_y=my, mirror_z=mz)
 
 

@@ -108,10 +108,10 @@ struct Impl2 : Impl<orc::Solver2D<T, S>> {
   }
 };
 
-template <class T, class S>
-struct Impl3 : Impl<orc::Solver3D<T, S>> {
-  using Impl<orc::Solver3D<T, S>>::Impl;
-  int q() const override { return 19; }
+template <class T, class S, int Q>
+struct Impl3 : Impl<orc::Solver3D<T, S, Q>> {
+  using Impl<orc::Solver3D<T, S, Q>>::Impl;
+  int q() const override { return Q; }
   int dim() const override { return 3; }
   void dims(int32_t* d, int64_t* n) const override {
     d[0] = this->s.W, d[1] = this->s.H, d[2] = this->s.D;
@@ -122,7 +122,9 @@ struct Impl3 : Impl<orc::Solver3D<T, S>> {
     this->s.write_params();
   }
   void init_field(const float*) override { this->s.init_field(); }
-  void set_faces(int kind) override { this->s.faces = kind; }
+  void set_faces(int kind) override {
+    if (Q == 19) this->s.faces = kind;  // the Zou-He faces are written for D3Q19
+  }
   void read_slots(uint32_t* out) override { std::memcpy(out, this->s.slot.data(), sizeof(uint32_t) * size_t(this->s.N)); }
   // Params3 order of src/shaders/common3d.ts:53-54: tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb.
   void params(double* o) const override {
@@ -132,6 +134,21 @@ struct Impl3 : Impl<orc::Solver3D<T, S>> {
     for (int k = 0; k < 7; k++) o[k] = v[k];
   }
 };
+
+}  // namespace
+
+namespace {
+
+template <int Q>
+static void* create_3d(const lbm_config3d* c, int real_bits) {
+  if (real_bits == 64) return static_cast<Base*>(new Impl3<double, orc::StoreF64, Q>(*c));
+  switch (c->precision) {
+    case LBM_FP16: return static_cast<Base*>(new Impl3<float, orc::StoreF16, Q>(*c));
+    case LBM_FP16_SR: return static_cast<Base*>(new Impl3<float, orc::StoreF16SR, Q>(*c));
+    case LBM_BF16: return static_cast<Base*>(new Impl3<float, orc::StoreBF16, Q>(*c));
+    default: return static_cast<Base*>(new Impl3<float, orc::StoreF32, Q>(*c));
+  }
+}
 
 }  // namespace
 
@@ -145,13 +162,9 @@ void* orc_create_2d(const lbm_config2d* c, int real_bits) {
 
 void* orc_create_3d(const lbm_config3d* c, int real_bits) {
   if (!c || c->width < 3 || c->height < 1 || c->depth < 1) return nullptr;
-  if (real_bits == 64) return static_cast<Base*>(new Impl3<double, orc::StoreF64>(*c));
-  switch (c->precision) {
-    case LBM_FP16: return static_cast<Base*>(new Impl3<float, orc::StoreF16>(*c));
-    case LBM_FP16_SR: return static_cast<Base*>(new Impl3<float, orc::StoreF16SR>(*c));
-    case LBM_BF16: return static_cast<Base*>(new Impl3<float, orc::StoreBF16>(*c));
-    default: return static_cast<Base*>(new Impl3<float, orc::StoreF32>(*c));
-  }
+  if (c->velocity_set == 27) return create_3d<27>(c, real_bits);
+  if (c->velocity_set != 0 && c->velocity_set != 19) return nullptr;
+  return create_3d<19>(c, real_bits);
 }
 
 void orc_destroy(void* h) { delete static_cast<Base*>(h); }
@@ -203,8 +216,16 @@ void orc_tables3(int* cx, int* cy, int* cz, double* w, int* opp, int* my, int* m
     opp[i] = orc::opp3(i), my[i] = orc::mirror_y3(i), mz[i] = orc::mirror_z3(i);
   }
 }
+void orc_tables27(int* cx, int* cy, int* cz, double* w, int* opp, int* my, int* mz) {
+  using L = orc::Lattice3<27>;
+  for (int i = 0; i < 27; i++) {
+    cx[i] = L::CX[i], cy[i] = L::CY[i], cz[i] = L::CZ[i], w[i] = L::W[i];
+    opp[i] = L::opp(i), my[i] = L::mirror_y(i), mz[i] = L::mirror_z(i);
+  }
+}
 uint32_t orc_flag_bits(int lattice, int which) {
   if (lattice == 2) return which == 0 ? orc::FLAG_SOLID2 : which == 1 ? orc::FLAG_INLET2 : orc::FLAG_OUTLET2;
+  if (lattice == 27) return which == 0 ? orc::Lattice3<27>::SOLID : which == 1 ? orc::Lattice3<27>::INLET : orc::Lattice3<27>::OUTLET;
   return which == 0 ? orc::FLAG_SOLID3 : which == 1 ? orc::FLAG_INLET3 : orc::FLAG_OUTLET3;
 }
 double orc_trt_lambda3(double tau) { return orc::trt_lambda3(tau); }

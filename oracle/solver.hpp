@@ -471,8 +471,9 @@ struct Solver2D {
 // ============================================================================================
 // D3Q19
 // ============================================================================================
-template <class T, class S>
+template <class T, class S, int Q = 19>
 struct Solver3D {
+  using L = Lattice3<Q>;
   using E = typename S::elem;
   int W, H, D;
   int64_t N, WH;
@@ -493,7 +494,7 @@ struct Solver3D {
   int parity = 0;
   ForceBook book;
   Ramp ramp;
-  int OPP[19], MY[19], MZ[19];
+  int OPP[Q], MY[Q], MZ[Q];
   // Open x planes: 0 = the reference's equilibrium inlet and outlet (src/shaders/step3d.ts:190-209),
   // 1 = Zou-He faces after Hecht and Harting (zou_he_face).
   int faces = 0;
@@ -501,8 +502,8 @@ struct Solver3D {
   explicit Solver3D(const lbm_config3d& c)
       : W(c.width), H(c.height), D(c.depth), N(int64_t(c.width) * c.height * c.depth),
         WH(int64_t(c.width) * c.height), cfg(c) {
-    for (int i = 0; i < 19; i++) OPP[i] = opp3(i), MY[i] = mirror_y3(i), MZ[i] = mirror_z3(i);
-    f[0].assign(size_t(19 * N), S::template store<T>(T(0)));
+    for (int i = 0; i < Q; i++) OPP[i] = L::opp(i), MY[i] = L::mirror_y(i), MZ[i] = L::mirror_z(i);
+    f[0].assign(size_t(Q * N), S::template store<T>(T(0)));
     f[1] = f[0];
     flags.assign(size_t(N), 0u);
     slot.assign(size_t(N), 0u);
@@ -527,25 +528,40 @@ struct Solver3D {
 
   T ld(const std::vector<E>& b, int64_t k) const { return S::template load<T>(b[size_t(k)]); }
   int64_t at(int x, int y, int z) const { return int64_t(x) + int64_t(W) * (int64_t(y) + int64_t(H) * z); }
-  bool is_solid(int64_t s) const { return (flags[size_t(s)] & FLAG_SOLID3) != 0; }
+  bool is_solid(int64_t s) const { return (flags[size_t(s)] & L::SOLID) != 0; }
 
   // src/shaders/common3d.ts:78-81
   static T feq(int i, T rho, T ux, T uy, T uz) {
-    T cu = T(3) * (T(CX3[i]) * ux + T(CY3[i]) * uy + T(CZ3[i]) * uz);
-    return T(W3[i]) * (rho - T(1) + rho * (cu + T(0.5) * cu * cu - T(1.5) * (ux * ux + uy * uy + uz * uz)));
+    T cu = T(3) * (T(L::CX[i]) * ux + T(L::CY[i]) * uy + T(L::CZ[i]) * uz);
+    return T(L::W[i]) * (rho - T(1) + rho * (cu + T(0.5) * cu * cu - T(1.5) * (ux * ux + uy * uy + uz * uz)));
   }
 
   // src/shaders/step3d.ts:119-168
   T pull_slow(const std::vector<E>& src, int i, int x, int y, int z, int64_t idx, uint32_t fl, bool outlet,
               T force[3]) const {
     const int j = OPP[i];
-    const int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
+    const int sx = x - L::CX[i], sy = y - L::CY[i], sz = z - L::CZ[i];
     if ((fl & (1u << i)) == 0) return ld(src, i * N + at(sx, sy, sz));
     const bool outX = sx < 0 || sx >= W;
     const bool outY = sy < 0 || sy >= H;
     const bool outZ = sz < 0 || sz >= D;
     if (outX && outlet) return ld(src, i * N + at(W - 2, y, z));
-    if (outY && outZ) return ld(src, j * N + idx);
+    if (outY && outZ) {
+      // Reflected through both slip faces: c_y and c_z negated, read at the node (x - c_x, y, z). In
+      // D3Q19 every such direction has c_x = 0, so this is the node itself and the opposite
+      // direction, the reference's bounce-back (step3d.ts:132); D3Q27's corner directions need the
+      // double reflection for a slip edge to keep a uniform flow uniform. A solid reflected node
+      // bounces back with force, as on one face.
+      const int64_t rs = at(sx, y, z);
+      if (is_solid(rs)) {
+        T fj = ld(src, j * N + idx);
+        force[0] += T(L::CX[j]) * (T(2) * fj);
+        force[1] += T(L::CY[j]) * (T(2) * fj);
+        force[2] += T(L::CZ[j]) * (T(2) * fj);
+        return fj;
+      }
+      return ld(src, MY[MZ[i]] * N + rs);
+    }
     if (outY || outZ) {
       int rx = sx, ry = sy, rz = sz;
       int m = MY[i];
@@ -558,9 +574,9 @@ struct Solver3D {
       int64_t rs = at(rx, ry, rz);
       if (is_solid(rs)) {
         T fj = ld(src, j * N + idx);
-        force[0] += T(CX3[j]) * (T(2) * fj);
-        force[1] += T(CY3[j]) * (T(2) * fj);
-        force[2] += T(CZ3[j]) * (T(2) * fj);
+        force[0] += T(L::CX[j]) * (T(2) * fj);
+        force[1] += T(L::CY[j]) * (T(2) * fj);
+        force[2] += T(L::CZ[j]) * (T(2) * fj);
         return fj;
       }
       return ld(src, m * N + rs);
@@ -574,7 +590,7 @@ struct Solver3D {
     T q = wclamp<T>(dF / std::max<T>(dF - dS, T(1e-6)), T(1e-3), T(1));
     T fi = fj;
     if (q < T(0.5)) {
-      int nx = x + CX3[i], ny = y + CY3[i], nz = z + CZ3[i];
+      int nx = x + L::CX[i], ny = y + L::CY[i], nz = z + L::CZ[i];
       if (nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D) {
         int64_t n = at(nx, ny, nz);
         if (!is_solid(n)) fi = T(2) * q * fj + (T(1) - T(2) * q) * ld(src, j * N + n);
@@ -582,9 +598,9 @@ struct Solver3D {
     } else {
       fi = fj / (T(2) * q) + (T(2) * q - T(1)) / (T(2) * q) * ld(src, i * N + idx);
     }
-    force[0] += T(CX3[j]) * (fj + fi);
-    force[1] += T(CY3[j]) * (fj + fi);
-    force[2] += T(CZ3[j]) * (fj + fi);
+    force[0] += T(L::CX[j]) * (fj + fi);
+    force[1] += T(L::CY[j]) * (fj + fi);
+    force[2] += T(L::CZ[j]) * (fj + fi);
     return fi;
   }
 
@@ -592,7 +608,7 @@ struct Solver3D {
   static T csum(const int* c, const T* f) {
     bool first = true;
     T acc = T(0);
-    for (int i = 0; i < 19; i++) {
+    for (int i = 0; i < Q; i++) {
       if (c[i] == 0) continue;
       if (first) {
         acc = c[i] > 0 ? f[i] : -f[i];
@@ -623,15 +639,15 @@ struct Solver3D {
   // u_x = sum_{c_x = 0} f + 2 sum_{c_x = +1} f at the outlet, so that the node's moments are exactly
   // the prescribed ones. Slip-face rows use the mirrored populations as known ones, so edges and
   // corners of the planes take the same rule.
-  void zou_he_face(T f[19], bool inlet) const {
+  void zou_he_face(T f[Q], bool inlet) const {
     const int in = inlet ? 1 : -1;
     T s0 = T(0), sk = T(0), ny = T(0), nz = T(0);
-    for (int i = 0; i < 19; i++) {
-      if (CX3[i] == 0) {
+    for (int i = 0; i < Q; i++) {
+      if (L::CX[i] == 0) {
         s0 = s0 + f[i];
-        if (CY3[i]) ny = CY3[i] > 0 ? ny + f[i] : ny - f[i];
-        if (CZ3[i]) nz = CZ3[i] > 0 ? nz + f[i] : nz - f[i];
-      } else if (CX3[i] == -in) {
+        if (L::CY[i]) ny = L::CY[i] > 0 ? ny + f[i] : ny - f[i];
+        if (L::CZ[i]) nz = L::CZ[i] > 0 ? nz + f[i] : nz - f[i];
+      } else if (L::CX[i] == -in) {
         sk = sk + f[i];
       }
     }
@@ -645,12 +661,12 @@ struct Solver3D {
       ux = s0 + T(2) * sk;
     }
     const T ru = rho * ux;
-    for (int i = 1; i < 19; i++) {
-      if (CX3[i] != in) continue;
-      const T k = (CY3[i] || CZ3[i]) ? T(1) / T(6) : T(1) / T(3);  // 6 w_i
+    for (int i = 1; i < Q; i++) {
+      if (L::CX[i] != in) continue;
+      const T k = (L::CY[i] || L::CZ[i]) ? T(1) / T(6) : T(1) / T(3);  // 6 w_i
       T v = in > 0 ? f[OPP[i]] + k * ru : f[OPP[i]] - k * ru;
-      if (CY3[i]) v = CY3[i] > 0 ? v - ny : v + ny;
-      if (CZ3[i]) v = CZ3[i] > 0 ? v - nz : v + nz;
+      if (L::CY[i]) v = L::CY[i] > 0 ? v - ny : v + ny;
+      if (L::CZ[i]) v = L::CZ[i] > 0 ? v - nz : v + nz;
       f[i] = v;
     }
   }
@@ -662,23 +678,23 @@ struct Solver3D {
     const int64_t rem = idx - int64_t(z) * WH;
     const int y = int(rem / W);
     const int x = int(rem - int64_t(y) * W);
-    T f[19];
+    T f[Q];
     bool outlet = false;
     if (fl == 0) {
       f[0] = ld(src, idx);
-      for (int i = 1; i < 19; i++) f[i] = ld(src, i * N + idx - CZ3[i] * WH - int64_t(CY3[i]) * W - CX3[i]);
+      for (int i = 1; i < Q; i++) f[i] = ld(src, i * N + idx - L::CZ[i] * WH - int64_t(L::CY[i]) * W - L::CX[i]);
     } else {
-      if (fl & FLAG_SOLID3) return;
-      const bool inlet = (fl & FLAG_INLET3) != 0;
+      if (fl & L::SOLID) return;
+      const bool inlet = (fl & L::INLET) != 0;
       if (inlet && !faces) {
-        for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), P.uIn, T(0), T(0)), idx, i);
+        for (int i = 0; i < Q; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), P.uIn, T(0), T(0)), idx, i);
         return;
       }
-      outlet = (fl & FLAG_OUTLET3) != 0;
+      outlet = (fl & L::OUTLET) != 0;
       T force[3] = {T(0), T(0), T(0)};
       f[0] = ld(src, idx);
       // (a Zou-He inlet rebuilds the populations entering from x = -1 instead of pulling them)
-      for (int i = 1; i < 19; i++) f[i] = inlet && CX3[i] > 0 ? T(0) : pull_slow(src, i, x, y, z, idx, fl, outlet, force);
+      for (int i = 1; i < Q; i++) f[i] = inlet && L::CX[i] > 0 ? T(0) : pull_slow(src, i, x, y, z, idx, fl, outlet, force);
       uint32_t sl = slot[size_t(idx)];
       if (sl != 0) {
         T* cf = &cellForce[size_t(sl - 1) * 4];
@@ -691,24 +707,24 @@ struct Solver3D {
     }
     // moments (step3d.ts:38-45)
     T drho = f[0];
-    for (int i = 1; i < 19; i++) drho = drho + f[i];
+    for (int i = 1; i < Q; i++) drho = drho + f[i];
     T rho = T(1) + drho;
     T invRho = T(1) / rho;
-    T ux = csum(CX3, f) * invRho;
-    T uy = csum(CY3, f) * invRho;
-    T uz = csum(CZ3, f) * invRho;
+    T ux = csum(L::CX, f) * invRho;
+    T uy = csum(L::CY, f) * invRho;
+    T uz = csum(L::CZ, f) * invRho;
     T usq = T(1.5) * (ux * ux + uy * uy + uz * uz);
     if (outlet) {
-      for (int i = 0; i < 19; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), ux, uy, uz), idx, i);
+      for (int i = 0; i < Q; i++) dst[size_t(i * N + idx)] = put(feq(i, T(1), ux, uy, uz), idx, i);
       return;
     }
     // c_i . u in x, y, z order (step3d.ts:14-17, 47)
-    T cu[19], eqS[19], eqA[19];
-    for (int a = 1; a < 19; a += 2) {
+    T cu[Q], eqS[Q], eqA[Q];
+    for (int a = 1; a < Q; a += 2) {
       bool first = true;
       T acc = T(0);
       const T u[3] = {ux, uy, uz};
-      const int c[3] = {CX3[a], CY3[a], CZ3[a]};
+      const int c[3] = {L::CX[a], L::CY[a], L::CZ[a]};
       for (int k = 0; k < 3; k++) {
         if (c[k] == 0) continue;
         if (first) {
@@ -719,8 +735,8 @@ struct Solver3D {
         }
       }
       cu[a] = T(3) * acc;
-      eqS[a] = T(W3[a]) * (drho + rho * (T(0.5) * cu[a] * cu[a] - usq));
-      eqA[a] = T(W3[a]) * rho * cu[a];
+      eqS[a] = T(L::W[a]) * (drho + rho * (T(0.5) * cu[a] * cu[a] - usq));
+      eqA[a] = T(L::W[a]) * rho * cu[a];
     }
     // sponge (step3d.ts:213-219)
     T omegaBase = T(1) / P.tau;
@@ -733,16 +749,16 @@ struct Solver3D {
     // Smagorinsky (step3d.ts:59-74)
     T omP = omegaBase;
     if (P.smagC2 > T(0)) {
-      T n[19];
+      T n[Q];
       n[0] = T(0);
-      for (int a = 1; a < 19; a += 2) {
+      for (int a = 1; a < Q; a += 2) {
         n[a] = f[a] - (eqS[a] + eqA[a]);
         n[a + 1] = f[a + 1] - (eqS[a] - eqA[a]);
       }
-      int cxx[19], cyy[19], czz[19], cxy[19], cxz[19], cyz[19];
-      for (int i = 0; i < 19; i++) {
-        cxx[i] = CX3[i] * CX3[i], cyy[i] = CY3[i] * CY3[i], czz[i] = CZ3[i] * CZ3[i];
-        cxy[i] = CX3[i] * CY3[i], cxz[i] = CX3[i] * CZ3[i], cyz[i] = CY3[i] * CZ3[i];
+      int cxx[Q], cyy[Q], czz[Q], cxy[Q], cxz[Q], cyz[Q];
+      for (int i = 0; i < Q; i++) {
+        cxx[i] = L::CX[i] * L::CX[i], cyy[i] = L::CY[i] * L::CY[i], czz[i] = L::CZ[i] * L::CZ[i];
+        cxy[i] = L::CX[i] * L::CY[i], cxz[i] = L::CX[i] * L::CZ[i], cyz[i] = L::CY[i] * L::CZ[i];
       }
       T pxx = csum(cxx, n), pyy = csum(cyy, n), pzz = csum(czz, n);
       T pxy = csum(cxy, n), pxz = csum(cxz, n), pyz = csum(cyz, n);
@@ -754,20 +770,20 @@ struct Solver3D {
     T omM = T(1) / (T(0.5) + P.lambda / (T(1) / omP - T(0.5)));
     // absorbing layer (step3d.ts:76-90)
     T s0 = T(0);
-    T sS[19] = {}, sA[19] = {};
+    T sS[Q] = {}, sA[Q] = {};
     if (sigma > T(0)) {
       T ut = P.uIn;
       T dq = T(1.5) * rho * (ut * ut - ux * ux - uy * uy - uz * uz);
-      s0 += sigma * T(W3[0]) * -dq;
-      for (int a = 1; a < 19; a += 2) {
-        T cut = T(3) * T(CX3[a]) * ut;
-        sS[a] += sigma * T(W3[a]) * (T(0.5) * rho * (cut * cut - cu[a] * cu[a]) - dq);
-        sA[a] += sigma * T(W3[a]) * rho * (cut - cu[a]);
+      s0 += sigma * T(L::W[0]) * -dq;
+      for (int a = 1; a < Q; a += 2) {
+        T cut = T(3) * T(L::CX[a]) * ut;
+        sS[a] += sigma * T(L::W[a]) * (T(0.5) * rho * (cut * cut - cu[a] * cu[a]) - dq);
+        sA[a] += sigma * T(L::W[a]) * rho * (cut - cu[a]);
       }
     }
     // collide (step3d.ts:92-102)
-    dst[size_t(idx)] = put(f[0] - omP * (f[0] - T(W3[0]) * (drho - rho * usq)) + s0, idx, 0);
-    for (int a = 1; a < 19; a += 2) {
+    dst[size_t(idx)] = put(f[0] - omP * (f[0] - T(L::W[0]) * (drho - rho * usq)) + s0, idx, 0);
+    for (int a = 1; a < Q; a += 2) {
       int b = a + 1;
       T sym = omP * (T(0.5) * (f[a] + f[b]) - eqS[a]);
       T asym = omM * (T(0.5) * (f[a] - f[b]) - eqA[a]);
@@ -802,17 +818,17 @@ struct Solver3D {
 #pragma omp parallel for schedule(static)
     for (int64_t idx = 0; idx < N; idx++) {
       T* m = &mac[size_t(idx * 4)];
-      if (flags[size_t(idx)] & FLAG_SOLID3) {
+      if (flags[size_t(idx)] & L::SOLID) {
         m[0] = m[1] = m[2] = m[3] = T(0);
         continue;
       }
       T drho = T(0), jx = T(0), jy = T(0), jz = T(0);
-      for (int i = 0; i < 19; i++) {
+      for (int i = 0; i < Q; i++) {
         T v = ld(b, i * N + idx);
         drho += v;
-        jx += T(CX3[i]) * v;
-        jy += T(CY3[i]) * v;
-        jz += T(CZ3[i]) * v;
+        jx += T(L::CX[i]) * v;
+        jy += T(L::CY[i]) * v;
+        jz += T(L::CZ[i]) * v;
       }
       T rho = T(1) + drho;
       m[0] = jx / rho, m[1] = jy / rho, m[2] = jz / rho, m[3] = rho;
@@ -836,16 +852,16 @@ struct Solver3D {
       int y = int((idx - int64_t(z) * WH) / W);
       int x = int(idx - int64_t(z) * WH - int64_t(y) * W);
       if (solid_at(x, y, z)) {
-        flags[size_t(idx)] = FLAG_SOLID3;
+        flags[size_t(idx)] = L::SOLID;
         slot[size_t(idx)] = 0;
         continue;
       }
       uint32_t bits = 0;
-      if (x == 0) bits = FLAG_INLET3;
-      if (x == W - 1) bits = FLAG_OUTLET3;
+      if (x == 0) bits = L::INLET;
+      if (x == W - 1) bits = L::OUTLET;
       bool touches = false;
-      for (int i = 1; i < 19; i++) {
-        int sx = x - CX3[i], sy = y - CY3[i], sz = z - CZ3[i];
+      for (int i = 1; i < Q; i++) {
+        int sx = x - L::CX[i], sy = y - L::CY[i], sz = z - L::CZ[i];
         if (in_range(sx, sy, sz)) {
           if (solid_at(sx, sy, sz)) {
             bits |= 1u << i;
@@ -855,19 +871,17 @@ struct Solver3D {
         }
         bits |= 1u << i;
         bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
-        if (!outX && outY != outZ) {
-          int rx = sx, ry = sy, rz = sz;
-          if (outY) ry = y;
-          else rz = z;
+        if (!outX && (outY || outZ)) {
+          int rx = sx, ry = outY ? y : sy, rz = outZ ? z : sz;
           if (solid_at(rx, ry, rz)) touches = true;
         }
       }
       flags[size_t(idx)] = bits;
       slot[size_t(idx)] = touches ? ++counter : 0u;
-      if (old[size_t(idx)] & FLAG_SOLID3) {
+      if (old[size_t(idx)] & L::SOLID) {
         T rho = T(0), ux = T(0), uy = T(0), uz = T(0), n = T(0);
-        for (int i = 1; i < 19; i++) {
-          int qx = x + CX3[i], qy = y + CY3[i], qz = z + CZ3[i];
+        for (int i = 1; i < Q; i++) {
+          int qx = x + L::CX[i], qy = y + L::CY[i], qz = z + L::CZ[i];
           if (!in_range(qx, qy, qz)) continue;
           const T* m = &mac[size_t(at(qx, qy, qz) * 4)];
           if (m[3] != T(0)) {
@@ -889,7 +903,7 @@ struct Solver3D {
           uy = T(0);
           uz = T(0);
         }
-        for (int i = 0; i < 19; i++) {
+        for (int i = 0; i < Q; i++) {
           E v = S::template store<T>(feq(i, rho, ux, uy, uz));
           f[0][size_t(i * N + idx)] = v;
           f[1][size_t(i * N + idx)] = v;
@@ -901,8 +915,8 @@ struct Solver3D {
   // src/solver3d.ts:231-242 + src/shaders/aux3d.ts:113-123
   void init_field() {
     for (int64_t idx = 0; idx < N; idx++) {
-      bool solid = (flags[size_t(idx)] & FLAG_SOLID3) != 0;
-      for (int i = 0; i < 19; i++) {
+      bool solid = (flags[size_t(idx)] & L::SOLID) != 0;
+      for (int i = 0; i < Q; i++) {
         E v = S::template store<T>(solid ? T(0) : feq(i, T(1), P.uIn, T(0), T(0)));
         f[0][size_t(i * N + idx)] = v;
         f[1][size_t(i * N + idx)] = v;

@@ -196,8 +196,18 @@ MY3 = np.array([int(np.where((C3 == c * [1, -1, 1]).all(axis=1))[0][0]) for c in
 MZ3 = np.array([int(np.where((C3 == c * [1, 1, -1]).all(axis=1))[0][0]) for c in C3])
 
 
-def run3d(cfg: dict, sdf, steps: int):
-    """Returns (f - w as (19, N), list of (fx, fy, fz) per step)."""
+# D3Q27, built independently of any table: every velocity of {-1, 0, 1}^3, weighted by how many
+# components are non-zero (8/27, 2/27, 1/54, 1/216). Its order is itertools.product's, not the oracle's.
+C27 = np.array([[cx, cy, cz] for cx in (-1, 0, 1) for cy in (-1, 0, 1) for cz in (-1, 0, 1)])
+W27 = np.array([(8 / 27, 2 / 27, 1 / 54, 1 / 216)[int(np.abs(c).sum())] for c in C27])
+
+
+def run3d(cfg: dict, sdf, steps: int, C3=C3, W3=W3):
+    """Returns (f - w as (Q, N), list of (fx, fy, fz) per step) for the velocity set C3, W3."""
+    Q = len(C3)
+    OPP3 = _opp(C3)
+    MY3 = np.array([int(np.where((C3 == c * [1, -1, 1]).all(axis=1))[0][0]) for c in C3])
+    MZ3 = np.array([int(np.where((C3 == c * [1, 1, -1]).all(axis=1))[0][0]) for c in C3])
     W, H, D = cfg["width"], cfg["height"], cfg["depth"]
     tau = cfg["tau"]
     lam = cfg.get("lambda")
@@ -218,13 +228,16 @@ def run3d(cfg: dict, sdf, steps: int):
     F = np.where(fluid[None], feq(C3, W3, rho0, u0), W3[:, None, None, None])  # solids hold f - w = 0
     om_p0, s_out = _relax_rates(cfg, W, xs.astype(float))
     sigma = absorb * s_out ** 2
-    pairs = [(a, a + 1) for a in range(1, 19, 2)]
+    rest = int(np.where((C3 == 0).all(axis=1))[0][0])
+    pairs = [(a, int(OPP3[a])) for a in range(Q) if a < OPP3[a]]
     forces = []
     for _ in range(steps):
         Fin = np.empty_like(F)
         force = np.zeros((D, H, W, 3))
-        Fin[0] = F[0]
-        for i in range(1, 19):
+        Fin[rest] = F[rest]
+        for i in range(Q):
+            if i == rest:
+                continue
             j = OPP3[i]
             sx, sy, sz = xs - C3[i, 0], ys - C3[i, 1], zs - C3[i, 2]
             ox = (sx < 0) | (sx >= W)
@@ -244,6 +257,11 @@ def run3d(cfg: dict, sdf, steps: int):
             fs = face & ~(ox & outlet)
             v = np.where(fs & ~r_solid, mirrored, v)
             through = fs & r_solid & fluid
+            # slip edges (y and z out, x in): specular reflection through both faces
+            edge = ~ox & oy & oz
+            e_solid = solid[zs, ys, sxc]
+            v = np.where(edge & ~e_solid, F[MY3[MZ3[i]], zs, ys, sxc], v)
+            through |= edge & e_solid & fluid
             for d in range(3):
                 force[..., d] += np.where(through, C3[j, d] * 2 * (F[j] - W3[j]), 0.0)
             inside = ~ox & ~oy & ~oz
@@ -279,7 +297,7 @@ def run3d(cfg: dict, sdf, steps: int):
             om_p = 1 / (0.5 * (t0 + np.sqrt(t0 * t0 + 18 * np.sqrt(2) * smag_c2 * norm / rho)))
         om_m = _omega_minus(om_p, lam)
         out = np.empty_like(Fin)
-        out[0] = Fin[0] - om_p * (Fin[0] - Fe[0])
+        out[rest] = Fin[rest] - om_p * (Fin[rest] - Fe[rest])
         for a, b in pairs:
             fp, fm = 0.5 * (Fin[a] + Fin[b]), 0.5 * (Fin[a] - Fin[b])
             ep, em = 0.5 * (Fe[a] + Fe[b]), 0.5 * (Fe[a] - Fe[b])
@@ -294,4 +312,4 @@ def run3d(cfg: dict, sdf, steps: int):
         u_inflow[..., 0] = u_in
         out = np.where(inlet[None], feq(C3, W3, np.ones_like(rho), u_inflow), out)
         F = np.where(fluid[None], out, F)
-    return (F - W3[:, None, None, None]).reshape(19, -1), np.array(forces)
+    return (F - W3[:, None, None, None]).reshape(Q, -1), np.array(forces)

@@ -352,3 +352,60 @@ def test_d3q19_zou_he_faces_match_oracle(precision, variant):
     np.testing.assert_array_equal(gpu.read_populations_raw(), ref.read_populations_raw())
     for s in (gpu, ref, o32, o64):
         s.destroy()
+
+
+@pytest.mark.parametrize("precision,smag", [("fp32", 0.0), ("fp32", 0.12), ("fp16", 0.0), ("bf16", 0.0)])
+def test_d3q27_fields_match_oracle(precision, smag):
+    """D3Q27 (velocity_set 27): GPU against the oracle's D3Q27 on the parity tunnel plus a body on a slip
+    edge (corner directions reflected through both faces meet a solid node), flags bit-exact,
+    populations within the D3Q19 bounds, force samples at the same steps."""
+    cfg, sdf = case3(precision, smag=smag)
+    cfg["velocitySet"] = 27
+    W, H, D = cfg["width"], cfg["height"], cfg["depth"]
+    g.add_sphere(sdf, W, H, D, 12, 0.3, D - 0.8, 1.5)
+    gpu = Solver3D.create(dict(cfg))
+    o32 = OracleSolver3D(dict(cfg), 32)
+    assert gpu.Q == 27 and o32.Q == 27
+    for s in (gpu, o32):
+        s.set_sdf(sdf)
+        s.init_field()
+    np.testing.assert_array_equal(gpu.read_flags(), o32.read_flags())
+    tol = {"fp32": TOL32, "fp16": TOL16, "bf16": TOLBF}[precision]
+    done = 0
+    for n in (1, 10, 100):
+        gpu.run(n - done)
+        o32.steps(n - done)
+        done = n
+        e = _err(gpu.read_populations(), o32.read_populations())
+        print(f"d3q27 {precision} smag {smag} step {n}: |gpu-o32| {e:.3e}")
+        assert np.isfinite(gpu.read_populations()).all()
+        assert e <= tol[n], (n, e)
+    sg, vg = gpu.read_forces_raw()
+    so, vo = o32.read_forces_raw()
+    np.testing.assert_array_equal(sg, so)
+    assert np.abs(vg - vo).max() / np.abs(vo).max() < {"fp32": 1e-4, "fp16": 2e-3, "bf16": 2e-2}[precision]
+    m = gpu.read_macro().reshape(D, H, W, 4)
+    mo = o32.read_macro().reshape(D, H, W, 4)
+    assert np.abs(m - mo).max() < {"fp32": 1e-6, "fp16": 1e-3, "bf16": 5e-3}[precision]
+    for s in (gpu, o32):
+        s.destroy()
+
+
+def test_d3q27_uniform_flow_and_unsupported_paths():
+    """Without a body the inflow equilibrium is a fixed point of the D3Q27 step on the GPU too; the
+    paths written for D3Q19 only (Zou-He faces, pinned launch variants, the adjoint) refuse D3Q27."""
+    from cells_amd.lbm import LbmError
+
+    cfg = dict(width=40, height=18, depth=14, precision="fp32", tau=0.53, uIn=0.07, spongeFraction=0.3, spongeTau=1,
+               absorb=0.05, velocitySet=27, forceEvery=5)
+    s = Solver3D.create(cfg)
+    s.init_field()
+    f0 = s.read_populations()
+    s.run(60)
+    assert np.abs(s.read_populations() - f0).max() < 1e-6
+    assert len(s.read_forces()) == 12
+    with pytest.raises(LbmError):
+        s.set_open_faces("zou-he")
+    with pytest.raises(LbmError):
+        s.set_launch(64, 1)
+    s.destroy()
