@@ -121,6 +121,9 @@ def trt_lambda3(tau: float) -> float:
 _lib = None
 
 
+ABI_VERSION = 2  # LBM_ABI_VERSION of include/lbm.h that the structures below follow
+
+
 def load_library(path: str = LIB_PATH):
     """Loads the gfx950 library; raises if it has not been built."""
     global _lib
@@ -189,6 +192,8 @@ def load_library(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    if lib.lbm_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} has ABI {lib.lbm_abi_version()}, this module expects {ABI_VERSION}: rebuild it")
     _lib = lib
     return lib
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2 /* 2: lbm_config3d gained velocity_set */
 
 /* Status codes. */
 #define LBM_OK 0

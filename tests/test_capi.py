@@ -37,7 +37,7 @@ def test_host_only_calls():
     from cells_amd import lbm
 
     lib = lbm.load_library()  # binds every signature; raises if a symbol is missing
-    assert lib.lbm_abi_version() == 1
+    assert lib.lbm_abi_version() == 2
     c2 = lbm.Config2D()
     lib.lbm_config2d_defaults(C.byref(c2))
     assert abs(c2.lambda_ - 3 / 16) < 1e-15 and c2.force_every == 1
