@@ -102,7 +102,7 @@ test('addon exports every entry point', () => {
   for (const k of ['create2d', 'create3d', 'destroy', 'update2d', 'update3d', 'setSdf', 'initField', 'resetForces',
     'enqueueSteps', 'enqueueMacro', 'syncAsync', 'readMacroAsync', 'readForcesAsync', 'readPopulations', 'readFlags',
     'info', 'timeSteps', 'setInletRamp', 'adjointForwardAsync', 'adjointBackwardAsync', 'rebuildFlags', 'setOpenFaces', 'brush', 'brushStroke', 'deviceName', 'copyRoofline', 'autotune']) assert.strictEqual(typeof a[k], 'function', k);
-  assert.strictEqual(a.abiVersion(), 1);
+  assert.strictEqual(a.abiVersion(), 2);
 });
 
 // An exclusive pass (ShapeAdjoint) starts after the pending waits and locks the solver until it settles.
