@@ -200,25 +200,20 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
   float f[27];
   const bool outlet = (fl & FLAG_OUTLET27) != 0u;
   if (!(fl & FLAG_SLOW27)) {
-    // Source of each population by arithmetic: a direction whose bit is set leaves the grid (no
-    // solid is involved on these nodes): across x at the outlet (copy from (W-2, y, z), the
-    // previous node of the row), or across a slip face or edge (the mirrored population at the
-    // reflected node; which of the three applies is the same for the whole row).
+    // Source of each population by arithmetic. On these nodes a direction leaves the grid only
+    // across a slip face or edge of the row (the mirrored population at the reflected node) or, at
+    // the outlet node, across x (copy from (W-2, y, z), the previous node of the row); no solid is
+    // involved. So the source is a row-uniform population and cell offset (scalar registers) plus
+    // the lane's x, with the outlet node's nine c_x < 0 directions replaced.
+    const int rowbase = p.W * y + int(p.WH) * z;
 #pragma unroll
     for (int i = 0; i < 27; i++) {
       const int sy = y - CY27[i], sz = z - CZ27[i];
       const bool outY = sy < 0 || sy >= p.H, outZ = sz < 0 || sz >= p.D;
-      const int delta = CZ27[i] * int(p.WH) + CY27[i] * p.W + CX27[i];
-      uint32_t off = uint32_t(i) * N + uint32_t(int(idx) - delta);
-      if (i != 0 && (fl & (1u << i))) {
-        if (outlet && CX27[i] < 0) {
-          off = uint32_t(i) * N + idx - 1u;
-        } else if (outY || outZ) {
-          const int m = outY && outZ ? mirror_y27(mirror_z27(i)) : outY ? mirror_y27(i) : mirror_z27(i);
-          const uint32_t ry = uint32_t(outY ? y : sy), rz = uint32_t(outZ ? z : sz);
-          off = uint32_t(m) * N + uint32_t(x - CX27[i]) + uint32_t(p.W) * (ry + uint32_t(p.H) * rz);
-        }
-      }
+      const int m = outY && outZ ? mirror_y27(mirror_z27(i)) : outY ? mirror_y27(i) : outZ ? mirror_z27(i) : i;
+      const int c = rowbase - (outY ? 0 : CY27[i] * p.W) - (outZ ? 0 : CZ27[i] * int(p.WH)) - CX27[i];
+      uint32_t off = uint32_t(m) * N + uint32_t(c + x);
+      if (CX27[i] < 0 && outlet) off = uint32_t(i) * N + idx - 1u;
       f[i] = S::load(src, off);
     }
     if (outlet) {  // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity
