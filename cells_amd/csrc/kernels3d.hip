@@ -289,7 +289,7 @@ struct FeqIn {
 // node's flag word must be exactly the face bits of its row (the host moves nodes whose pulls
 // differ, e.g. where the mirror node is solid, to the boundary list and marks them FLAG_LIST3).
 // Outlet nodes of face rows go to the boundary list as well.
-template <class S, bool REC, bool SMAG>
+template <class S, bool REC, bool SMAG, bool ZH>
 __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
                                            uint32_t nbb, const Params3& p, const FeqIn& fin,
@@ -310,7 +310,7 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
 // shares one store per population (whole lines per store instead of a line with a one-node hole
 // plus a one-node store, which costs read-modify-writes where the lines go to HBM rather than stay
 // in cache). Both give the same values; the autotuner picks per grid.
-template <class S, bool REC, bool SMAG, bool MERGE>
+template <class S, bool REC, bool SMAG, bool MERGE, bool ZH>
 __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   using E = typename S::elem;
   if (blockIdx.z < zl) {
     const uint32_t lb = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-    if (lb < nlb + nbb) list_part3<S, REC, SMAG>(src, dst, flags, lt, nlb, nbb, p, fin, xtab, fs, rc, lb);
+    if (lb < nlb + nbb) list_part3<S, REC, SMAG, ZH>(src, dst, flags, lt, nlb, nbb, p, fin, xtab, fs, rc, lb);
     return;
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
-  const bool zh = p.faces != 0u;
+  constexpr bool zh = ZH;  // Zou-He faces (p.faces != 0), compiled apart so the default kernels carry none of it
   if (!MERGE && inlet && !zh) {
     // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0), no collision, no force.
 #pragma unroll
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
 // replaces the gathered f_j with the interpolation against its record's second operand and adds
 // c_j (f_j + f_i) to the force; a slip face mirroring a solid node (step3d.ts:139-146) keeps f_j and
 // adds c_j 2 f_j.
-template <class S, bool BODY, bool REC, bool SMAG>
+template <class S, bool BODY, bool REC, bool SMAG, bool ZH>
 __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
                                            const Params3& p, const FeqIn& fin, const float4* __restrict__ xtab,
@@ -442,7 +442,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   const uint32_t idx = lt.list[k];
   const uint32_t fl = flags[idx];
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
-  const bool zh = p.faces != 0u;
+  constexpr bool zh = ZH;
   if ((fl & FLAG_INLET3) && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, fin.v[i], nw, i);
@@ -518,7 +518,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
 // Boundary-list part of the step: list block lb of the launch. List blocks [0, nlb) take the nodes
 // without body links; the nbb blocks after them take the nodes with body links and carry the force
 // sample (an FP64 reduction over those blocks).
-template <class S, bool REC, bool SMAG>
+template <class S, bool REC, bool SMAG, bool ZH>
 __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t nlb,
                                            uint32_t nbb, const Params3& p, const FeqIn& fin,
@@ -527,11 +527,11 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
   float force[3] = {0.0f, 0.0f, 0.0f};
   if (lb < nlb) {
     const uint32_t k = lb * blockDim.x + threadIdx.x;
-    if (k < lt.n_link) list_node3<S, false, REC, SMAG>(src, dst, flags, lt, k, p, fin, xtab, rc, force);
+    if (k < lt.n_link) list_node3<S, false, REC, SMAG, ZH>(src, dst, flags, lt, k, p, fin, xtab, rc, force);
     return;
   }
   const uint32_t kb = (lb - nlb) * blockDim.x + threadIdx.x;
-  if (kb < lt.n_body) list_node3<S, true, REC, SMAG>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, rc, force);
+  if (kb < lt.n_body) list_node3<S, true, REC, SMAG, ZH>(src, dst, flags, lt, lt.n_link + kb, p, fin, xtab, rc, force);
   if (fs.sample) force_sample<3>(force, fs, lb - nlb, nbb);
 }
 
@@ -682,21 +682,30 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const Lis
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   const Record none{nullptr, nullptr};
-  if (rec && merge)  // recording steps (the adjoint) never use the subgrid model
-    step3_kernel<S, true, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec,
-                                                             desc, row_clean);
+  if (rec && merge)  // recording steps (the adjoint) never use the subgrid model or the Zou-He faces
+    step3_kernel<S, true, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                    *rec, desc, row_clean);
   else if (rec)
-    step3_kernel<S, true, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, *rec,
-                                                              desc, row_clean);
+    step3_kernel<S, true, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                     *rec, desc, row_clean);
+  else if (p.faces && p.smagC2 > 0.0f)
+    step3_kernel<S, false, true, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                    none, desc, row_clean);
+  else if (p.faces && merge)
+    step3_kernel<S, false, false, true, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                    none, desc, row_clean);
+  else if (p.faces)
+    step3_kernel<S, false, false, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                     none, desc, row_clean);
   else if (p.smagC2 > 0.0f)
-    step3_kernel<S, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
-                                                              desc, row_clean);
+    step3_kernel<S, false, true, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                     none, desc, row_clean);
   else if (merge)
-    step3_kernel<S, false, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
-                                                              desc, row_clean);
+    step3_kernel<S, false, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                     none, desc, row_clean);
   else
-    step3_kernel<S, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, none,
-                                                               desc, row_clean);
+    step3_kernel<S, false, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
+                                                                      none, desc, row_clean);
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
