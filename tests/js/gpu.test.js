@@ -40,6 +40,20 @@ async function main() {
   assert.ok(du < 1e-6 && drho > 1e-6, `${du} ${drho}`);
   assert.throws(() => s3.setOpenFaces('open'), /unknown open faces/);
   s3.setOpenFaces('equilibrium');
+
+  // --- D3Q27 velocity set through the same wrapper
+  const q27 = await m.Solver3D.create(0, { width: W, height: H, depth: D, precision: 'fp32', tau: 0.524, uIn: 0.05,
+    spongeFraction: 0.15, spongeTau: 1, absorb: 0.02, forceEvery: 1, velocitySet: 27 });
+  q27.setSdf(sdf);
+  q27.initField();
+  await q27.run(50, 25);
+  const f27 = await q27.readForces();
+  assert.strictEqual(f27.length, 50);
+  assert.ok(Math.abs(f27[49].fx / f[49].fx - 1) < 0.2, `${f27[49].fx} ${f[49].fx}`);
+  assert.strictEqual((await q27.readPopulations()).length, 27 * W * H * D);
+  assert.throws(() => q27.setOpenFaces('zouHe'), /D3Q19 only/);
+  q27.destroy();
+  console.log(`ok d3q27: fx ${f27[49].fx.toExponential(4)} against d3q19 ${f[49].fx.toExponential(4)}`);
   console.log(`ok zou-he faces: inlet |u - (uIn, 0, 0)| ${du.toExponential(2)}, |rho - 1| ${drho.toExponential(2)}`);
 
   // --- stale read across resetForces (2D and 3D)
