@@ -155,6 +155,15 @@ void inlet_feq3(float u, float* out) {
   }
 }
 
+// The same for D3Q27 (kernels27.hip's feq27 in the same operation order).
+void inlet_feq27(float u, float* out) {
+  const float rho = 1.0f, uy = 0.0f, uz = 0.0f;
+  for (int i = 0; i < 27; i++) {
+    float cu = 3.0f * (float(CX27[i]) * u + float(CY27[i]) * uy + float(CZ27[i]) * uz);
+    out[i] = W27[i] * (rho - 1.0f + rho * (cu + 0.5f * cu * cu - 1.5f * (u * u + uy * uy + uz * uz)));
+  }
+}
+
 // Host resolution of one pull of a boundary node, following the rule order of pullSlow3
 // (src/shaders/step3d.ts:119-168) and pullSlow (src/shaders/step.ts:119-175). Every pull becomes a
 // gather index (population * N + cell); a Bouzidi pull also gets its record (q and the second
@@ -569,7 +578,9 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
                  s->bulk_block, s->stream, rec);
   } else if (s->Q == 27) {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
-    launch_step27(s->prec, src, dst, s->flags, s->sdf, s->xtab, s->p3, fs, s->list, s->n_link, s->stream);
+    float fin[27];
+    inlet_feq27(s->p3.uIn, fin);
+    launch_step27(s->prec, src, dst, s->flags, s->sdf, s->xtab, s->p3, fin, fs, s->list, s->n_link, s->stream);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];

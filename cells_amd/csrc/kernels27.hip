@@ -180,26 +180,32 @@ __device__ __forceinline__ float pull27(const typename S::elem* __restrict__ src
   return fi;
 }
 
-// src/shaders/step3d.ts:170-223 for node (x, y, z) over 27 directions. Returns whether the node is
-// a FLAG_SLOW27 node (the only kind that adds to force).
+// Inlet equilibrium feq(i, 1, (uIn, 0, 0)), uniform over the inlet plane: computed on the host per
+// step and passed by value.
+struct FeqIn27 {
+  float v[27];
+};
+
+// src/shaders/step3d.ts:170-223 for node (x, y, z) over 27 directions, leaving the values to store in
+// out: the inlet's equilibrium (no pulls, no collision), the outlet's equilibrium at rho = 1 with its
+// own velocity, or the collision. All nodes of a wave then store together (27 stores per wave,
+// whole lines, also for the waves holding the inlet or outlet node). Returns whether the node is a
+// FLAG_SLOW27 node (the only kind that adds to force).
 template <class S, bool SMAG>
-__device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
-                                       const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                       const float4* __restrict__ xtab, const Params3& p, int x, int y, int z,
-                                       float (&force)[3]) {
+__device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src, const uint32_t* __restrict__ flags,
+                                       const float* __restrict__ sdf, const float4* __restrict__ xtab, const Params3& p,
+                                       const FeqIn27& fin, int x, int y, int z, uint32_t idx, uint32_t fl,
+                                       float (&force)[3], float (&out)[27]) {
   const uint32_t N = p.N;
-  const uint32_t idx = uint32_t(x) + uint32_t(p.W) * (uint32_t(y) + uint32_t(p.H) * uint32_t(z));
-  const uint32_t fl = flags[idx];
   const bool slow = (fl & FLAG_SLOW27) != 0u;
-  if (fl & FLAG_SOLID27) return false;
   if (fl & FLAG_INLET27) {  // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0)
 #pragma unroll
-    for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * N, idx, feq27(i, 1.0f, p.uIn, 0.0f, 0.0f));
+    for (int i = 0; i < 27; i++) out[i] = fin.v[i];
     return slow;
   }
   float f[27];
   const bool outlet = (fl & FLAG_OUTLET27) != 0u;
-  if (!(fl & FLAG_SLOW27)) {
+  if (!slow) {
     // Source of each population by arithmetic. On these nodes a direction leaves the grid only
     // across a slip face or edge of the row (the mirrored population at the reflected node) or, at
     // the outlet node, across x (copy from (W-2, y, z), the previous node of the row); no solid is
@@ -216,12 +222,6 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
       if (CX27[i] < 0 && outlet) off = uint32_t(i) * N + idx - 1u;
       f[i] = S::load(src, off);
     }
-    if (outlet) {  // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity
-      const Mom27 m = moments27(f);
-#pragma unroll
-      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * N, idx, feq27(i, 1.0f, m.ux, m.uy, m.uz));
-      return slow;
-    }
   } else {
     f[0] = S::load(src, idx);
 #pragma unroll
@@ -233,18 +233,14 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
         f[i] = S::load(src + size_t(i) * N, uint32_t(int(idx) - delta));
       }
     }
-    if (outlet) {  // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity
-      const Mom27 m = moments27(f);
-#pragma unroll
-      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * N, idx, feq27(i, 1.0f, m.ux, m.uy, m.uz));
-      return slow;
-    }
   }
   const Mom27 m = moments27(f);
-  float out[27];
-  collide27<SMAG>(f, m, xtab[x], p, out);
+  if (outlet) {  // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity
 #pragma unroll
-  for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * N, idx, out[i]);
+    for (int i = 0; i < 27; i++) out[i] = feq27(i, 1.0f, m.ux, m.uy, m.uz);
+  } else {
+    collide27<SMAG>(f, m, xtab[x], p, out);
+  }
   return slow;
 }
 
@@ -255,12 +251,22 @@ template <class S, bool SMAG>
 __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
-                                                     const float4* __restrict__ xtab, Params3 p,
+                                                     const float4* __restrict__ xtab, Params3 p, FeqIn27 fin,
                                                      double* __restrict__ partials, int sample) {
   const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = int(blockIdx.y), z = int(blockIdx.z);
   float force[3] = {0.0f, 0.0f, 0.0f};
   bool slow = false;
-  if (x < p.W) slow = node27<S, SMAG>(src, dst, flags, sdf, xtab, p, x, int(blockIdx.y), int(blockIdx.z), force);
+  if (x < p.W) {
+    const uint32_t idx = uint32_t(x) + uint32_t(p.W) * (uint32_t(y) + uint32_t(p.H) * uint32_t(z));
+    const uint32_t fl = flags[idx];
+    if (!(fl & FLAG_SOLID27)) {
+      float out[27];
+      slow = node27<S, SMAG>(src, flags, sdf, xtab, p, fin, x, y, z, idx, fl, force, out);
+#pragma unroll
+      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * p.N, idx, out[i]);
+    }
+  }
   if (!sample || __ballot(slow) == 0ull) return;  // uniform over the wave
   double v[3] = {double(force[0]), double(force[1]), double(force[2])};
   wave_sum<3>(v);
@@ -411,16 +417,19 @@ inline uint32_t blocks_for27(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 template <class S>
 void step27_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab, const Params3& p,
-              const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves, hipStream_t st) {
+              const float* feq_in, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
+              hipStream_t st) {
+  FeqIn27 fin;
+  for (int i = 0; i < 27; i++) fin.v[i] = feq_in[i];
   using E = typename S::elem;
   const uint32_t bx = step27_block_x(p.W);
   const dim3 grid(blocks_for27(uint32_t(p.W), bx), uint32_t(p.H), uint32_t(p.D));
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   if (p.smagC2 > 0.0f)
-    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fs.partials, fs.sample);
+    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fin, fs.partials, fs.sample);
   else
-    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fs.partials, fs.sample);
+    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fin, fs.partials, fs.sample);
   if (fs.sample) reduce27_kernel<<<1, 64, 0, st>>>(fs.partials, force_waves, n_force_waves, fs.ring, fs.state);
 }
 
@@ -434,12 +443,12 @@ size_t step27_partials(int W, int H, int D) {
 }
 
 void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab,
-                   const Params3& p, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
-                   hipStream_t st) {
+                   const Params3& p, const float* feq_in, const ForceSink& fs, const uint32_t* force_waves,
+                   uint32_t n_force_waves, hipStream_t st) {
   switch (prec) {
-    case 1: step27_t<StoreF16>(src, dst, flags, sdf, xtab, p, fs, force_waves, n_force_waves, st); break;
-    case 2: step27_t<StoreBF16>(src, dst, flags, sdf, xtab, p, fs, force_waves, n_force_waves, st); break;
-    default: step27_t<StoreF32>(src, dst, flags, sdf, xtab, p, fs, force_waves, n_force_waves, st); break;
+    case 1: step27_t<StoreF16>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    case 2: step27_t<StoreBF16>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    default: step27_t<StoreF32>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
   }
 }
 
