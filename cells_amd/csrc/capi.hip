@@ -489,6 +489,11 @@ int rebuild_flags_impl(lbm_solver* s) {
     if (!waves.empty())
       HIP_TRY(hipMemcpyAsync(s->list, waves.data(), 4 * waves.size(), hipMemcpyHostToDevice, s->stream));
     s->n_link = uint32_t(waves.size());
+    // rows without a solid or FLAG_SLOW27 node: the step derives their flag words from x alone
+    std::vector<uint8_t> clean(N / W, 1);
+    for (uint32_t k = 0; k < N; k++)
+      if (fl[k] & (FLAG_SOLID27 | FLAG_SLOW27)) clean[k / W] = 0;
+    HIP_TRY(hipMemcpyAsync(s->row_clean, clean.data(), clean.size(), hipMemcpyHostToDevice, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->tables_gen++;
     return LBM_OK;
@@ -580,7 +585,8 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[27];
     inlet_feq27(s->p3.uIn, fin);
-    launch_step27(s->prec, src, dst, s->flags, s->sdf, s->xtab, s->p3, fin, fs, s->list, s->n_link, s->stream);
+    launch_step27(s->prec, src, dst, s->flags, s->row_clean, s->sdf, s->xtab, s->p3, fin, fs, s->list, s->n_link,
+                  s->stream);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];

@@ -64,13 +64,14 @@ uint32_t body_list_blocks(uint32_t n_body, uint32_t bx, bool sample);
 
 // ---- D3Q27 (prec LBM_FP32 / LBM_FP16 / LBM_BF16), same parameter block and relaxation table ----
 // One launch per step (plus a one-wave reduction when a force sample is due); feq_in: the 27 inlet
-// equilibria feq(i, 1, (uIn, 0, 0)) of the step (inlet_feq27). Partials: 3 doubles per
+// equilibria feq(i, 1, (uIn, 0, 0)) of the step (inlet_feq27); row_clean: one byte per row (y + H z),
+// 1 where the row holds no solid or FLAG_SLOW27 node. Partials: 3 doubles per
 // wave index, up to step27_partials(W, H, D) waves; force_waves lists, ascending, the n_force_waves
 // wave indices holding a FLAG_SLOW27 node (wave = (block (z H + y) gx + x / bx) (bx / 64) + x % bx / 64,
 // bx = step27_block_x(W)).
-void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab,
-                   const Params3& p, const float* feq_in, const ForceSink& fs, const uint32_t* force_waves,
-                   uint32_t n_force_waves, hipStream_t st);
+void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean,
+                   const float* sdf, const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs,
+                   const uint32_t* force_waves, uint32_t n_force_waves, hipStream_t st);
 uint32_t step27_block_x(int W);
 size_t step27_partials(int W, int H, int D);
 void launch_macro27(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);

@@ -250,7 +250,8 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
 template <class S, bool SMAG>
 __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
-                                                     const uint32_t* __restrict__ flags, const float* __restrict__ sdf,
+                                                     const uint32_t* __restrict__ flags,
+                                                     const uint8_t* __restrict__ row_clean, const float* __restrict__ sdf,
                                                      const float4* __restrict__ xtab, Params3 p, FeqIn27 fin,
                                                      double* __restrict__ partials, int sample) {
   const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -259,7 +260,11 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
   bool slow = false;
   if (x < p.W) {
     const uint32_t idx = uint32_t(x) + uint32_t(p.W) * (uint32_t(y) + uint32_t(p.H) * uint32_t(z));
-    const uint32_t fl = flags[idx];
+    // In a clean row (no solid or FLAG_SLOW27 node) only the inlet and outlet bits matter, and they
+    // follow from x, so the loads need not wait for the flag word.
+    const uint32_t fl = row_clean[uint32_t(y) + uint32_t(p.H) * uint32_t(z)]
+                            ? (x == 0 ? FLAG_INLET27 : x == p.W - 1 ? FLAG_OUTLET27 : 0u)
+                            : flags[idx];
     if (!(fl & FLAG_SOLID27)) {
       float out[27];
       slow = node27<S, SMAG>(src, flags, sdf, xtab, p, fin, x, y, z, idx, fl, force, out);
@@ -416,8 +421,8 @@ __global__ __launch_bounds__(256) void flags27_kernel(const float* __restrict__ 
 inline uint32_t blocks_for27(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 template <class S>
-void step27_t(const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab, const Params3& p,
-              const float* feq_in, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
+void step27_t(const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean, const float* sdf,
+              const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
               hipStream_t st) {
   FeqIn27 fin;
   for (int i = 0; i < 27; i++) fin.v[i] = feq_in[i];
@@ -427,9 +432,9 @@ void step27_t(const void* src, void* dst, const uint32_t* flags, const float* sd
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   if (p.smagC2 > 0.0f)
-    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fin, fs.partials, fs.sample);
+    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
   else
-    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, sdf, xtab, p, fin, fs.partials, fs.sample);
+    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
   if (fs.sample) reduce27_kernel<<<1, 64, 0, st>>>(fs.partials, force_waves, n_force_waves, fs.ring, fs.state);
 }
 
@@ -442,13 +447,13 @@ size_t step27_partials(int W, int H, int D) {
   return size_t((uint32_t(W) + bx - 1) / bx) * size_t(H) * size_t(D) * (bx / 64u);
 }
 
-void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const float* sdf, const float4* xtab,
-                   const Params3& p, const float* feq_in, const ForceSink& fs, const uint32_t* force_waves,
-                   uint32_t n_force_waves, hipStream_t st) {
+void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean,
+                   const float* sdf, const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs,
+                   const uint32_t* force_waves, uint32_t n_force_waves, hipStream_t st) {
   switch (prec) {
-    case 1: step27_t<StoreF16>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
-    case 2: step27_t<StoreBF16>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
-    default: step27_t<StoreF32>(src, dst, flags, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    case 1: step27_t<StoreF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    case 2: step27_t<StoreBF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    default: step27_t<StoreF32>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
   }
 }
 
