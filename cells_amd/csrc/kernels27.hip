@@ -235,9 +235,25 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
     }
   }
   const Mom27 m = moments27(f);
-  if (outlet) {  // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity
+  if (outlet) {
+    // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's velocity, per opposite
+    // pair (c_b = -c_a): w (+-cu + cu^2 / 2 - 3 u^2 / 2) with the shared terms once (the outlet node
+    // shares its wave with 63 colliding nodes, so this branch costs the whole wave)
+    out[0] = W27[0] * -m.usq;
 #pragma unroll
-    for (int i = 0; i < 27; i++) out[i] = feq27(i, 1.0f, m.ux, m.uy, m.uz);
+    for (int a = 1; a < 27; a += 2) {
+      float cu = 0.0f;
+      if (CX27[a] > 0) cu = cu + m.ux;
+      if (CX27[a] < 0) cu = cu - m.ux;
+      if (CY27[a] > 0) cu = cu + m.uy;
+      if (CY27[a] < 0) cu = cu - m.uy;
+      if (CZ27[a] > 0) cu = cu + m.uz;
+      if (CZ27[a] < 0) cu = cu - m.uz;
+      cu = 3.0f * cu;
+      const float t = 0.5f * cu * cu - m.usq;
+      out[a] = W27[a] * (t + cu);
+      out[a + 1] = W27[a] * (t - cu);
+    }
   } else {
     collide27<SMAG>(f, m, xtab[x], p, out);
   }
