@@ -819,10 +819,16 @@ int lbm_set_open_faces(lbm_solver* s, int kind) {
   if (s->Q == 27 && kind != LBM_FACES_EQUILIBRIUM) return fail(LBM_ERR_UNSUPPORTED, "open faces: Zou-He faces are D3Q19 only");
   if (kind != LBM_FACES_EQUILIBRIUM && kind != LBM_FACES_ZOU_HE) return fail(LBM_ERR_INVALID, "open faces: unknown kind");
   if (kind == s->faces) return LBM_OK;
+  const int before = s->faces;
   s->faces = kind;
   write_params3(s);
   // which inlet nodes the bulk pass can take, and the inlet list nodes' gathers, depend on the kind
-  return rebuild_flags_impl(s);
+  const int rc = rebuild_flags_impl(s);
+  if (rc != LBM_OK) {  // keep the previous kind with the tables it was built for
+    s->faces = before;
+    write_params3(s);
+  }
+  return rc;
 }
 
 namespace {
