@@ -699,6 +699,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   if (!s || !cfg || !J) return fail(LBM_ERR_INVALID, "bad arguments");
   if (cfg->steps <= 0 || cfg->window_start < 0 || cfg->window_start >= cfg->steps || cfg->checkpoint_every <= 0)
     return fail(LBM_ERR_INVALID, "adjoint: need 0 <= window_start < steps and checkpoint_every > 0");
+  if (int rc = check_usable(s)) return rc;
   if (int rc = check_supported(s)) return rc;
   if (!s->adj) s->adj = new lbm_adjoint();
   lbm_adjoint* a = s->adj;
@@ -784,6 +785,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
 
 int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) {
   if (!s || !dJ_dsdf || n < s->N) return fail(LBM_ERR_INVALID, "dJ_dsdf must hold N doubles");
+  if (int rc = check_usable(s)) return rc;
   lbm_adjoint* a = s->adj;
   if (!a || !a->have_forward) return fail(LBM_ERR_INVALID, "adjoint: call lbm_adjoint_forward first");
   if (a->forward_gen != s->tables_gen)

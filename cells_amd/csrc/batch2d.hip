@@ -304,6 +304,8 @@ int lbm_batch_enqueue_steps(lbm_batch* b, int64_t n) {
   int rc = refresh(b);
   if (rc) return rc;
   for (lbm_solver* s : b->t)
+    if (int rc2 = check_usable(s)) return rc2;
+  for (lbm_solver* s : b->t)
     if (s->step != b->step || s->parity != b->parity)
       return fail(LBM_ERR_INVALID, "a tunnel of the batch was stepped or re-initialised on its own; call "
                                    "lbm_batch_init_all after initField");

@@ -293,20 +293,19 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
   const int Q = s->Q;
   std::vector<uint32_t> grp[3], body;
   bool list_marks = false;
-  s->n_bulk = 0;
-  s->n_solid = 0;
+  int64_t n_bulk = 0, n_solid = 0;
   auto resolve = [&](uint32_t idx, int i, uint32_t* g, BzLink* rec) {
     return d2 ? resolve2(s, fl.data(), idx, i, g, rec) : resolve3(s, fl.data(), idx, i, g, rec);
   };
   for (uint32_t k = 0; k < N; k++) {
     const uint32_t v = fl[k];
     if (v & solid) {
-      s->n_solid++;
+      n_solid++;
       continue;
     }
     if (d2) {
       if (v == 0 || v == plain_in || v == plain_out) {
-        s->n_bulk++;
+        n_bulk++;
         continue;
       }
     } else {
@@ -326,7 +325,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
         ok = resolve(k, i, &gi, &rec) == PULL_GATHER && gi == row_gather3(s, r, k, (v & outlet_flag) != 0u, i);
       }
       if (ok) {
-        s->n_bulk++;
+        n_bulk++;
         continue;
       }
       if (bk != 0) {
@@ -385,6 +384,51 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
     if (v >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: gather index out of range");
   for (const BzLink& r : recs)
     if (r.gb != NO_OPERAND && r.gb >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: operand out of range");
+  // The new device tables are built beside the old ones and swapped in only once every allocation
+  // and copy succeeded, so a failure leaves the handle's tables and counts as they were.
+  struct Tables {
+    uint32_t *gtab = nullptr, *bmask = nullptr, *xmask = nullptr, *lbase = nullptr;
+    BzLink* links = nullptr;
+    void release() {
+      for (void* p : {(void*)gtab, (void*)bmask, (void*)xmask, (void*)lbase, (void*)links})
+        if (p) (void)hipFree(p);
+      *this = Tables{};
+    }
+  } fresh;
+  auto upload = [&]() -> int {
+    if (s->debug_fail_rebuilds > 0) {
+      s->debug_fail_rebuilds--;
+      return lbm::fail(LBM_ERR_ALLOC, "boundary tables: device allocation failed (test hook)");
+    }
+    if (!nl) return LBM_OK;
+    HIP_TRY(hipMalloc(&fresh.gtab, sizeof(uint32_t) * gt.size()));
+    HIP_TRY(hipMalloc(&fresh.bmask, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&fresh.xmask, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&fresh.lbase, sizeof(uint32_t) * nl));
+    HIP_TRY(hipMalloc(&fresh.links, sizeof(BzLink) * std::max<size_t>(1, recs.size())));
+    HIP_TRY(hipMemcpyAsync(fresh.gtab, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(fresh.bmask, bm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(fresh.xmask, xm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(fresh.lbase, lb.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
+    if (!recs.empty())
+      HIP_TRY(hipMemcpyAsync(fresh.links, recs.data(), sizeof(BzLink) * recs.size(), hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+  };
+  if (const int rc = upload()) {
+    fresh.release();
+    return rc;
+  }
+  // Swap: steps enqueued before this call may still read the old tables (and the list), so they
+  // finish before the old buffers go and the list is overwritten.
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (void* p : {(void*)s->gtab, (void*)s->bmask, (void*)s->xmask, (void*)s->lbase, (void*)s->links})
+    if (p) (void)hipFree(p);
+  s->gtab = fresh.gtab;
+  s->bmask = fresh.bmask;
+  s->xmask = fresh.xmask;
+  s->lbase = fresh.lbase;
+  s->links = fresh.links;
   s->h_flags = fl;
   s->h_list = lst;
   s->h_gtab = gt;
@@ -399,24 +443,9 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
   s->n_link = n_link;
   s->n_body = nl - n_link;
   s->n_links = uint32_t(recs.size());
-  for (void** pp : {(void**)&s->gtab, (void**)&s->bmask, (void**)&s->xmask, (void**)&s->lbase, (void**)&s->links}) {
-    if (*pp) HIP_TRY(hipFree(*pp));
-    *pp = nullptr;
-  }
-  if (nl) {
-    HIP_TRY(hipMalloc(&s->gtab, sizeof(uint32_t) * gt.size()));
-    HIP_TRY(hipMalloc(&s->bmask, sizeof(uint32_t) * nl));
-    HIP_TRY(hipMalloc(&s->xmask, sizeof(uint32_t) * nl));
-    HIP_TRY(hipMalloc(&s->lbase, sizeof(uint32_t) * nl));
-    HIP_TRY(hipMalloc(&s->links, sizeof(BzLink) * std::max<size_t>(1, recs.size())));
-    HIP_TRY(hipMemcpyAsync(s->gtab, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->bmask, bm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->xmask, xm.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->lbase, lb.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
-    if (!recs.empty())
-      HIP_TRY(hipMemcpyAsync(s->links, recs.data(), sizeof(BzLink) * recs.size(), hipMemcpyHostToDevice, s->stream));
-    HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
-  }
+  s->n_bulk = n_bulk;
+  s->n_solid = n_solid;
+  if (nl) HIP_TRY(hipMemcpyAsync(s->list, lst.data(), sizeof(uint32_t) * nl, hipMemcpyHostToDevice, s->stream));
   if (list_marks) HIP_TRY(hipMemcpyAsync(s->flags, fl.data(), sizeof(uint32_t) * N, hipMemcpyHostToDevice, s->stream));
   if (!d2) {
     // rows whose flag words are exactly what their position implies: inlet at x = 0, plain fluid
@@ -446,7 +475,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
 
 // rebuildFlags (src/solver.ts:205-217, src/solver3d.ts:216-228): macro on the current buffer, flags
 // and refill on the GPU, then the boundary list (and 2D force slots) from the flags, in cell order.
-int rebuild_flags_impl(lbm_solver* s) {
+int rebuild_core(lbm_solver* s) {
   const uint32_t N = s->N;
   if (s->nd == 2) {
     launch_macro2(static_cast<const float*>(s->f[s->parity]), s->flags, s->mac, s->p2, s->stream);
@@ -515,6 +544,17 @@ int rebuild_flags_impl(lbm_solver* s) {
   return build_links(s, fl);
 }
 
+// Once the flag kernel ran, the device flags belong to the new geometry: a rebuild that fails after
+// it leaves them out of step with the boundary tables, so the handle refuses to step until a later
+// rebuild succeeds (check_usable).
+int tables_done(lbm_solver* s, int rc) {
+  if (rc == LBM_OK) s->invalid.clear();
+  else s->invalid = g_err;
+  return rc;
+}
+
+int rebuild_flags_impl(lbm_solver* s) { return tables_done(s, rebuild_core(s)); }
+
 
 int alloc_common(lbm_solver* s) {
   const size_t N = s->N;
@@ -569,6 +609,14 @@ int enqueue_one(lbm_solver* s) { return enqueue_step(s, nullptr, -1); }
 namespace lbm {
 
 int rebuild_flags(lbm_solver* s) { return rebuild_flags_impl(s); }
+
+int check_usable(const lbm_solver* s) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (!s->invalid.empty())
+    return fail(LBM_ERR_INVALID, "the last geometry rebuild failed (" + s->invalid +
+                                     "); set the SDF or rebuild the flags again before stepping");
+  return LBM_OK;
+}
 
 int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
   const int every = s->force_every();
@@ -771,7 +819,7 @@ int lbm_update_2d(lbm_solver* s, const lbm_config2d* cfg) {
     std::vector<uint32_t> fl(s->N);
     HIP_TRY(hipMemcpyAsync(fl.data(), s->flags, sizeof(uint32_t) * s->N, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
-    return build_links(s, fl);
+    return tables_done(s, build_links(s, fl));
   }
   return LBM_OK;
 }
@@ -824,11 +872,23 @@ int lbm_set_open_faces(lbm_solver* s, int kind) {
   write_params3(s);
   // which inlet nodes the bulk pass can take, and the inlet list nodes' gathers, depend on the kind
   const int rc = rebuild_flags_impl(s);
-  if (rc != LBM_OK) {  // keep the previous kind with the tables it was built for
+  if (rc != LBM_OK) {
+    // Back to the previous kind: the flag kernel has rewritten the device flags already, so its
+    // tables are rebuilt too (the geometry is unchanged, so nothing is refilled). Should that fail
+    // as well, the handle refuses to step until a rebuild succeeds.
+    const std::string msg = g_err;
     s->faces = before;
     write_params3(s);
+    const int rc2 = rebuild_flags_impl(s);
+    return fail(rc, msg + (rc2 == LBM_OK ? "; the previous faces are kept" : "; rebuilding the previous faces failed too: " + g_err));
   }
-  return rc;
+  return LBM_OK;
+}
+
+int lbm_debug_fail_rebuilds(lbm_solver* s, int n) {
+  if (!s || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  s->debug_fail_rebuilds = n;
+  return LBM_OK;
 }
 
 namespace {
@@ -930,6 +990,7 @@ int lbm_reset_forces(lbm_solver* s) {
 
 int lbm_enqueue_steps(lbm_solver* s, int64_t n) {
   if (!s || n < 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (int rc = check_usable(s)) return rc;
   for (int64_t k = 0; k < n; k++) enqueue_one(s);
   HIP_TRY(hipGetLastError());
   return LBM_OK;
@@ -1089,6 +1150,7 @@ int lbm_read_flags(lbm_solver* s, uint32_t* out, size_t n) {
 
 int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms) {
   if (!s || n < 0 || !elapsed_ms) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (int rc = check_usable(s)) return rc;
   hipEvent_t a, b;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
@@ -1107,6 +1169,7 @@ int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms) {
 
 int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms) {
   if (!s || n < 0 || !total_ms || !step_ms || !max_step_ms) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (int rc = check_usable(s)) return rc;
   std::vector<hipEvent_t> ev(size_t(n + 1));
   for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], s->stream));
@@ -1136,19 +1199,25 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 // for this W (and, in 3D, the three row orders and two store variants) is timed over trial_steps
 // steps (after two warm-up steps) on a copy of the solver's own state, the fastest is kept, and the
 // populations, step, parity and force ring are restored. Results are cached per (lattice,
-// precision, W, H, D) for the life of the process. Block width and row order do not change a result
+// precision, W, H, D, open-face kind, subgrid model on/off, force cadence) for the life of the
+// process. Block width and row order do not change a result
 // (cells are independent within a step and the force sum is ordered by node, not by block); the
 // store variants are separate compilations and may differ in the last bits.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  if (int rc = check_usable(s)) return rc;
   if (s->Q == 27) {  // one launch shape (whole rows of at most 256 threads)
     if (chosen) *chosen = int32_t(step27_block_x(s->W));
     return LBM_OK;
   }
   static std::mutex cache_mu;
   static std::map<std::string, std::pair<int, int>> cache;
+  // Everything that selects a different kernel instantiation or changes the work of a step is part
+  // of the key: lattice, precision, grid, open-face kind, subgrid model on/off, force cadence.
+  const bool smag = (s->nd == 2 ? s->cfg2.smagorinsky : s->cfg3.smagorinsky) > 0;
   const std::string key = std::to_string(s->nd) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
-                          std::to_string(s->H) + "x" + std::to_string(s->D);
+                          std::to_string(s->H) + "x" + std::to_string(s->D) + ":faces" + std::to_string(s->faces) +
+                          ":smag" + std::to_string(int(smag)) + ":every" + std::to_string(s->force_every());
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     auto it = cache.find(key);
@@ -1159,30 +1228,68 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       return LBM_OK;
     }
   }
+  // Trials run on the solver's own buffers and sample forces as it is configured to (the sampling
+  // changes which launch shape wins). The populations, the force ring and its counter are saved
+  // first; the guard restores them, the step, parity and pending sample steps, and frees the
+  // copies, whether the trials finish or a HIP call fails part way.
   const size_t fbytes = size_t(s->Q) * s->N * s->elem;
-  void* save[2] = {nullptr, nullptr};
-  HIP_TRY(hipMalloc(&save[0], fbytes));
-  HIP_TRY(hipMalloc(&save[1], fbytes));
-  HIP_TRY(hipMemcpyAsync(save[0], s->f[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(save[1], s->f[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
-  const int64_t step0 = s->step;
-  const int parity0 = s->parity;
-  const size_t nsteps = s->force_steps.size();
-  // Trials sample forces as the solver is configured to (the sampling changes which launch shape
-  // wins); the force ring and its counter are saved here and restored afterwards.
   const size_t ring_bytes = sizeof(double) * 3 * HISTORY_LEN;
-  void* save_ring = nullptr;
-  void* save_state = nullptr;
-  HIP_TRY(hipMalloc(&save_ring, ring_bytes));
-  HIP_TRY(hipMalloc(&save_state, 4));
-  HIP_TRY(hipMemcpyAsync(save_ring, s->ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(save_state, s->state, 4, hipMemcpyDeviceToDevice, s->stream));
+  struct Guard {
+    lbm_solver* s;
+    size_t fbytes, ring_bytes;
+    void* save[2] = {nullptr, nullptr};
+    void* ring = nullptr;
+    void* state = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    int64_t step0;
+    int parity0, block0, sweep0;
+    size_t nsteps;
+    bool saved = false;  // every copy was enqueued: from here on the state must be restored
+    bool keep_shape = false;
+    int restore() {
+      saved = false;
+      HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->ring, ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->state, state, 4, hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipStreamSynchronize(s->stream));
+      return LBM_OK;
+    }
+    ~Guard() {
+      if (saved) (void)restore();
+      (void)hipStreamSynchronize(s->stream);
+      for (void* p : {save[0], save[1], ring, state})
+        if (p) (void)hipFree(p);
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+      s->step = step0;
+      s->parity = parity0;
+      s->force_steps.resize(std::min(nsteps, s->force_steps.size()));
+      if (!keep_shape) {
+        s->bulk_block = block0;
+        s->sweep = sweep0;
+      }
+    }
+  } g{s, fbytes, ring_bytes};
+  g.step0 = s->step;
+  g.parity0 = s->parity;
+  g.block0 = s->bulk_block;
+  g.sweep0 = s->sweep;
+  g.nsteps = s->force_steps.size();
+  HIP_TRY(hipMalloc(&g.save[0], fbytes));
+  HIP_TRY(hipMalloc(&g.save[1], fbytes));
+  HIP_TRY(hipMalloc(&g.ring, ring_bytes));
+  HIP_TRY(hipMalloc(&g.state, 4));
+  HIP_TRY(hipEventCreate(&g.a));
+  HIP_TRY(hipEventCreate(&g.b));
+  HIP_TRY(hipMemcpyAsync(g.save[0], s->f[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(g.save[1], s->f[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(g.ring, s->ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(g.state, s->state, 4, hipMemcpyDeviceToDevice, s->stream));
+  g.saved = true;
   std::vector<int> tried;
   int best = s->bulk_block, best_sweep = s->sweep;
   double best_ms = 1e300;
-  hipEvent_t a, b;
-  HIP_TRY(hipEventCreate(&a));
-  HIP_TRY(hipEventCreate(&b));
   // every candidate is timed twice, interleaved with the others, and keeps its faster time
   std::vector<std::pair<int, int>> cand;
   for (int req = 64; req <= 1024; req += 64) {
@@ -1203,12 +1310,13 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       s->sweep = cand[c].second;
       enqueue_one(s);
       enqueue_one(s);
-      HIP_TRY(hipEventRecord(a, s->stream));
+      HIP_TRY(hipEventRecord(g.a, s->stream));
       for (int k = 0; k < trial_steps; k++) enqueue_one(s);
-      HIP_TRY(hipEventRecord(b, s->stream));
-      HIP_TRY(hipEventSynchronize(b));
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(g.b, s->stream));
+      HIP_TRY(hipEventSynchronize(g.b));
       float ms = 0;
-      HIP_TRY(hipEventElapsedTime(&ms, a, b));
+      HIP_TRY(hipEventElapsedTime(&ms, g.a, g.b));
       best_of[c] = std::min(best_of[c], double(ms));
     }
   }
@@ -1219,22 +1327,10 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       best_sweep = cand[c].second;
     }
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(s->ring, save_ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(s->state, save_state, 4, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  (void)hipFree(save[0]);
-  (void)hipFree(save[1]);
-  (void)hipFree(save_ring);
-  (void)hipFree(save_state);
-  s->step = step0;
-  s->parity = parity0;
-  s->force_steps.resize(nsteps);
+  if (int rc = g.restore()) return rc;
   s->bulk_block = best;
   s->sweep = best_sweep;
+  g.keep_shape = true;
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     cache[key] = {best, best_sweep};

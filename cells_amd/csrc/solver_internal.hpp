@@ -95,6 +95,11 @@ struct lbm_solver {
   std::vector<uint32_t> h_link_node;  // cell of each Bouzidi record
   std::vector<uint8_t> h_link_dir;    // direction of each Bouzidi record
   uint64_t tables_gen = 0;
+  // Non-empty after a geometry rebuild failed part way (the flag kernel had already rewritten the
+  // device flags, so they no longer match the boundary tables): stepping is refused with this
+  // message until a rebuild succeeds.
+  std::string invalid;
+  int debug_fail_rebuilds = 0;  // test hook (lbm_debug_fail_rebuilds)
   struct lbm_adjoint* adj = nullptr;  // discrete-adjoint state, created on first use
   int64_t n_bulk = 0, n_solid = 0;
   double* partials = nullptr;
@@ -122,6 +127,8 @@ namespace lbm {
 // Helpers of capi.hip used by the batch driver and the adjoint.
 int create_2d_on(const lbm_config2d* cfg, int device, hipStream_t stream, lbm_solver** out);
 int rebuild_flags(lbm_solver* s);
+// LBM_OK when the solver's tables match its flags, else the error of the failed rebuild.
+int check_usable(const lbm_solver* s);
 void record_steps(lbm_solver* s, int64_t n);
 // One step on the solver's stream: force sample as configured (sample < 0) or forced on / off
 // (1 / 0); with rec, the step also writes the adjoint's record.

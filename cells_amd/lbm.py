@@ -146,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_update_3d": ([P, C.POINTER(Config3D)], C.c_int),
         "lbm_set_inlet_ramp": ([P, C.c_int, C.c_double, C.c_double], C.c_int),
         "lbm_set_open_faces": ([P, C.c_int], C.c_int),
+        "lbm_debug_fail_rebuilds": ([P, C.c_int], C.c_int),
         "lbm_set_sdf": ([P, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_rebuild_flags": ([P], C.c_int),
         "lbm_brush": ([P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int], C.c_int),
@@ -246,6 +247,10 @@ class _Base:
     def rebuild_flags(self):
         """rebuildFlags (src/solver.ts:205-217): flags, boundary tables and refill from the current SDF."""
         _check(_lib.lbm_rebuild_flags(self._h))
+
+    def debug_fail_rebuilds(self, n: int):
+        """Test hook: the next n boundary-table rebuilds fail as a device allocation failure would."""
+        _check(_lib.lbm_debug_fail_rebuilds(self._h, int(n)))
 
     rebuildFlags = rebuild_flags
 

@@ -131,6 +131,15 @@ int lbm_set_inlet_ramp(lbm_solver* s, int kind, double u_target, double steps);
  * the outlet node rho = 1 with no transverse velocity, and both nodes then collide. Takes effect
  * from the next enqueued step; rebuilds the boundary tables. The adjoint accepts only the default. */
 int lbm_set_open_faces(lbm_solver* s, int kind);
+/* If a geometry rebuild (lbm_set_sdf, lbm_rebuild_flags, lbm_brush*, lbm_set_open_faces, a 2D mode
+ * change in lbm_update_2d) fails after the flag kernel has rewritten the device flags, the handle
+ * keeps its previous boundary tables but refuses to step (LBM_ERR_INVALID) until a later rebuild
+ * succeeds; lbm_set_open_faces rebuilds with the previous kind by itself. */
+
+/* No counterpart in the reference: a test hook. The next n boundary-table rebuilds of this handle
+ * fail where a device allocation of the new tables would (LBM_ERR_ALLOC): after the flag kernel ran,
+ * before the handle's tables change. */
+int lbm_debug_fail_rebuilds(lbm_solver* s, int n);
 
 /* setSdf (src/solver.ts:200-203, src/solver3d.ts:211-214): n = N floats, negative inside solids.
  * Rebuilds flags, the boundary-cell list (in cell order) and refills cells the body uncovered. */
@@ -201,8 +210,9 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 /* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
  * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid (in 3D, each with
  * the three row orders and the two store variants of lbm_launch_info) on a copy of the solver's
- * state, with force samples as configured, keeps the fastest (cached per lattice, precision and grid
- * size for the process), and restores populations, step and force ring. chosen (may be NULL)
+ * state, with force samples as configured, keeps the fastest (cached for the process per lattice,
+ * precision, grid size, open-face kind, subgrid model on/off and force cadence), and restores
+ * populations, step and force ring (also when it fails part way). chosen (may be NULL)
  * receives the width. Block width and row order never change a result; the two store variants agree
  * to floating-point rounding (their compiled code may fuse different multiply-adds). */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
