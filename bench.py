@@ -212,30 +212,24 @@ def cpu_baseline(W, H, D, budget_s=12.0):
 
 
 def sweep_bench(steps):
-    """The 64-tunnel NACA0012 sweep: one batch vs the same tunnels as sequential plain runs."""
-    import numpy as np
-
+    """The 64-tunnel NACA0012 sweep: one batch (autotuned launch shape) vs the same tunnels as
+    sequential plain runs, compared with divergence taken into account (cells_amd.sweep.compare_runs)."""
     from cells_amd import sweep
 
     cases = sweep.sweep_cases(64, seed=0)
-    tb, rb = sweep.run_batched(cases, steps)
-    ts, rs = sweep.run_sequential(cases, steps)
-    worst = 0.0
-    for c, (sb, vb), (ss, vs) in zip(cases, rb, rs):
-        cd_b, cl_b = sweep.coefficients(sb, vb, c["chord"], min(2000, steps // 2))
-        cd_s, cl_s = sweep.coefficients(ss, vs, c["chord"], min(2000, steps // 2))
-        worst = max(worst, abs(cd_b - cd_s) / max(abs(cd_s), 1e-12), abs(cl_b - cl_s) / max(abs(cl_s), 1e-3))
-    bad = []
-    for i, (c, (sb, vb)) in enumerate(zip(cases, rb)):
-        ok = np.isfinite(vb).all(axis=1)
-        if not ok.all():
-            bad.append(dict(tunnel=i, nx=c["config"]["width"], alpha_deg=round(c["alpha_deg"], 3),
-                            re=round(c["re"], 1), tau=c["config"]["tau"], first_bad_step=int(sb[~ok][0])))
+    ib, iseq = {}, {}
+    tb, rb = sweep.run_batched(cases, steps, info=ib)
+    ts, rs = sweep.run_sequential(cases, steps, info=iseq)
+    cmp = sweep.compare_runs(cases, rb, rs, min(2000, steps // 2))
+    # the device-side divergence records agree with the traces and between the two runs
+    cmp["device_divergence_records_agree"] = bool(ib["diverged_at"] == iseq["diverged_at"] and all(
+        d["first_bad_step"] == ib["diverged_at"][d["tunnel"]] for d in cmp["diverged"]) and sum(
+        v != 0 for v in ib["diverged_at"]) == len(cmp["diverged"]))
     bm, sm = sweep.mlups(cases, steps, tb), sweep.mlups(cases, steps, ts)
-    return dict(tunnels=len(cases), cells=sum(c["config"]["width"] * c["config"]["height"] for c in cases),
-                steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
-                batched_s=tb, sequential_s=ts, mean_cd_cl_max_rel_diff=worst,
-                finite=not bad, nonfinite_tunnels=bad)
+    cells = sum(c["config"]["width"] * c["config"]["height"] for c in cases)
+    return dict(tunnels=len(cases), cells=cells, steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
+                batched_s=tb, sequential_s=ts, launch=dict(tile=ib["tile"], order=ib["order"]),
+                batched_algorithmic_gbs=bm * 76e-3, batched_roofline_frac=bm * 76e-3 / HBM_PEAK_GBS, **cmp)
 
 
 def adjoint2d_case(fd=True):

@@ -5,7 +5,9 @@
 // single-tunnel solver (capi.hip) created on the batch's stream, so setSdf / initField / readMacro
 // / readForces keep their per-tunnel semantics; only the stepping is shared. One launch covers every
 // tunnel through a flat block table: blocks [0, n_bulk) are bulk tiles of 256 consecutive cells of
-// one tunnel, then the link tiles and the body tiles of the tunnels' boundary lists, 256 nodes each.
+// one tunnel, then the link tiles and the body tiles of the tunnels' boundary lists. The tile width
+// (64 to 512 cells or nodes, one thread each) and the order of the tunnels in the table are picked
+// by timing (lbm_batch_autotune); neither changes a stored bit.
 // Every tile kind runs the same per-cell code as the single-tunnel kernels (d2q9.hpp), and the body
 // tiles of each tunnel reduce that tunnel's momentum-exchange force into its own FP64 ring in the
 // same block order, so a batched tunnel computes exactly what it would alone. Per-tunnel parameters, the inlet ramp and the force cadence are read
@@ -49,8 +51,10 @@ __device__ __forceinline__ uint32_t div_w(uint32_t n, uint32_t m, uint32_t l) {
 }
 
 // Block table layout: bulk tiles of every tunnel, then link tiles of every tunnel, then body tiles
-// of every tunnel (each tunnel has at least one body tile, which writes its force samples).
-__global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict__ T,
+// of every tunnel (each tunnel has at least one body tile, which writes its force samples); tiles of
+// TILE cells or list nodes, one thread each.
+template <uint32_t TILE>
+__global__ __launch_bounds__(TILE) void batch2_step(const TunnelDesc* __restrict__ T,
                                                    const uint16_t* __restrict__ block_tunnel, uint32_t n_bulk_blocks,
                                                    uint32_t link_end, int parity, const uint64_t* __restrict__ base_step,
                                                    uint32_t k) {
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict_
   const float* src = d.f[parity];
   float* dst = d.f[parity ^ 1];
   if (b < n_bulk_blocks) {
-    const uint32_t idx = (b - d.bulk_block0) * 256u + threadIdx.x;
+    const uint32_t idx = (b - d.bulk_block0) * TILE + threadIdx.x;
     if (idx >= p.N) return;
     const uint32_t y = div_w(idx, d.magic, d.shift);
     const uint32_t x = idx - y * uint32_t(p.W);
@@ -75,17 +79,17 @@ __global__ __launch_bounds__(256) void batch2_step(const TunnelDesc* __restrict_
     return;
   }
   if (b < link_end) {
-    const uint32_t kl = (b - d.link_block0) * 256u + threadIdx.x;
+    const uint32_t kl = (b - d.link_block0) * TILE + threadIdx.x;
     float none[2] = {0.0f, 0.0f};
     if (kl < d.lt.n_link) list_cell2<false, false>(src, dst, d.flags, d.xtab, d.lt, kl, p, Record{nullptr, nullptr}, none);
     return;
   }
   const uint32_t tile = b - d.body_block0;
-  const uint32_t kk = tile * 256u + threadIdx.x;
+  const uint32_t kk = tile * TILE + threadIdx.x;
   float force[2] = {0.0f, 0.0f};
   if (kk < d.lt.n_body) list_cell2<true, false>(src, dst, d.flags, d.xtab, d.lt, d.lt.n_link + kk, p, Record{nullptr, nullptr}, force);
   if (d.force_every > 0 && (step + 1) % uint64_t(d.force_every) == 0) {
-    ForceSink fs{d.partials, d.counter, d.ring, d.state, 1};
+    ForceSink fs{d.partials, d.counter, d.ring, d.state, 1, step + 1};
     force_sample<2>(force, fs, tile, d.n_body_blocks);
   }
 }
@@ -110,6 +114,8 @@ struct lbm_batch {
   int ramp_kind = 0;
   double ramp_steps = 0;
   bool dirty = true;
+  uint32_t tile = 256;  // cells (bulk) or list nodes per block
+  int order = 0;        // 0: tunnels in creation order; 1: largest tunnel first
   static constexpr int GRAPH_STEPS = 32;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
@@ -131,13 +137,18 @@ void drop_graphs(lbm_batch* b) {
     }
 }
 
-// Rebuilds the descriptor and block tables (after geometry, parameter or ramp changes).
+// Rebuilds the descriptor and block tables (after geometry, parameter, ramp or launch-shape changes).
 int refresh(lbm_batch* b) {
   if (!b->dirty) return LBM_OK;
   const size_t n = b->t.size();
+  const uint32_t T = b->tile;
+  std::vector<size_t> perm(n);
+  for (size_t i = 0; i < n; i++) perm[i] = i;
+  if (b->order == 1)
+    std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t c) { return b->t[a]->N > b->t[c]->N; });
   b->desc.assign(n, TunnelDesc{});
   uint32_t blocks = 0;
-  for (size_t i = 0; i < n; i++) {
+  for (size_t i : perm) {
     lbm_solver* s = b->t[i];
     TunnelDesc& d = b->desc[i];
     d.p = s->p2;
@@ -156,21 +167,21 @@ int refresh(lbm_batch* b) {
     d.u_target = s->cfg2.u_in;
     d.ramp_steps = b->ramp_steps;
     d.bulk_block0 = blocks;
-    blocks += (s->N + 255) / 256;
+    blocks += (s->N + T - 1) / T;
   }
   if (blocks != b->n_bulk_blocks) drop_graphs(b);
   b->n_bulk_blocks = blocks;
-  for (size_t i = 0; i < n; i++) {
+  for (size_t i : perm) {
     TunnelDesc& d = b->desc[i];
     d.link_block0 = blocks;
-    blocks += list_blocks(d.lt.n_link, 256);
+    blocks += list_blocks(d.lt.n_link, T);
   }
   if (blocks != b->link_end) drop_graphs(b);
   b->link_end = blocks;
-  for (size_t i = 0; i < n; i++) {
+  for (size_t i : perm) {
     TunnelDesc& d = b->desc[i];
     d.body_block0 = blocks;
-    d.n_body_blocks = body_list_blocks(d.lt.n_body, 256, true);
+    d.n_body_blocks = body_list_blocks(d.lt.n_body, T, true);
     blocks += d.n_body_blocks;
   }
   if (blocks > 0x7fffffffu) return fail(LBM_ERR_INVALID, "batch too large for one launch");
@@ -179,12 +190,14 @@ int refresh(lbm_batch* b) {
   b->block_tunnel.assign(blocks, 0);
   for (size_t i = 0; i < n; i++) {
     const TunnelDesc& d = b->desc[i];
-    for (uint32_t k = d.bulk_block0; k < d.bulk_block0 + (b->t[i]->N + 255) / 256; k++) b->block_tunnel[k] = uint16_t(i);
-    for (uint32_t k = d.link_block0; k < d.link_block0 + list_blocks(d.lt.n_link, 256); k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.bulk_block0; k < d.bulk_block0 + (b->t[i]->N + T - 1) / T; k++) b->block_tunnel[k] = uint16_t(i);
+    for (uint32_t k = d.link_block0; k < d.link_block0 + list_blocks(d.lt.n_link, T); k++) b->block_tunnel[k] = uint16_t(i);
     for (uint32_t k = d.body_block0; k < d.body_block0 + d.n_body_blocks; k++) b->block_tunnel[k] = uint16_t(i);
   }
   if (blocks > b->block_cap) {
     if (b->d_block_tunnel) HIP_TRY(hipFree(b->d_block_tunnel));
+    b->d_block_tunnel = nullptr;
+    b->block_cap = 0;
     HIP_TRY(hipMalloc(&b->d_block_tunnel, sizeof(uint16_t) * blocks));
     b->block_cap = blocks;
     drop_graphs(b);  // the captured kernels hold the old table pointer
@@ -198,8 +211,16 @@ int refresh(lbm_batch* b) {
 }
 
 void launch_step(lbm_batch* b, int parity, uint32_t k) {
-  batch2_step<<<b->n_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_block_tunnel, b->n_bulk_blocks, b->link_end, parity,
-                                                  b->d_step, k);
+#define LBM_BATCH_LAUNCH(T)                                                                                      \
+  batch2_step<T><<<b->n_blocks, T, 0, b->stream>>>(b->d_desc, b->d_block_tunnel, b->n_bulk_blocks, b->link_end, \
+                                                   parity, b->d_step, k)
+  switch (b->tile) {
+    case 64: LBM_BATCH_LAUNCH(64); break;
+    case 128: LBM_BATCH_LAUNCH(128); break;
+    case 512: LBM_BATCH_LAUNCH(512); break;
+    default: LBM_BATCH_LAUNCH(256); break;
+  }
+#undef LBM_BATCH_LAUNCH
 }
 
 // Graph of GRAPH_STEPS steps starting at the given parity, reading the step from d_step + k.
@@ -342,6 +363,144 @@ int lbm_batch_reset_clock(lbm_batch* b) {
   b->step = 0;
   b->parity = 0;
   b->dirty = true;
+  return LBM_OK;
+}
+
+int lbm_batch_set_launch(lbm_batch* b, int32_t tile, int32_t order) {
+  if (!b) return fail(LBM_ERR_INVALID, "null batch");
+  if (tile != 64 && tile != 128 && tile != 256 && tile != 512) return fail(LBM_ERR_INVALID, "tile must be 64, 128, 256 or 512");
+  if (order != 0 && order != 1) return fail(LBM_ERR_INVALID, "order must be 0 or 1");
+  if (uint32_t(tile) != b->tile || order != b->order) {
+    b->tile = uint32_t(tile);
+    b->order = order;
+    b->dirty = true;
+    drop_graphs(b);
+  }
+  return LBM_OK;
+}
+
+int lbm_batch_launch_info(lbm_batch* b, int32_t* tile, int32_t* order) {
+  if (!b) return fail(LBM_ERR_INVALID, "null batch");
+  if (tile) *tile = int32_t(b->tile);
+  if (order) *order = b->order;
+  return LBM_OK;
+}
+
+// Times every tile width with both tunnel orders over trial_steps launches (after two untimed ones;
+// every candidate twice, interleaved, keeping its faster time) on the tunnels' own state, keeps the
+// fastest, and restores every tunnel's populations and force ring (the batch's and tunnels' step
+// counters are not advanced). Neither the tile width nor the order changes a result: cells are
+// independent within a step and each force sample is summed per wave of body nodes in node order.
+int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t* order) {
+  if (!b || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
+  for (lbm_solver* s : b->t)
+    if (int rc = check_usable(s)) return rc;
+  struct Saved {
+    void* f[2] = {nullptr, nullptr};
+    void* ring = nullptr;
+    void* state = nullptr;
+  };
+  struct Guard {
+    lbm_batch* b;
+    std::vector<Saved> sv;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint32_t tile0;
+    int order0;
+    bool saved = false, keep = false;
+    int restore() {
+      saved = false;
+      for (size_t i = 0; i < b->t.size(); i++) {
+        lbm_solver* s = b->t[i];
+        const size_t fb = size_t(s->Q) * s->N * s->elem;
+        HIP_TRY(hipMemcpyAsync(s->f[0], sv[i].f[0], fb, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(s->f[1], sv[i].f[1], fb, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(s->ring, sv[i].ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(s->state, sv[i].state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, b->stream));
+      }
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      return LBM_OK;
+    }
+    ~Guard() {
+      if (saved) (void)restore();
+      (void)hipStreamSynchronize(b->stream);
+      for (Saved& x : sv)
+        for (void* p : {x.f[0], x.f[1], x.ring, x.state})
+          if (p) (void)hipFree(p);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (!keep && (b->tile != tile0 || b->order != order0)) {
+        b->tile = tile0;
+        b->order = order0;
+        b->dirty = true;
+        drop_graphs(b);
+      }
+    }
+  } g{b, std::vector<Saved>(b->t.size())};
+  g.tile0 = b->tile;
+  g.order0 = b->order;
+  for (size_t i = 0; i < b->t.size(); i++) {
+    lbm_solver* s = b->t[i];
+    const size_t fb = size_t(s->Q) * s->N * s->elem;
+    HIP_TRY(hipMalloc(&g.sv[i].f[0], fb));
+    HIP_TRY(hipMalloc(&g.sv[i].f[1], fb));
+    HIP_TRY(hipMalloc(&g.sv[i].ring, sizeof(double) * 3 * HISTORY_LEN));
+    HIP_TRY(hipMalloc(&g.sv[i].state, 4 * STATE_WORDS));
+  }
+  HIP_TRY(hipEventCreate(&g.e0));
+  HIP_TRY(hipEventCreate(&g.e1));
+  for (size_t i = 0; i < b->t.size(); i++) {
+    lbm_solver* s = b->t[i];
+    const size_t fb = size_t(s->Q) * s->N * s->elem;
+    HIP_TRY(hipMemcpyAsync(g.sv[i].f[0], s->f[0], fb, hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(g.sv[i].f[1], s->f[1], fb, hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(g.sv[i].ring, s->ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(g.sv[i].state, s->state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, b->stream));
+  }
+  g.saved = true;
+  const std::pair<uint32_t, int> cand[] = {{64, 0}, {128, 0}, {256, 0}, {512, 0}, {64, 1}, {128, 1}, {256, 1}, {512, 1}};
+  const size_t nc = sizeof(cand) / sizeof(cand[0]);
+  std::vector<double> best_of(nc, 1e300);
+  for (int rep = 0; rep < 2; rep++) {
+    for (size_t c = 0; c < nc; c++) {
+      b->tile = cand[c].first;
+      b->order = cand[c].second;
+      b->dirty = true;
+      if (int rc = refresh(b)) return rc;
+      set_u64_kernel<<<1, 1, 0, b->stream>>>(b->d_step, uint64_t(b->step));
+      launch_step(b, b->parity, 0);
+      launch_step(b, b->parity ^ 1, 1);
+      HIP_TRY(hipEventRecord(g.e0, b->stream));
+      for (int k = 0; k < trial_steps; k++) launch_step(b, b->parity ^ (k & 1), uint32_t(2 + k));
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipEventRecord(g.e1, b->stream));
+      HIP_TRY(hipEventSynchronize(g.e1));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, g.e0, g.e1));
+      best_of[c] = std::min(best_of[c], double(ms));
+    }
+  }
+  size_t best = 0;
+  for (size_t c = 1; c < nc; c++)
+    if (best_of[c] < best_of[best]) best = c;
+  if (int rc = g.restore()) return rc;
+  b->tile = cand[best].first;
+  b->order = cand[best].second;
+  b->dirty = true;
+  drop_graphs(b);
+  g.keep = true;
+  if (int rc = refresh(b)) return rc;
+  if (tile) *tile = int32_t(b->tile);
+  if (order) *order = b->order;
+  return LBM_OK;
+}
+
+int lbm_batch_diverged(lbm_batch* b, int64_t* first_steps, size_t n) {
+  if (!b || !first_steps || n < b->t.size()) return fail(LBM_ERR_INVALID, "first_steps must hold one value per tunnel");
+  std::vector<uint64_t> v(b->t.size(), 0);
+  for (size_t i = 0; i < b->t.size(); i++)
+    HIP_TRY(hipMemcpyAsync(&v[i], b->t[i]->state + 2, sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->t.size(); i++) first_steps[i] = int64_t(v[i]);
   return LBM_OK;
 }
 

@@ -576,7 +576,7 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMalloc(&s->partials, sizeof(double) * 3 * maxwaves));
   HIP_TRY(hipMalloc(&s->counter, 4));
   HIP_TRY(hipMalloc(&s->ring, sizeof(double) * 3 * HISTORY_LEN));
-  HIP_TRY(hipMalloc(&s->state, 4));
+  HIP_TRY(hipMalloc(&s->state, 4 * STATE_WORDS));
   HIP_TRY(hipMalloc(&s->xtab, sizeof(float4) * size_t(s->W)));
   if (s->nd == 3) {
     HIP_TRY(hipMalloc(&s->row_clean, size_t(s->H) * size_t(s->D)));
@@ -587,7 +587,7 @@ int alloc_common(lbm_solver* s) {
   HIP_TRY(hipMemsetAsync(s->flags, 0, 4 * N, s->stream));
   HIP_TRY(hipMemsetAsync(s->mac, 0, 16 * N, s->stream));
   HIP_TRY(hipMemsetAsync(s->counter, 0, 4, s->stream));
-  HIP_TRY(hipMemsetAsync(s->state, 0, 4, s->stream));
+  HIP_TRY(hipMemsetAsync(s->state, 0, 4 * STATE_WORDS, s->stream));
   HIP_TRY(hipMemsetAsync(s->ring, 0, sizeof(double) * 3 * HISTORY_LEN, s->stream));
   std::vector<float> far(N, 1e6f);  // src/solver.ts:133, src/solver3d.ts:142 (FAR)
   s->sdf_host = far;
@@ -620,7 +620,7 @@ int check_usable(const lbm_solver* s) {
 
 int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
   const int every = s->force_every();
-  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0};
+  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0, uint64_t(s->step + 1)};
   fs.sample = sample >= 0 ? sample : ((every > 0 && (s->step + 1) % every == 0) ? 1 : 0);
   void* src = s->f[s->parity];
   void* dst = s->f[s->parity ^ 1];
@@ -1111,6 +1111,15 @@ int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* v
   return rc;
 }
 
+int lbm_diverged(lbm_solver* s, int64_t* first_step) {
+  if (!s || !first_step) return fail(LBM_ERR_INVALID, "bad arguments");
+  uint64_t v = 0;
+  HIP_TRY(hipMemcpyAsync(&v, s->state + 2, sizeof(v), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  *first_step = int64_t(v);
+  return LBM_OK;
+}
+
 int lbm_read_forces(lbm_solver* s, int64_t* steps, double* values, int64_t cap, int64_t* count) {
   int64_t t = 0;
   int rc = lbm_read_forces_begin(s, &t);
@@ -1251,7 +1260,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipMemcpyAsync(s->ring, ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
-      HIP_TRY(hipMemcpyAsync(s->state, state, 4, hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMemcpyAsync(s->state, state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipStreamSynchronize(s->stream));
       return LBM_OK;
     }
@@ -1279,13 +1288,13 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   HIP_TRY(hipMalloc(&g.save[0], fbytes));
   HIP_TRY(hipMalloc(&g.save[1], fbytes));
   HIP_TRY(hipMalloc(&g.ring, ring_bytes));
-  HIP_TRY(hipMalloc(&g.state, 4));
+  HIP_TRY(hipMalloc(&g.state, 4 * STATE_WORDS));
   HIP_TRY(hipEventCreate(&g.a));
   HIP_TRY(hipEventCreate(&g.b));
   HIP_TRY(hipMemcpyAsync(g.save[0], s->f[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(g.save[1], s->f[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(g.ring, s->ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
-  HIP_TRY(hipMemcpyAsync(g.state, s->state, 4, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(g.state, s->state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, s->stream));
   g.saved = true;
   std::vector<int> tried;
   int best = s->bulk_block, best_sweep = s->sweep;

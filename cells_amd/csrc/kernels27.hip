@@ -303,7 +303,8 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
 // the wave, appended to the history ring (src/shaders/aux3d.ts:155-183's ring semantics).
 __global__ __launch_bounds__(64) void reduce27_kernel(const double* __restrict__ partials,
                                                       const uint32_t* __restrict__ waves, uint32_t nwaves,
-                                                      double* __restrict__ ring, uint32_t* __restrict__ state) {
+                                                      double* __restrict__ ring, uint32_t* __restrict__ state,
+                                                      uint64_t step) {
   double acc[3] = {0.0, 0.0, 0.0};
   for (uint32_t k = threadIdx.x; k < nwaves; k += 64u)
 #pragma unroll
@@ -314,6 +315,7 @@ __global__ __launch_bounds__(64) void reduce27_kernel(const double* __restrict__
     const uint32_t slot = st % HISTORY_LEN;
 #pragma unroll
     for (int d = 0; d < 3; d++) ring[size_t(slot) * 3 + d] = acc[d];
+    note_divergence<3>(acc, state, step);
     state[0] = st + 1u;
   }
 }
@@ -451,7 +453,7 @@ void step27_t(const void* src, void* dst, const uint32_t* flags, const uint8_t* 
     step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
   else
     step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
-  if (fs.sample) reduce27_kernel<<<1, 64, 0, st>>>(fs.partials, force_waves, n_force_waves, fs.ring, fs.state);
+  if (fs.sample) reduce27_kernel<<<1, 64, 0, st>>>(fs.partials, force_waves, n_force_waves, fs.ring, fs.state, fs.step);
 }
 
 }  // namespace

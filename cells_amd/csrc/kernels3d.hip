@@ -645,7 +645,9 @@ __global__ __launch_bounds__(256) void flags3_kernel(const float* __restrict__ s
   }
 }
 
-__global__ void reset_state_kernel(uint32_t* state) { state[0] = 0u; }
+__global__ void reset_state_kernel(uint32_t* state) {
+  for (int k = 0; k < STATE_WORDS; k++) state[k] = 0u;
+}
 
 // ============================================================================ launchers
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }

@@ -192,15 +192,33 @@ struct Params3 {
 };
 
 // Where the step kernels put a force sample: per-block FP64 partials, a completion counter,
-// the FP64 history ring and its sample counter. sample == 0 skips the reduction.
+// the FP64 history ring and its state words (STATE_WORDS u32: the sample count, then at word 2 the
+// 64-bit number of the first step whose sample was not finite, 0 while none was). sample == 0
+// skips the reduction; step is the number of the step the sample belongs to (1 for the first).
 struct ForceSink {
   double* partials;
   uint32_t* counter;
   double* ring;
   uint32_t* state;
   int32_t sample;
+  uint64_t step;
 };
 
 constexpr int HISTORY_LEN = 8192;
+constexpr int STATE_WORDS = 4;
+
+// The reference's blow-up rule (src/tunnel3d.ts:97-99): a force sample with a non-finite component
+// means the flow has diverged. Records the step of the first such sample; called by the one thread
+// that appends the sample, and a tunnel's samples are appended in stream order, so no atomics.
+template <int DIM>
+__device__ __forceinline__ void note_divergence(const double (&v)[DIM], uint32_t* state, uint64_t step) {
+  bool finite = true;
+#pragma unroll
+  for (int d = 0; d < DIM; d++) finite = finite && __builtin_isfinite(v[d]);
+  if (finite) return;
+  uint64_t* first = reinterpret_cast<uint64_t*>(state + 2);
+  if (__hip_atomic_load(first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    __hip_atomic_store(first, step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 }  // namespace lbm

@@ -65,6 +65,7 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uin
     const uint32_t slot = st % HISTORY_LEN;
 #pragma unroll
     for (int d = 0; d < DIM; d++) fs.ring[size_t(slot) * DIM + d] = acc[d];
+    note_divergence<DIM>(acc, fs.state, fs.step);
     __hip_atomic_store(fs.state, st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(fs.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }

@@ -162,6 +162,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_read_forces_wait": ([P, C.c_int64], C.c_int),
         "lbm_read_forces_end": ([P, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
         "lbm_read_forces": ([P, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_diverged": ([P, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_batch_diverged": ([P, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_info": ([P, C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                       C.POINTER(C.c_int32)], C.c_int),
         "lbm_read_populations": ([P, C.c_void_p, C.c_size_t], C.c_int),
@@ -188,6 +190,9 @@ def load_library(path: str = LIB_PATH):
         "lbm_batch_enqueue_steps": ([P, C.c_int64], C.c_int),
         "lbm_batch_sync": ([P], C.c_int),
         "lbm_batch_time_steps": ([P, C.c_int64, C.POINTER(C.c_double)], C.c_int),
+        "lbm_batch_autotune": ([P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "lbm_batch_launch_info": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+        "lbm_batch_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -292,6 +297,18 @@ class _Base:
         steps, vals = self.read_forces_raw()
         T = ForceSample if self.dim == 2 else ForceSample3
         return [T(int(s), *map(float, v)) for s, v in zip(steps, vals)]
+
+    def diverged_at(self) -> int:
+        """Step of the first non-finite force sample since initField / resetForces, 0 if none
+        (hasDiverged, src/tunnel3d.ts:97-99, kept on the device)."""
+        v = C.c_int64()
+        _check(_lib.lbm_diverged(self._h, C.byref(v)))
+        return int(v.value)
+
+    def has_diverged(self) -> bool:
+        return self.diverged_at() != 0
+
+    hasDiverged = has_diverged
 
     def read_populations(self) -> np.ndarray:
         """Current distribution buffer, decoded to float32 (Q, N) of (f - w)."""
@@ -516,10 +533,31 @@ class Batch:
     def sync(self):
         _check(_lib.lbm_batch_sync(self._h))
 
+    def diverged_at(self) -> np.ndarray:
+        """Per tunnel, the step of its first non-finite force sample (0: none so far)."""
+        out = np.zeros(len(self.tunnels), dtype=np.int64)
+        _check(_lib.lbm_batch_diverged(self._h, _ptr(out), out.size))
+        return out
+
     def time_steps(self, n: int) -> float:
         ms = C.c_double()
         _check(_lib.lbm_batch_time_steps(self._h, int(n), C.byref(ms)))
         return float(ms.value)
+
+    def autotune(self, trial_steps: int = 20):
+        """Times the tile widths and tunnel orders on the current state (restored); returns the
+        chosen (tile, order)."""
+        t, o = C.c_int32(), C.c_int32()
+        _check(_lib.lbm_batch_autotune(self._h, int(trial_steps), C.byref(t), C.byref(o)))
+        return int(t.value), int(o.value)
+
+    def launch_info(self):
+        t, o = C.c_int32(), C.c_int32()
+        _check(_lib.lbm_batch_launch_info(self._h, C.byref(t), C.byref(o)))
+        return int(t.value), int(o.value)
+
+    def set_launch(self, tile: int, order: int = 0):
+        _check(_lib.lbm_batch_set_launch(self._h, int(tile), int(order)))
 
     def destroy(self):
         if getattr(self, "_h", None):

@@ -190,6 +190,13 @@ int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* v
 /* begin + wait + end. */
 int lbm_read_forces(lbm_solver* s, int64_t* steps, double* values, int64_t cap, int64_t* count);
 
+/* hasDiverged (src/tunnel3d.ts:97-99), kept on the device: *first_step receives the number of the
+ * first step whose force sample had a non-finite component since the last lbm_init_field /
+ * lbm_reset_forces, or 0 while every sample was finite. Samples are checked as the step kernels
+ * append them, so a flow that blew up between two reads is caught even if the read is skipped.
+ * Waits for the work enqueued before it. */
+int lbm_diverged(lbm_solver* s, int64_t* first_step);
+
 /* Readonly fields W, H, D, N, step (src/solver.ts:51-57, src/solver3d.ts:43-50). */
 int lbm_info(const lbm_solver* s, int32_t* dims /* W, H, D */, int64_t* n_cells, int64_t* step,
              int32_t* lattice_q, int32_t* precision);
@@ -285,8 +292,20 @@ lbm_solver* lbm_batch_tunnel(lbm_batch* b, int32_t i);
 int lbm_batch_set_inlet_ramp(lbm_batch* b, int kind, double steps);
 int lbm_batch_reset_clock(lbm_batch* b);
 int lbm_batch_enqueue_steps(lbm_batch* b, int64_t n);
+/* lbm_diverged for every tunnel of the batch (first_steps holds n >= the tunnel count values). */
+int lbm_batch_diverged(lbm_batch* b, int64_t* first_steps, size_t n);
 int lbm_batch_sync(lbm_batch* b);
 int lbm_batch_time_steps(lbm_batch* b, int64_t n, double* elapsed_ms);
+/* No counterpart in the reference (its page sweeps workgroup sizes by hand, src/bench.ts:110-118):
+ * times the tile widths 64, 128, 256 and 512 (cells or list nodes per block) with the tunnels in
+ * creation order and largest first, trial_steps launches each on the tunnels' own state, keeps the
+ * fastest and restores every tunnel's populations and force ring; tile and order (may be NULL)
+ * receive the choice. Neither changes a result. */
+int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t* order);
+/* The launch shape in use, and setting it by hand (tile 64/128/256/512; order 0 creation order,
+ * 1 largest tunnel first). */
+int lbm_batch_launch_info(lbm_batch* b, int32_t* tile, int32_t* order);
+int lbm_batch_set_launch(lbm_batch* b, int32_t tile, int32_t order);
 
 #ifdef __cplusplus
 }
