@@ -253,27 +253,46 @@ def adjoint2d_case(fd=True):
     s = Solver.create(dict(cfg))
     sdf0, dsdf = sdf_of(theta)
     s.set_sdf(sdf0)
-    # plain forward: the same run through the ordinary step path
-    s.set_inlet_ramp("linear", u, 3000)
-    s.init_field()
-    s.sync()
-    t0 = time.perf_counter()
-    s.encode_steps(T)
-    s.sync()
-    t_plain = time.perf_counter() - t0
-    adj = ShapeAdjoint(s, T, window, (1.0, 0.0), 2 / (u * u * chord), checkpoint_every=K, ramp=("linear", u, 3000),
-                       record_budget_bytes=16 << 30)
-    adj.forward()
-    info = adj.info()  # builds the reverse-time tables (once per geometry)
-    t0 = time.perf_counter()
-    J = adj.forward()
-    t1 = time.perf_counter()
-    g_sdf = adj.backward()
-    t2 = time.perf_counter()
+
+    def plain_forward():
+        # the same run through the ordinary step path
+        s.set_inlet_ramp("linear", u, 3000)
+        s.init_field()
+        s.sync()
+        t0 = time.perf_counter()
+        s.encode_steps(T)
+        s.sync()
+        return time.perf_counter() - t0
+
+    def timed(budget):
+        """(adjoint forward s, backward s, J, dJ/dsdf, info): the fastest of 3 rounds after one untimed
+        run that builds the reverse-time tables."""
+        adj = ShapeAdjoint(s, T, window, (1.0, 0.0), 2 / (u * u * chord), checkpoint_every=K,
+                           ramp=("linear", u, 3000), record_budget_bytes=budget)
+        adj.forward()
+        adj.info()
+        tf, tb = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            J = adj.forward()
+            t1 = time.perf_counter()
+            g_sdf = adj.backward()
+            tf.append(t1 - t0)
+            tb.append(time.perf_counter() - t1)
+        return min(tf), min(tb), J, g_sdf, adj.info(), adj
+
+    t_plain = min(plain_forward() for _ in range(3))
+    # the configured plan: a checkpoint every 200 steps, every segment recomputed in the backward pass
+    tf0, tb0, J, g_sdf, info, adj = timed(0)
+    # every segment recorded by the forward pass instead (16 B per cell and step; 4.7 GB here)
+    tf1, tb1, J1, g1, info1, _ = timed(16 << 30)
+    assert J1 == J and np.array_equal(g1, g_sdf), "recorded and recomputed segments must agree bit for bit"
     grad = parameter_gradient(g_sdf, dsdf)
     out = dict(grid=[W, H], steps=T, J=J, grad=grad.tolist(), info=info, plain_forward_s=t_plain,
-               adjoint_forward_s=t1 - t0, backward_s=t2 - t1, ratio=(t2 - t0) / t_plain,
-               backward_over_forward=(t2 - t1) / t_plain)
+               adjoint_forward_s=tf0, backward_s=tb0, ratio=(tf0 + tb0) / t_plain,
+               backward_over_forward=tb0 / t_plain, memory_plan="checkpoint every 200 steps, recompute",
+               recorded=dict(info=info1, adjoint_forward_s=tf1, backward_s=tb1, ratio=(tf1 + tb1) / t_plain),
+               timing="fastest of 3 rounds")
     # the FP64 reverse-mode reference of the same case (committed fixture, CPU torch in float64)
     try:
         ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
@@ -283,17 +302,32 @@ def adjoint2d_case(fd=True):
         out["max_rel_err_vs_fp64"] = float(np.max(np.abs(grad - g_ref) / np.abs(g_ref)))
     except (OSError, KeyError, ValueError):
         pass
+    try:  # central differences of the FP64 oracle (tests/golden/make_adjoint_fd_fp64.py)
+        ref = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                          "adjoint_fd_fp64.json")))["naca_384x128"]
+        for h, key in (("1e-3", "fd_h0.001"), ("1e-4", "fd_h0.0001")):
+            fdv = np.array(ref[key])
+            out[f"max_rel_err_vs_fp64_fd_h{h}"] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
+        out["fp64_fd_solid_flips"] = int(sum(ref["flips_h0.001"]) + sum(ref["flips_h0.0001"]))
+    except (OSError, KeyError, ValueError):
+        pass
     if fd:
-        # central differences at h = 1e-3 and, because the smallest components are then within a
-        # few FP32 roundings of J, also at h = 1e-2
+        # central differences of the GPU's FP32 forward at h = 1e-3 and 1e-2 (at 1e-3 the smallest
+        # components are within a few FP32 roundings of J); flips counts the nodes whose solid flag
+        # differs between theta +- h e_p and theta, which would make J jump
+        solid0 = sdf0.reshape(H, W)[:, 1:-1] < 0
         for h, key in ((1e-3, "fd"), (1e-2, "fd_h1e-2")):
-            fdv = []
+            fdv, flips = [], 0
             for p in range(len(theta)):
                 e = np.zeros_like(theta)
                 e[p] = h
-                fdv.append((adj.forward(sdf_of(theta + e)[0]) - adj.forward(sdf_of(theta - e)[0])) / (2 * h))
+                sp, sm = sdf_of(theta + e)[0], sdf_of(theta - e)[0]
+                flips += int(np.count_nonzero((sp.reshape(H, W)[:, 1:-1] < 0) != solid0))
+                flips += int(np.count_nonzero((sm.reshape(H, W)[:, 1:-1] < 0) != solid0))
+                fdv.append((adj.forward(sp) - adj.forward(sm)) / (2 * h))
             fdv = np.array(fdv)
             out[key] = fdv.tolist()
+            out[key + "_solid_flips"] = flips
             out["max_rel_err_vs_" + key] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
             out["max_err_over_max_grad_vs_" + key] = float(np.max(np.abs(grad - fdv)) / np.abs(fdv).max())
     s.destroy()
@@ -342,33 +376,44 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
 
     from cells_amd.lbm import LbmError
 
-    for gb in budget_gb:
-        adj = ShapeAdjoint(s, T, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=K,
-                           ramp=("smoothstep", u, 1000), record_budget_bytes=int(gb * (1 << 30)))
-        try:
-            adj.forward()
-            break
-        except LbmError:
-            adj.release()
-    info = adj.info()
-    # three rounds of (plain forward, adjoint forward, backward), each phase's fastest round kept
-    t_plain, t_fwd, t_bwd = [], [], []
-    for _ in range(3):
-        t_plain.append(plain_forward())
-        t0 = time.perf_counter()
-        J = adj.forward()
-        t1 = time.perf_counter()
-        g_sdf = adj.backward()
-        t2 = time.perf_counter()
-        t_fwd.append(t1 - t0)
-        t_bwd.append(t2 - t1)
-    t_plain, t_fwd, t_bwd = min(t_plain), min(t_fwd), min(t_bwd)
+    def timed(budgets):
+        """The first budget that can be allocated: (budget GB, forward s, backward s, J, dJ/dsdf, info),
+        the fastest of 3 rounds after one untimed run that builds the reverse-time tables."""
+        for gb in budgets:
+            adj = ShapeAdjoint(s, T, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=K,
+                               ramp=("smoothstep", u, 1000), record_budget_bytes=int(gb * (1 << 30)))
+            try:
+                adj.forward()
+                break
+            except LbmError:
+                adj.release()
+        info = adj.info()
+        tf, tb = [], []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            J = adj.forward()
+            t1 = time.perf_counter()
+            g_sdf = adj.backward()
+            tf.append(t1 - t0)
+            tb.append(time.perf_counter() - t1)
+        adj.release()
+        return gb, min(tf), min(tb), J, g_sdf, info
+
+    t_plain = min(plain_forward() for _ in range(3))
+    # the configured plan: a checkpoint every 100 steps (30 x 159 MB in FP16), every segment
+    # recomputed in the backward pass into one recorded segment's worth of scratch (6.7 GB)
+    _, t_fwd, t_bwd, J, g_sdf, info = timed((0,))
+    # every segment recorded by the forward pass instead (16 B per cell and step, 201 GB)
+    gb, tf1, tb1, J1, g1, info1 = timed(budget_gb)
     grad = parameter_gradient(g_sdf, dsdf)
-    adj.release()
     s.destroy()
     out = dict(grid=[W, H, D], steps=T, J=J, grad=grad.tolist(), info=info, plain_forward_s=t_plain,
                adjoint_forward_s=t_fwd, backward_s=t_bwd, ratio=(t_fwd + t_bwd) / t_plain,
-               backward_over_forward=t_bwd / t_plain, record_budget_gb=gb, timing="fastest of 3 rounds")
+               backward_over_forward=t_bwd / t_plain, memory_plan="checkpoint every 100 steps, recompute",
+               recorded=dict(record_budget_gb=gb, info=info1, adjoint_forward_s=tf1, backward_s=tb1,
+                             ratio=(tf1 + tb1) / t_plain,
+                             same_gradient_bits=bool(J1 == J and np.array_equal(g1, g_sdf))),
+               timing="fastest of 3 rounds")
     if fd_coeffs:
         s32 = Solver3D.create(cfg("fp32"))
         a32 = ShapeAdjoint(s32, T, window, (1.0, 0.0, 0.0), 2 / (u * u * area), checkpoint_every=K,

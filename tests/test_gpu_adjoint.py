@@ -105,6 +105,7 @@ def test_adjoint3d_sphere_harmonics_match_finite_differences():
 
     sdf0, dsdf = sdf_of(coeffs)
     grads = {}
+    fd = {}
     for precision in ("fp32", "fp16"):
         s, adj = make(precision)
         J0 = adj.forward(sdf0)
@@ -126,6 +127,26 @@ def test_adjoint3d_sphere_harmonics_match_finite_differences():
         print(f"{precision}: adjoint {[round(g[p], 5) for p in check]}; FD(fp32) {[round(fd[p], 5) for p in check]}")
         for p in check:
             assert abs(g[p] - fd[p]) <= tol * abs(fd[p]) + 2e-3 * scale, (precision, p)
+    # the FP64 reference of the same objective: central differences of the FP64 oracle, all 16
+    # coefficients, h = 1e-4, no node flips between fluid and solid (tests/golden/make_adjoint_fd_fp64.py)
+    ref = _fp64_fd("sphere_48x24x24")
+    assert sum(ref["flips_h0.0001"]) == 0
+    f64 = np.array(ref["fd_h0.0001"])
+    big = np.abs(f64) >= 0.01 * np.abs(f64).max()
+    for precision, tol in (("fp32", 0.01), ("fp16", 0.02)):
+        g = grads[precision]
+        rel = np.abs(g[big] - f64[big]) / np.abs(f64[big])
+        print(f"{precision} adjoint vs FP64 FD: max rel {rel.max():.2e} on {int(big.sum())} components; "
+              f"largest other {np.abs(g[~big]).max():.1e}")
+        assert rel.max() < tol
+        assert np.abs(g[~big]).max() <= 2e-3 * np.abs(f64).max()
+
+
+def _fp64_fd(case):
+    import json
+    import os
+
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adjoint_fd_fp64.json")))[case]
 
 
 def test_autograd_function_semantics():
@@ -161,4 +182,31 @@ def test_adjoint2d_naca_384x128_matches_fp64_fixture():
     print(f"J {out['J']:.9f} ref {ref['J']:.9f}; max relative gradient error {err.max():.2e}")
     assert abs(out["J"] - ref["J"]) <= 1e-4 * abs(ref["J"])
     assert err.max() < 1e-3
+    # and against central differences of the FP64 oracle (h = 1e-4; no node flips at +-h; at h = 1e-3
+    # the difference quotient of a_1 is 2% off its own h -> 0 limit)
+    fd = _fp64_fd("naca_384x128")
+    assert sum(fd["flips_h0.0001"]) == 0 and sum(fd["flips_h0.001"]) == 0
+    rel = np.abs(g - np.array(fd["fd_h0.0001"])) / np.abs(np.array(fd["fd_h0.0001"]))
+    print(f"vs FP64 central differences (h 1e-4): max rel {rel.max():.2e}; J {ref['J']:.12f} (reverse mode) "
+          f"{fd['J']:.12f} (oracle)")
+    assert abs(fd["J"] - ref["J"]) <= 1e-9 * abs(ref["J"])  # two FP64 references of one objective
+    assert rel.max() < 0.01
+    # the configured memory plan (recompute from checkpoints) and the recorded one agree bit for bit
+    assert out["info"]["stored_segments"] == 0 and out["recorded"]["info"]["stored_segments"] == 30
+    print(f"adjoint / forward: {out['ratio']:.2f} (recompute), {out['recorded']['ratio']:.2f} (recorded)")
+
+
+def test_adjoint3d_sphere_256x128x128_fp16():
+    """The 3D shape-gradient case at full size (256 x 128 x 128, FP16-storage forward, FP32 adjoint,
+    3000 steps, checkpoints every 100 steps): recomputed and recorded segments give the same gradient
+    bit for bit, and the gradient of four coefficients that the symmetry leaves non-zero (l = 0, 1, 2) matches
+    central differences of the FP32 forward (h = 1e-3) within 1%."""
+    import bench
+
+    out = bench.adjoint3d_case(fd_coeffs=(0, 3, 6, 8))
+    print({k: out[k] for k in ("ratio", "max_rel_err_vs_fd", "max_rel_err_fp32_adjoint_vs_fd")},
+          out["recorded"]["ratio"])
+    assert out["recorded"]["same_gradient_bits"]
+    assert out["info"]["stored_segments"] == 0 and out["info"]["segments"] == 30
+    assert out["max_rel_err_vs_fd"] < 0.01 and out["max_rel_err_fp32_adjoint_vs_fd"] < 0.01
 
