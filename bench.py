@@ -441,6 +441,35 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
     return out
 
 
+# Single workloads for profiling (bench.py --only NAME): the D3Q19 / D3Q27 tunnels of the bench line
+# at --steps steps, the 64-tunnel sweep at --sweep-steps, the D2Q9 512 x 128 cylinder, the copy
+# roofline kernels, and the two shape-gradient cases without finite differences.
+ONLY = {
+    "fp32_192x96x96": lambda a: run_gpu(192, 96, 96, "fp32", a.steps, a.warmup, profile=False),
+    "fp16_192x96x96": lambda a: run_gpu(192, 96, 96, "fp16", a.steps, a.warmup, profile=False),
+    "fp16sr_192x96x96": lambda a: run_gpu(192, 96, 96, "fp16sr", a.steps, a.warmup, profile=False),
+    "bf16_192x96x96": lambda a: run_gpu(192, 96, 96, "bf16", a.steps, a.warmup, profile=False),
+    "fp32_256x128x128": lambda a: run_gpu(256, 128, 128, "fp32", a.steps, a.warmup, profile=False),
+    "fp16_256x128x128": lambda a: run_gpu(256, 128, 128, "fp16", a.steps, a.warmup, profile=False),
+    "d3q27_fp32_192x96x96": lambda a: run_gpu(192, 96, 96, "fp32", a.steps, a.warmup, profile=False, velocity_set=27),
+    "d3q27_fp16_192x96x96": lambda a: run_gpu(192, 96, 96, "fp16", a.steps, a.warmup, profile=False, velocity_set=27),
+    "d2q9_512x128_cylinder": lambda a: d2q9_config0(a.steps),
+    "sweep_64": lambda a: sweep_bench(a.sweep_steps),
+    "adjoint_naca_384x128": lambda a: adjoint2d_case(fd=False),
+    "adjoint_sphere_256x128x128": lambda a: adjoint3d_case(fd_coeffs=()),
+    "copy_roofline": lambda a: copy_rooflines(),
+}
+
+
+def copy_rooflines():
+    from cells_amd.lbm import copy_roofline
+
+    n1, n4 = 192 * 96 * 96, 256 * 128 * 128
+    return {"fp32_192x96x96_gbs": copy_roofline(n1, 19, 4), "fp16_192x96x96_gbs": copy_roofline(n1, 19, 2),
+            "fp32_256x128x128_gbs": copy_roofline(n4, 19, 4), "fp16_256x128x128_gbs": copy_roofline(n4, 19, 2),
+            "d2q9_fp32_7430144_gbs": copy_roofline(7430144, 9, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -450,11 +479,16 @@ def main():
     ap.add_argument("--quick", action="store_true", help="FP32 192x96x96 only (for profiling)")
     ap.add_argument("--sweep-steps", type=int, default=10000, help="steps of the 64-tunnel sweep (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D and 3D shape-gradient runs")
+    ap.add_argument("--only", choices=sorted(ONLY), help="run one workload and print its own JSON line "
+                    "(for kernel traces and counter passes of that workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
               f"across GPUs; see the sharding design notes).", file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        print(json.dumps({"only": args.only, "result": ONLY[args.only](args)}))
+        return
 
     W, H, D = 192, 96, 96
     main_run = run_gpu(W, H, D, "fp32", args.steps, args.warmup)
@@ -467,10 +501,7 @@ def main():
                                                       args.warmup // 2)
         for prec in ("fp32", "fp16"):  # the D3Q27 velocity set on the same tunnel
             variants[f"d3q27_{prec}_192x96x96"] = run_gpu(W, H, D, prec, args.steps, args.warmup, velocity_set=27)
-    from cells_amd.lbm import copy_roofline
-
-    copy = {"fp32_192x96x96_gbs": copy_roofline(W * H * D, 19, 4), "fp16_192x96x96_gbs": copy_roofline(W * H * D, 19, 2),
-            "fp32_256x128x128_gbs": copy_roofline(256 * 128 * 128, 19, 4)}
+    copy = copy_rooflines()
     sweep_res = sweep_bench(args.sweep_steps) if (args.sweep_steps > 0 and not args.quick) else None
     precision = None if args.quick else precision_study()
     d2q9 = None if args.quick else d2q9_config0()

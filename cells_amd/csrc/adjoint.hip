@@ -767,7 +767,11 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
                  a->rec_dq[slot] + size_t(n % a->K) * std::max<uint32_t>(s->n_links, 1)};
       if (int rc = enqueue_step(s, &rec, 1)) return rc;
     } else {
-      if (int rc = enqueue_step(s, nullptr, 1)) return rc;
+      // The recording kernel with nothing to record: the same compiled arithmetic as the steps that
+      // record (here and when the backward pass recomputes this segment), so J and the recomputed
+      // trajectory are bit for bit those of the recorded plan.
+      const Record none{nullptr, nullptr};
+      if (int rc = enqueue_step(s, &none, 1)) return rc;
     }
     HIP_TRY(hipGetLastError());
     if ((n + 1) % 4096 == 0)

@@ -181,7 +181,7 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   }
   if (inlet) zou_he_inlet(f, y, p);
   if (outlet) zou_he_outlet(f);
-  if (REC) record2(f, rc, idx);
+  if (REC && rc.mom) record2(f, rc, idx);
   const float4 t = xtab[x];
   Relax2 r;
   r.omBase = t.x;
@@ -247,7 +247,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
         }
         force[0] += float(CX2[j]) * (fj + fi);
         force[1] += float(CY2[j]) * (fj + fi);
-        if (REC) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[i], gb[i] != NO_OPERAND);
+        if (REC && rc.dq) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[i], gb[i] != NO_OPERAND);
         f[i] = fi;
       }
     }
@@ -260,7 +260,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
   }
   if (fl & FLAG_INLET2) zou_he_inlet(f, y, p);
   else if (fl & FLAG_OUTLET2) zou_he_outlet(f);
-  if (REC) record2(f, rc, idx);
+  if (REC && rc.mom) record2(f, rc, idx);
   const float4 t = xtab[x];
   Relax2 r;
   r.omBase = t.x;
