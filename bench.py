@@ -403,7 +403,7 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
     # the configured plan: a checkpoint every 100 steps (30 x 159 MB in FP16), every segment
     # recomputed in the backward pass into one recorded segment's worth of scratch (6.7 GB)
     _, t_fwd, t_bwd, J, g_sdf, info = timed((0,))
-    # every segment recorded by the forward pass instead (16 B per cell and step, 201 GB)
+    # every segment recorded by the forward pass instead (8 B per cell and step in FP16 storage, 101 GB)
     gb, tf1, tb1, J1, g1, info1 = timed(budget_gb)
     grad = parameter_gradient(g_sdf, dsdf)
     s.destroy()

@@ -17,7 +17,7 @@
 //   mu_n(y)   = B^T C'(m_n(y))^T lambda(y),
 // where m_n(y) = (drho, u) are the moments of the collision input that the forward step recorded.
 // C' depends on the populations only through these moments, so the record is 16 bytes per cell
-// instead of the Q populations. For a Bouzidi link (x, i) with operands f_j, b:
+// (8 with 16-bit population storage, whose moments are kept as halves) instead of the Q populations. For a Bouzidi link (x, i) with operands f_j, b:
 //   dJ/dq += (d f_i / d q)_n * (mu_n[i](x) + w_n c_j . d),
 // accumulated in FP64 per link; the host chains dq/dsdf at the link's two end nodes.
 //
@@ -61,15 +61,26 @@ struct AdjTables {
 };
 
 struct AdjStep {
-  const float* mu_next;  // mu_{n+1}
+  const float* mu_next;  // mu_{n+1}, population a of cell u at a * Ns + u
+  uint32_t Ns;           // population stride of the adjoint state (mu_stride)
   float* mu;             // mu_n
-  const float4* mom;     // record of step n: moments per cell
+  const void* mom;       // record of step n: moments per cell (float4, or Moments16 when half_mom)
+  uint32_t half_mom;
   const float* dq;       // record of step n: d f_i / d q per link
   double* acc;           // dJ/dq per link
   float w_next, w_cur;   // objective weights of the force samples of steps n+1 and n
   float ut;              // inlet speed of step n
   float dx, dy, dz;      // force direction of the objective
 };
+
+// The record of cell idx as float4 (drho, ux, uy, uz).
+__device__ __forceinline__ float4 get_moments(const AdjStep& as, uint32_t idx) {
+  if (as.half_mom) {
+    const Moments16 h = static_cast<const Moments16*>(as.mom)[idx];
+    return make_float4(float(h.drho), float(h.ux), float(h.uy), float(h.uz));
+  }
+  return static_cast<const float4*>(as.mom)[idx];
+}
 
 // ---------------------------------------------------------------------------- collision VJPs
 // mu = C'^T lambda for the TRT collision at moments (drho, u) with relaxation rates omP (symmetric),
@@ -218,7 +229,7 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
     }
   }
   float mu[Q];
-  const float4 m = as.mom[idx];
+  const float4 m = get_moments(as, idx);
   const float u[3] = {m.y, m.z, Q == 19 ? m.w : 0.0f};
   if (kind == AK_INLET3) {
 #pragma unroll
@@ -257,7 +268,7 @@ __device__ __forceinline__ void adj_list_node(const AdjTables& at, const AdjStep
 // the previous one wrote last, as the forward step does.
 __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, const float4* __restrict__ xtab,
                                                     Params3 p, uint32_t zl, uint32_t descend) {
-  const uint32_t N = p.N;
+  const uint32_t N = as.Ns;  // the adjoint state's population stride
   if (blockIdx.z < zl) {
     const uint32_t k = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < at.n_list) adj_list_node<19>(at, as, xtab, uint32_t(p.W), N, k, as.ut);
@@ -299,12 +310,33 @@ __global__ __launch_bounds__(1024) void adj3_kernel(AdjTables at, AdjStep as, co
     const char* base = mb + (ptrdiff_t(pa) * ptrdiff_t(N) + delta) * 4;
     lam[a] = *reinterpret_cast<const float*>(base + off);
   }
-  const float4 m = as.mom[idx];
+  // The x planes next to the open ends (bulk_readers3 on the host admits a node only if its
+  // populations are read exactly so): at x = 1 nothing reads the populations with c_x < 0 (the
+  // inlet plane pulls nothing); at x = W - 1 nothing reads those with c_x > 0 (off the grid); at
+  // x = W - 2 the outlet node of the same row also reads those with c_x < 0 (its copy rule,
+  // src/shaders/step3d.ts:130), in the same direction.
+  const uint32_t W = uint32_t(p.W);
+#pragma unroll
+  for (int a = 1; a < 19; a++)
+    if ((x == 1u && CX3[a] < 0) || (x == W - 1u && CX3[a] > 0)) lam[a] = 0.0f;
+  if (x == W - 2u) {
+#pragma unroll
+    for (int a = 1; a < 19; a++)
+      if (CX3[a] < 0) lam[a] += as.mu_next[size_t(a) * N + idx + 1u];
+  }
+  const float4 m = get_moments(as, idx);
   const float4 t = xtab[x];
   if (inner && at.aflag[idx]) return;
   const float u[3] = {m.y, m.z, m.w};
   float mu[19];
-  trt_vjp<19>(lam, m.x, u, t.x, t.y, t.z, as.ut, mu);
+  if (x == 0u) {  // the inlet plane stores the inflow equilibrium whatever it pulled: mu = 0
+#pragma unroll
+    for (int i = 0; i < 19; i++) mu[i] = 0.0f;
+  } else if (x == W - 1u) {
+    outlet3_vjp(lam, m.x, u, mu);
+  } else {
+    trt_vjp<19>(lam, m.x, u, t.x, t.y, t.z, as.ut, mu);
+  }
   char* const d = reinterpret_cast<char*>(as.mu);
 #pragma unroll
   for (int i = 0; i < 19; i++) *reinterpret_cast<float*>(d + size_t(i) * N * 4 + off) = mu[i];
@@ -319,7 +351,7 @@ __device__ __forceinline__ float inlet_profile2(int y, float u, const Params2& p
 
 __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, const float4* __restrict__ xtab,
                                                     Params2 p, uint32_t yl) {
-  const uint32_t N = p.N;
+  const uint32_t N = as.Ns;  // the adjoint state's population stride
   if (blockIdx.y < yl) {
     const uint32_t k = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
     if (k < at.n_list) {
@@ -339,7 +371,7 @@ __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, co
     const int64_t nb = int64_t(idx) + int64_t(CY2[a]) * p.W + CX2[a];  // the downstream neighbour (in the grid)
     lam[a] = as.mu_next[size_t(a) * N + size_t(nb)];
   }
-  const float4 m = as.mom[idx];
+  const float4 m = get_moments(as, idx);
   const float u[3] = {m.y, m.z, 0.0f};
   const float4 t = xtab[x];
   float mu[9];
@@ -353,17 +385,27 @@ __global__ __launch_bounds__(1024) void adj2_kernel(AdjTables at, AdjStep as, co
 // ============================================================================ host side
 using namespace lbm;
 
+namespace {
+// Bytes of one cell's record: the moments are kept in the width of the population storage (a
+// 16-bit store holds them as halves, rounded to nearest; FP32 storage as floats).
+size_t mom_bytes(const lbm_solver* s) { return s->elem == 2 ? sizeof(Moments16) : sizeof(float4); }
+}  // namespace
+
 struct lbm_adjoint {
   lbm_adjoint_config cfg{};
   bool have_forward = false;
   uint64_t forward_gen = 0;  // geometry (list tables generation) of the forward run
   int64_t T = 0, K = 0, nseg = 0, nstored = 0;  // segments [nseg - nstored, nseg) were recorded by the forward pass
+  uint32_t Ns = 0;  // population stride of the adjoint state mu (mu_stride)
+  // The forward pass also records the segment before them into the recomputation slot (idle until the
+  // backward pass needs it); valid until a backward pass recomputes another segment there.
+  bool scratch_valid = false;
   double J = 0;
   std::vector<double> w;  // objective weight of the force sample of step n (n = 0 .. T), w[T] = 0
   // device memory
   std::vector<void*> ckpt;          // populations entering step seg * K
   void* final_state = nullptr;      // populations after step T (restored after the backward pass)
-  std::vector<float4*> rec_mom;     // per segment slot: K x N moments
+  std::vector<void*> rec_mom;       // per segment slot: K x N moments (mom_bytes each)
   std::vector<float*> rec_dq;       // per segment slot: K x n_links
   float* mu[2] = {nullptr, nullptr};
   double* acc = nullptr;
@@ -395,7 +437,7 @@ void free_buffers(lbm_adjoint* a) {
   for (void* p : a->ckpt)
     if (p) (void)hipFree(p);
   a->ckpt.clear();
-  for (float4* p : a->rec_mom)
+  for (void* p : a->rec_mom)
     if (p) (void)hipFree(p);
   a->rec_mom.clear();
   for (float* p : a->rec_dq)
@@ -507,10 +549,65 @@ void expected_reader(const lbm_solver* s, uint32_t u, int a, uint32_t* x, int* i
   *i = ia;
 }
 
+// The readers adj3_kernel's streamed rows assume for population a of node u (x = 1 .. W - 1): the
+// expected_reader, except at x = 1 for c_x < 0 (the inlet plane pulls nothing) and at x = W - 1 for
+// c_x > 0 (off the grid); at x = W - 2 for c_x < 0 also the outlet node of the same row, whose copy
+// rule reads u in direction a. Returns their number (at most 2).
+int bulk_readers3(const lbm_solver* s, uint32_t u, int a, uint32_t* xs, int* is) {
+  const uint32_t W = uint32_t(s->W), ux = u % W;
+  int n = 0;
+  uint32_t ex = NO_OPERAND;
+  int ei = -1;
+  expected_reader(s, u, a, &ex, &ei);
+  if (ex != NO_OPERAND && !(ux == 1u && CX3[a] < 0) && !(ux == W - 1u && CX3[a] > 0)) {
+    xs[n] = ex;
+    is[n++] = ei;
+  }
+  if (ux + 2u == W && CX3[a] < 0) {
+    xs[n] = u + 1u;
+    is[n++] = a;
+  }
+  return n;
+}
+
+// The index adj3_kernel's streamed rows load for population a of node u (the row pattern of its
+// face bits, read backwards, in the adjoint state's stride Ns), whether or not the node is streamed.
+
+// Population stride of the adjoint state: with a stride of N a power of two times a small factor
+// (2^22 at 256 x 128 x 128) the Q streams a node touches fall on the same memory channels; a stride
+// rounded to 1024 elements plus 17 x 4096 spreads them (the 19-stream copy kernel measured 5.80 TB/s
+// at the stride 1,769,472 = 27 x 2^16 and 6.36 TB/s at that stride + 69,632). Small grids keep N.
+uint32_t mu_stride(uint32_t N, int Q) {
+  if (N < 65536u) return N;
+  const uint64_t ns = (uint64_t(N) + 1023u) / 1024u * 1024u + 17u * 4096u;
+  return uint64_t(Q) * ns < (uint64_t(1) << 32) ? uint32_t(ns) : N;
+}
+uint64_t adj3_row_source(const lbm_solver* s, uint32_t Ns, uint32_t u, int a) {
+  const int64_t W = s->W, H = s->H, D = s->D, WH = W * H;
+  const int64_t y = (u / W) % H, z = u / WH;
+  const bool oY = (y == 0 && CY3[a] < 0) || (y == H - 1 && CY3[a] > 0);
+  const bool oZ = (z == 0 && CZ3[a] < 0) || (z == D - 1 && CZ3[a] > 0);
+  int pa = a;
+  int64_t delta = CZ3[a] * WH + CY3[a] * W + CX3[a];
+  if (oY && oZ) {
+    pa = opp3(a);
+    delta = 0;
+  } else if (oY) {
+    pa = mirror_y3(a);
+    delta = CZ3[a] * WH + CX3[a];
+  } else if (oZ) {
+    pa = mirror_z3(a);
+    delta = CY3[a] * W + CX3[a];
+  }
+  const int64_t v = int64_t(pa) * int64_t(Ns) + int64_t(u) + delta;
+  return v < 0 ? ~uint64_t(0) : uint64_t(v);
+}
+
 // Builds the adjoint tables for the solver's current geometry (after every rebuild of the list).
 int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   const uint32_t N = s->N;
   const int Q = s->Q;
+  const uint32_t Ns = mu_stride(N, Q);
   const bool d2 = s->nd == 2;
   const uint32_t solid = d2 ? FLAG_SOLID2 : FLAG_SOLID3;
   std::vector<uint32_t> pos(N, NO_OPERAND);
@@ -530,23 +627,45 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
         uint8_t& c = cnt[src[e].src];
         if ((c & 0x7f) < 0x7f) c++;
         const uint32_t a = src[e].src / N, u = src[e].src % N;
-        uint32_t ex = NO_OPERAND;
-        int ei = -1;
-        expected_reader(s, u, int(a), &ex, &ei);
-        if (!(ex == x && ei == i && src[e].w == 1.0f && src[e].v == 0.0f)) c |= 0x80;
+        uint32_t rx[2] = {NO_OPERAND, NO_OPERAND};
+        int ri[2] = {-1, -1};
+        const int nr = d2 ? 0 : bulk_readers3(s, u, int(a), rx, ri);
+        if (d2) expected_reader(s, u, int(a), &rx[0], &ri[0]);
+        bool expected = false;
+        for (int r = 0; r < (d2 ? 1 : nr); r++) expected = expected || (rx[r] == x && ri[r] == i);
+        if (!(expected && src[e].w == 1.0f && src[e].v == 0.0f)) c |= 0x80;
       }
     }
   }
-  // adjoint bulk nodes: plain collision and every population read exactly once, plainly
+  // adjoint bulk nodes: every population read exactly by the readers the streamed rows assume,
+  // plainly, and (2D) a plain collision or (3D) the collision, outlet or inlet rule of the node's x
+  // plane; nodes with Bouzidi links stay in the list, which accumulates their dJ/dq
   std::vector<uint8_t> aflag(N, 1);
   std::vector<uint32_t> alist;
   a->n_regular = 0;
+  const uint32_t Wu = uint32_t(s->W);
   for (uint32_t y = 0; y < N; y++) {
     const uint32_t fl = s->h_flags[y];
     if (fl & solid) continue;
-    const bool plain_kind = d2 ? !(fl & (FLAG_INLET2 | FLAG_OUTLET2)) : !(fl & (FLAG_INLET3 | FLAG_OUTLET3));
-    bool reg = plain_kind;
-    for (int i = 0; i < Q && reg; i++) reg = cnt[size_t(i) * N + y] == 1;
+    const bool links = pos[y] != NO_OPERAND && s->h_bmask[pos[y]] != 0u;
+    bool reg;
+    if (d2) {
+      reg = !(fl & (FLAG_INLET2 | FLAG_OUTLET2)) && !links;
+      for (int i = 0; i < Q && reg; i++) reg = cnt[size_t(i) * N + y] == 1;
+    } else {
+      const uint32_t yx = y % Wu;
+      const uint32_t kind = (fl & FLAG_INLET3) ? AK_INLET3 : (fl & FLAG_OUTLET3) ? AK_OUTLET3 : AK_PLAIN;
+      reg = !links && kind == (yx == 0u ? AK_INLET3 : yx + 1u == Wu ? AK_OUTLET3 : AK_PLAIN);
+      if (yx != 0u) {  // the inlet plane's mu is zero whoever reads it
+        for (int i = 0; i < Q && reg; i++) {
+          uint32_t rx[2];
+          int ri[2];
+          const int nr = bulk_readers3(s, y, i, rx, ri);
+          const uint8_t c = cnt[size_t(i) * N + y];
+          reg = (c & 0x80) == 0 && int(c) == nr;
+        }
+      }
+    }
     if (reg) {
       aflag[y] = 0;
       a->n_regular++;
@@ -596,12 +715,18 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
     for (int i = 0; i < Q; i++) {
       uint32_t ex = NO_OPERAND;
       int ei = -1;
-      expected_reader(s, y, i, &ex, &ei);
-      if (ex == NO_OPERAND) return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node has a reader off the grid");
+      if (d2) {
+        expected_reader(s, y, i, &ex, &ei);
+        if (ex == NO_OPERAND) return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node has a reader off the grid");
+      } else if (adj3_row_source(s, Ns, y, i) >= uint64_t(Q) * Ns) {
+        return fail(LBM_ERR_DEVICE, "adjoint tables: a streamed node's load leaves the buffer");
+      }
     }
   }
   for (const AdjEntry& en : ent)
     if (en.src >= uint64_t(Q) * N || en.a >= uint32_t(Q)) return fail(LBM_ERR_DEVICE, "adjoint tables: entry out of range");
+  if (a->Ns && a->Ns != Ns) return fail(LBM_ERR_DEVICE, "adjoint tables: state stride changed");
+  a->Ns = Ns;
   for (uint32_t v : lk)
     if (v != NO_OPERAND && v >= s->h_list.size()) return fail(LBM_ERR_DEVICE, "adjoint tables: link node out of range");
   free_tables(a);
@@ -612,7 +737,11 @@ int build_tables(lbm_solver* s, lbm_adjoint* a, const double* dir) {
   for (size_t k = 0; k < nl; k++) max_e = std::max(max_e, ne[k + 1] - ne[k]);
   std::vector<AdjEntry> ell(size_t(max_e) * nl, AdjEntry{0u, 0u, 0.0f, 0.0f});
   for (size_t k = 0; k < nl; k++)
-    for (uint32_t j = 0; j < ne[k + 1] - ne[k]; j++) ell[size_t(j) * nl + k] = ent[ne[k] + j];
+    for (uint32_t j = 0; j < ne[k + 1] - ne[k]; j++) {
+      AdjEntry en = ent[ne[k] + j];
+      en.src = (en.src / N) * Ns + en.src % N;  // into the adjoint state's stride
+      ell[size_t(j) * nl + k] = en;
+    }
   HIP_TRY(hipMalloc(&a->aflag, N));
   HIP_TRY(hipMalloc(&a->aidx, 4 * std::max<size_t>(1, nl)));
   HIP_TRY(hipMalloc(&a->akind, 4 * std::max<size_t>(1, nl)));
@@ -640,7 +769,9 @@ int adjoint_step(lbm_solver* s, lbm_adjoint* a, int64_t n, int slot, const float
   AdjStep as;
   as.mu_next = mu_next;
   as.mu = mu_cur;
-  as.mom = a->rec_mom[size_t(slot)] + size_t(n % a->K) * s->N;
+  as.Ns = a->Ns;
+  as.mom = static_cast<const char*>(a->rec_mom[size_t(slot)]) + size_t(n % a->K) * s->N * mom_bytes(s);
+  as.half_mom = mom_bytes(s) == sizeof(Moments16);
   as.dq = a->rec_dq[size_t(slot)] + size_t(n % a->K) * std::max<uint32_t>(s->n_links, 1);
   as.acc = a->acc;
   as.w_next = float(a->w[size_t(n + 1)]);
@@ -706,7 +837,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   const int64_t T = cfg->steps, K = cfg->checkpoint_every, nseg = (T + K - 1) / K;
   const size_t fbytes = size_t(s->Q) * s->N * s->elem;
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
-  const size_t seg_bytes = size_t(K) * s->N * sizeof(float4) + size_t(K) * nlk * 4;
+  const size_t seg_bytes = size_t(K) * s->N * mom_bytes(s) + size_t(K) * nlk * 4;
   const int64_t nstored = std::min<int64_t>(nseg, std::max<int64_t>(0, cfg->record_budget_bytes) / int64_t(seg_bytes));
   // The device buffers are kept from one forward run to the next while their layout is unchanged
   // (the usual case of finite differences or an optimisation loop over one tunnel).
@@ -721,7 +852,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
     a->rec_mom.assign(size_t(nslots), nullptr);
     a->rec_dq.assign(size_t(nslots), nullptr);
     for (int64_t g = 0; g < nslots; g++) {
-      HIP_TRY(hipMalloc(&a->rec_mom[size_t(g)], size_t(K) * s->N * sizeof(float4)));
+      HIP_TRY(hipMalloc(&a->rec_mom[size_t(g)], size_t(K) * s->N * mom_bytes(s)));
       HIP_TRY(hipMalloc(&a->rec_dq[size_t(g)], size_t(K) * nlk * 4));
     }
     HIP_TRY(hipMalloc(&a->final_state, fbytes));
@@ -757,13 +888,15 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
     return LBM_OK;
   };
   const int64_t first_stored = a->nseg - a->nstored;
+  a->scratch_valid = false;
   for (int64_t n = 0; n < a->T; n++) {
     const int64_t g = n / a->K;
     if (n % a->K == 0 && g < first_stored)
       HIP_TRY(hipMemcpyAsync(a->ckpt[size_t(g)], s->f[s->parity], fbytes, hipMemcpyDeviceToDevice, s->stream));
-    if (g >= first_stored) {
-      const size_t slot = size_t(g - first_stored);
-      Record rec{a->rec_mom[slot] + size_t(n % a->K) * s->N,
+    if (g + 1 >= first_stored) {
+      // the segment before the recorded ones goes to the recomputation slot (index nstored)
+      const size_t slot = g >= first_stored ? size_t(g - first_stored) : size_t(a->nstored);
+      Record rec{static_cast<char*>(a->rec_mom[slot]) + size_t(n % a->K) * s->N * mom_bytes(s),
                  a->rec_dq[slot] + size_t(n % a->K) * std::max<uint32_t>(s->n_links, 1)};
       if (int rc = enqueue_step(s, &rec, 1)) return rc;
     } else {
@@ -778,6 +911,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
       if (int rc = drain()) return rc;
   }
   if (int rc = drain()) return rc;
+  a->scratch_valid = first_stored > 0;
   HIP_TRY(hipMemcpyAsync(a->final_state, s->f[s->parity], fbytes, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   a->J = cfg->scale * sum / double(count);
@@ -800,8 +934,8 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
   const size_t fbytes = size_t(s->Q) * N * s->elem;
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
   for (float*& m : a->mu) {
-    if (!m) HIP_TRY(hipMalloc(&m, size_t(s->Q) * N * 4));
-    HIP_TRY(hipMemsetAsync(m, 0, size_t(s->Q) * N * 4, s->stream));
+    if (!m) HIP_TRY(hipMalloc(&m, size_t(s->Q) * a->Ns * 4));
+    HIP_TRY(hipMemsetAsync(m, 0, size_t(s->Q) * a->Ns * 4, s->stream));
   }
   if (!a->acc) HIP_TRY(hipMalloc(&a->acc, nlk * 8));
   HIP_TRY(hipMemsetAsync(a->acc, 0, nlk * 8, s->stream));
@@ -816,9 +950,12 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
     int slot;
     if (g >= first_stored) {
       slot = int(g - first_stored);
+    } else if (g + 1 == first_stored && a->scratch_valid) {
+      slot = int(a->nstored);  // recorded by the forward pass
     } else {
       // recompute the segment from its checkpoint into the scratch slot
       slot = int(a->nstored);
+      a->scratch_valid = false;
       if (hipMemcpyAsync(s->f[0], a->ckpt[size_t(g)], fbytes, hipMemcpyDeviceToDevice, s->stream) != hipSuccess) {
         rc = fail(LBM_ERR_DEVICE, "adjoint: checkpoint restore failed");
         break;
@@ -826,7 +963,8 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
       s->parity = 0;
       s->step = n0;
       for (int64_t k = n0; k < n1 && rc == LBM_OK; k++) {
-        Record rec{a->rec_mom[size_t(slot)] + size_t(k - n0) * N, a->rec_dq[size_t(slot)] + size_t(k - n0) * nlk};
+        Record rec{static_cast<char*>(a->rec_mom[size_t(slot)]) + size_t(k - n0) * N * mom_bytes(s),
+                   a->rec_dq[size_t(slot)] + size_t(k - n0) * nlk};
         rc = enqueue_step(s, &rec, 0);
       }
     }

@@ -41,7 +41,7 @@ __device__ __forceinline__ void zou_he_outlet(float (&f)[9]) {
 __device__ __forceinline__ void record2(const float (&f)[9], const Record& rc, uint32_t idx) {
   const float drho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
   const float invRho = 1.0f / (1.0f + drho);
-  rc.mom[idx] = make_float4(drho, (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) * invRho,
+  static_cast<float4*>(rc.mom)[idx] = make_float4(drho, (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) * invRho,
                             (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]) * invRho, 0.0f);
 }
 

@@ -34,12 +34,26 @@ struct ListTables {
 };
 
 // What a recording step writes for the discrete adjoint: per cell, the moments (drho, ux, uy, uz) of
-// the populations the collision takes (after the boundary rules); per Bouzidi link (record order of
+// the populations the collision takes (after the boundary rules), as a float4, or as four halves
+// (Moments16) when the populations are stored in 16 bits; per Bouzidi link (record order of
 // ListTables::links), the derivative of the interpolated population with respect to q.
 struct Record {
-  float4* mom;
+  void* mom;
   float* dq;
 };
+
+struct alignas(8) Moments16 {
+  _Float16 drho, ux, uy, uz;
+};
+
+// The record of one cell in the layout of the storage width (2: Moments16, rounded to nearest).
+template <int ELEM_BYTES>
+__device__ __forceinline__ void put_moments(void* mom, uint32_t idx, float drho, float ux, float uy, float uz) {
+  if constexpr (ELEM_BYTES == 2)
+    static_cast<Moments16*>(mom)[idx] = Moments16{_Float16(drho), _Float16(ux), _Float16(uy), _Float16(uz)};
+  else
+    static_cast<float4*>(mom)[idx] = make_float4(drho, ux, uy, uz);
+}
 
 // ---- D3Q19; prec is LBM_FP32 / LBM_FP16 / LBM_BF16 / LBM_FP16_SR ----
 // One time step in one launch: bulk rows (plain fluid, slip-face rows and plain inlet/outlet nodes,

@@ -392,7 +392,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   if (zh && inlet) zou_he_face3<true>(f, p.uIn);
   else if (zh && outlet) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
-  if (REC && !inlet && rc.mom) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
+  if (REC && !inlet && rc.mom) put_moments<sizeof(E)>(rc.mom, idx, m.drho, m.ux, m.uy, m.uz);
   if (!MERGE && outlet && !zh) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
 #pragma unroll
@@ -497,7 +497,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   if (zh && (fl & FLAG_INLET3)) zou_he_face3<true>(f, p.uIn);
   else if (zh && (fl & FLAG_OUTLET3)) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
-  if (REC && rc.mom) rc.mom[idx] = make_float4(m.drho, m.ux, m.uy, m.uz);
+  if (REC && rc.mom) put_moments<sizeof(typename S::elem)>(rc.mom, idx, m.drho, m.ux, m.uy, m.uz);
   if ((fl & FLAG_OUTLET3) && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);

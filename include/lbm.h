@@ -246,9 +246,12 @@ int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid)
  * equilibrium at the first step's inlet speed in 3D) for `steps` steps with the given inlet ramp and a force sample every step, and
  * returns J = scale * mean over the samples of steps window_start+1 .. steps of direction . F (the
  * sample of step k is taken from the populations entering it, as lbm_read_forces numbers them). It
- * records the last segments directly while record_budget_bytes allows (16 B per cell per step plus
- * 4 B per Bouzidi link per step), so that the backward pass does not recompute them, and keeps a
- * checkpoint of the populations at the start of every other segment of checkpoint_every steps.
+ * records the last segments directly while record_budget_bytes allows (the moments of the collision
+ * input, 16 B per cell per step with FP32 storage, 8 B with 16-bit storage, plus 4 B per Bouzidi link
+ * per step), so that the backward pass does not recompute them, keeps a checkpoint of the
+ * populations at the start of every other segment of checkpoint_every steps, and records the
+ * segment before the recorded ones into the one segment of scratch the backward pass recomputes
+ * into (so that, if no later backward pass overwrote it, that segment is not recomputed either).
  * lbm_adjoint_backward may be called any number of times after one forward run.
  * lbm_adjoint_backward(seed) then returns seed * dJ/dsdf at every node (N doubles) of the discrete
  * step: it recomputes each remaining segment from its checkpoint, runs the reverse-time kernels

@@ -210,3 +210,56 @@ def test_adjoint3d_sphere_256x128x128_fp16():
     assert out["info"]["stored_segments"] == 0 and out["info"]["segments"] == 30
     assert out["max_rel_err_vs_fd"] < 0.01 and out["max_rel_err_fp32_adjoint_vs_fd"] < 0.01
 
+
+
+def test_adjoint3d_open_planes_against_fp64_differences():
+    """A short tunnel with no sponge and the body two cells from the outlet plane, so that the
+    reverse step's rules for the x planes next to the open ends (inlet plane mu = 0, no reader of the
+    c_x < 0 populations at x = 1, the outlet's copy rule reading x = W - 2, the outlet equilibrium's
+    transpose at x = W - 1) all reach the body within the run: every spherical-harmonic coefficient's
+    gradient against central differences of the FP64 oracle (h = 1e-4, computed here), and the open
+    planes are streamed, not listed."""
+    from oracle.oracle import OracleSolver3D
+
+    W, H, D, u, r = 14, 10, 10, 0.05, 2.5
+    steps, window = 60, 30
+    c = (8.5, 4.7, 4.6)
+    cfg = dict(width=W, height=H, depth=D, precision="fp32", tau=0.6, uIn=u, spongeFraction=0.0, absorb=0.0,
+               forceEvery=1)
+    scale = 2 / (u * u * math.pi * r * r)
+    coeffs = np.zeros(16)
+
+    def sdf_of(b):
+        return shapes.sphere_sh(W, H, D, c, r, b)
+
+    def oracle_J(sdf):
+        o = OracleSolver3D(dict(cfg), 64)
+        o.set_sdf(sdf)
+        o.set_inlet_ramp("smoothstep", u, 20)
+        o.set_inlet(0.0)
+        o.init_field()
+        o.steps(steps)
+        st, v = o.read_forces_raw()
+        o.destroy()
+        return scale * v[st > steps - window, 0].mean()
+
+    s = Solver3D.create(dict(cfg))
+    adj = ShapeAdjoint(s, steps, window, (1.0, 0.0, 0.0), scale, checkpoint_every=20, ramp=("smoothstep", u, 20))
+    sdf0, dsdf = sdf_of(coeffs)
+    J = adj.forward(sdf0)
+    grad = parameter_gradient(adj.backward(), dsdf)
+    info = adj.info()
+    J64 = oracle_J(sdf0)
+    h = 1e-4
+    fd = np.array([(oracle_J(sdf_of(coeffs + h * e)[0]) - oracle_J(sdf_of(coeffs - h * e)[0])) / (2 * h)
+                   for e in np.eye(16)])
+    big = np.abs(fd) >= 0.01 * np.abs(fd).max()
+    rel = np.abs(grad[big] - fd[big]) / np.abs(fd[big])
+    print(f"J {J:.7f} (FP64 {J64:.7f}); {info}; adjoint vs FP64 differences: max rel {rel.max():.2e} on "
+          f"{int(big.sum())} components, largest error elsewhere {np.abs(grad - fd)[~big].max():.1e}")
+    assert abs(J - J64) <= 1e-5 * abs(J64)
+    assert rel.max() < 2e-3
+    assert np.abs(grad - fd)[~big].max() <= 1e-3 * np.abs(fd).max()
+    # the open planes and their neighbours (4 H D nodes) are streamed; the list holds the body's nodes
+    assert info["list_nodes"] < 4 * H * D
+    s.destroy()
