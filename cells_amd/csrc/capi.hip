@@ -618,9 +618,10 @@ int check_usable(const lbm_solver* s) {
   return LBM_OK;
 }
 
-int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
+int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials, uint32_t* counter) {
   const int every = s->force_every();
-  ForceSink fs{s->partials, s->counter, s->ring, s->state, 0, uint64_t(s->step + 1)};
+  ForceSink fs{partials ? partials : s->partials, counter ? counter : s->counter, s->ring, s->state, 0,
+               uint64_t(s->step + 1)};
   fs.sample = sample >= 0 ? sample : ((every > 0 && (s->step + 1) % every == 0) ? 1 : 0);
   void* src = s->f[s->parity];
   void* dst = s->f[s->parity ^ 1];
@@ -651,8 +652,14 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample) {
   }
   s->parity ^= 1;
   s->step++;
-  if (fs.sample) s->force_steps.push_back(s->step);
+  if (fs.sample) s->push_sample_step(s->step);
   return LBM_OK;
+}
+
+size_t force_partials_needed(const lbm_solver* s) {
+  if (s->Q == 27) return step27_partials(s->W, s->H, s->D);
+  const uint32_t bx = bulk_block_x(s->W, s->bulk_block);
+  return size_t(body_list_blocks(s->n_body, bx, true)) * (bx / 64u);
 }
 
 // Advances the host bookkeeping of a tunnel by n steps that a batch launch performed.
@@ -661,7 +668,7 @@ void record_steps(lbm_solver* s, int64_t n) {
   std::lock_guard<std::mutex> lk(s->mu);
   for (int64_t k = 0; k < n; k++) {
     s->step++;
-    if (every > 0 && s->step % every == 0) s->force_steps.push_back(s->step);
+    if (every > 0 && s->step % every == 0) s->push_sample_step(s->step);
   }
   s->parity ^= int(n & 1);
 }
@@ -983,7 +990,7 @@ int lbm_reset_forces(lbm_solver* s) {
   HIP_TRY(hipGetLastError());
   std::lock_guard<std::mutex> lk(s->mu);
   s->samples_read = 0;
-  s->force_steps.clear();
+  s->samples_pushed = 0;
   s->generation++;
   return LBM_OK;
 }
@@ -1084,7 +1091,7 @@ int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* v
   {
     std::lock_guard<std::mutex> lk(s->mu);
     // A reset while the copy was in flight makes these samples stale (src/solver.ts:322-323).
-    if (t.generation == s->generation && !s->force_steps.empty()) {
+    if (t.generation == s->generation && s->samples_pushed > s->samples_read) {
       const int64_t written = int64_t(*t.state);
       const int64_t first = std::max(s->samples_read, written - int64_t(HISTORY_LEN));
       const int64_t avail = written - first;
@@ -1094,13 +1101,11 @@ int lbm_read_forces_end(lbm_solver* s, int64_t ticket, int64_t* steps, double* v
         int64_t n = 0;
         for (int64_t k = first; k < written; k++, n++) {
           const int64_t slot = k % HISTORY_LEN;
-          const int64_t pos = k - s->samples_read;
-          steps[n] = pos < int64_t(s->force_steps.size()) ? s->force_steps[size_t(pos)] : -1;
+          const bool kept = k < s->samples_pushed && k >= s->samples_pushed - int64_t(HISTORY_LEN);
+          steps[n] = kept ? s->sample_steps[size_t(k % HISTORY_LEN)] : -1;
           for (int d = 0; d < dim; d++) values[n * dim + d] = t.ring[slot * dim + d];
         }
         *count = n;
-        const int64_t drop = std::min<int64_t>(written - s->samples_read, int64_t(s->force_steps.size()));
-        s->force_steps.erase(s->force_steps.begin(), s->force_steps.begin() + drop);
         s->samples_read = written;
       }
     }
@@ -1252,7 +1257,8 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     hipEvent_t a = nullptr, b = nullptr;
     int64_t step0;
     int parity0, block0, sweep0;
-    size_t nsteps;
+    int64_t pushed0;
+    std::vector<int64_t> steps0;
     bool saved = false;  // every copy was enqueued: from here on the state must be restored
     bool keep_shape = false;
     int restore() {
@@ -1273,7 +1279,8 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       if (b) (void)hipEventDestroy(b);
       s->step = step0;
       s->parity = parity0;
-      s->force_steps.resize(std::min(nsteps, s->force_steps.size()));
+      s->samples_pushed = pushed0;
+      s->sample_steps = steps0;
       if (!keep_shape) {
         s->bulk_block = block0;
         s->sweep = sweep0;
@@ -1284,7 +1291,8 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   g.parity0 = s->parity;
   g.block0 = s->bulk_block;
   g.sweep0 = s->sweep;
-  g.nsteps = s->force_steps.size();
+  g.pushed0 = s->samples_pushed;
+  g.steps0 = s->sample_steps;
   HIP_TRY(hipMalloc(&g.save[0], fbytes));
   HIP_TRY(hipMalloc(&g.save[1], fbytes));
   HIP_TRY(hipMalloc(&g.ring, ring_bytes));

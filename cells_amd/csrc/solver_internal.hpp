@@ -113,7 +113,15 @@ struct lbm_solver {
 
   // readForces bookkeeping (src/solver.ts:57-61, 236-241, 315-335)
   int64_t samples_read = 0;
-  std::deque<int64_t> force_steps;
+  // Step number of every force sample enqueued since the last reset, the newest HISTORY_LEN kept
+  // (sample k at sample_steps[k % HISTORY_LEN]): readForces never returns a sample more than
+  // HISTORY_LEN behind the newest, so nothing here grows while a run goes unread.
+  std::vector<int64_t> sample_steps = std::vector<int64_t>(size_t(lbm::HISTORY_LEN), 0);
+  int64_t samples_pushed = 0;
+  void push_sample_step(int64_t st) {
+    sample_steps[size_t(samples_pushed % lbm::HISTORY_LEN)] = st;
+    samples_pushed++;
+  }
   int64_t generation = 0;
   std::mutex mu;
   std::map<int64_t, lbm::Ticket> tickets;
@@ -131,8 +139,12 @@ int rebuild_flags(lbm_solver* s);
 int check_usable(const lbm_solver* s);
 void record_steps(lbm_solver* s, int64_t n);
 // One step on the solver's stream: force sample as configured (sample < 0) or forced on / off
-// (1 / 0); with rec, the step also writes the adjoint's record.
-int enqueue_step(lbm_solver* s, const Record* rec, int sample);
+// (1 / 0); with rec, the step also writes the adjoint's record. partials / counter replace the
+// handle's force-reduction scratch when given (lbm_enqueue_steps_noalloc).
+int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials = nullptr,
+                 uint32_t* counter = nullptr);
+// Force-reduction partials (3 doubles each) the next step of the solver writes, if it samples.
+size_t force_partials_needed(const lbm_solver* s);
 
 // The bulk kernel's view of a D3Q19 node (step3_kernel): the face bits of its row, whether the flag
 // word selects the bulk pass, and the gather index of each pull under the row's pattern.

@@ -154,6 +154,7 @@ def load_library(path: str = LIB_PATH):
         "lbm_init_field": ([P, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_reset_forces": ([P], C.c_int),
         "lbm_enqueue_steps": ([P, C.c_int64], C.c_int),
+        "lbm_enqueue_steps_noalloc": ([P, C.c_int64], C.c_int),
         "lbm_enqueue_macro": ([P], C.c_int),
         "lbm_run": ([P, C.c_int64, C.c_int64], C.c_int),
         "lbm_sync": ([P], C.c_int),
@@ -265,6 +266,11 @@ class _Base:
     def encode_steps(self, n: int, _pass=None):
         """encodeSteps: the pass argument of the reference has no equivalent and is ignored."""
         _check(_lib.lbm_enqueue_steps(self._h, int(n)))
+
+    def encode_steps_noalloc(self, n: int):
+        """encodeSteps without any allocation: the force reduction's scratch is the library's static
+        per-GPU array (calls on one GPU must not overlap)."""
+        _check(_lib.lbm_enqueue_steps_noalloc(self._h, int(n)))
 
     def encode_macro(self, _pass=None):
         _check(_lib.lbm_enqueue_macro(self._h))

@@ -165,6 +165,16 @@ int lbm_reset_forces(lbm_solver* s);
 
 /* encodeSteps (src/solver.ts:271-286): enqueue n steps (+ force samples every force_every). */
 int lbm_enqueue_steps(lbm_solver* s, int64_t n);
+/* No counterpart in the reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation): lbm_enqueue_steps, with the same results
+ * bit for bit, whose force reduction keeps its scratch (FP64 per-wave partials and a completion
+ * counter) in one static device array per GPU instead of the handle's buffers, and which allocates
+ * nothing, host or device. The array holds 32768 partials (768 KiB): a tunnel whose reduction needs
+ * more (one partial per wave of body-link nodes; D3Q27: per wave of the grid, above 2 M cells) is
+ * refused with LBM_ERR_UNSUPPORTED. All calls on one GPU share the array and the counter is cleared
+ * at the start of every call, so calls that overlap on one GPU (two handles on different streams)
+ * race; the call itself is stream-ordered like lbm_enqueue_steps. */
+int lbm_enqueue_steps_noalloc(lbm_solver* s, int64_t n);
 /* encodeMacro (src/solver.ts:288-292): enqueue the macro-field kernel for the current buffer. */
 int lbm_enqueue_macro(lbm_solver* s);
 /* run (src/solver.ts:295-307): n steps in chunks, macro after each chunk, then wait for the GPU. */
