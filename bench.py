@@ -107,25 +107,35 @@ def d2q9_config0(steps=5000):
     """D2Q9 512 x 128 cylinder tunnel (diameter 16 at (128, 64.5), tau 0.548, Zou-He inlet u = 0.1
     ramped linearly over 3000 steps, Zou-He outlet, slip walls, 15% sponge, absorbing layer 0.02,
     a force sample every step): MLUPS and algorithmic TB/s at 76 B per cell. At 65,536 cells one step
-    is a few microseconds, so this measures launch-bound throughput."""
+    is a few microseconds, so this measures launch-bound throughput; the floor is reported beside it:
+    the copy-roofline kernel (9 populations and the flag word in, 9 out, no arithmetic) at the same
+    size, one launch per step, and the same tunnel without force samples."""
     from cells_amd import geometry as g
-    from cells_amd.lbm import X_OPEN, Y_SLIP, Solver
+    from cells_amd.lbm import X_OPEN, Y_SLIP, Solver, copy_roofline
 
     W, H = 512, 128
-    s = Solver.create(dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=0.548,
-                           uIn=0.1, spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=1))
-    sdf = g.empty_sdf(W, H)
-    g.add_circle(sdf, W, H, 128, 64.5, 8)
-    s.set_sdf(sdf)
-    s.init_field()
-    s.set_inlet_ramp("linear", 0.1, 3000)
-    block = s.autotune(200)
-    s.time_steps(500)
-    ms = s.time_steps(steps)
-    s.destroy()
+
+    def run(force_every):
+        s = Solver.create(dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=0.548,
+                               uIn=0.1, spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=force_every))
+        sdf = g.empty_sdf(W, H)
+        g.add_circle(sdf, W, H, 128, 64.5, 8)
+        s.set_sdf(sdf)
+        s.init_field()
+        s.set_inlet_ramp("linear", 0.1, 3000)
+        block = s.autotune(200)
+        s.time_steps(500)
+        ms = s.time_steps(steps)
+        s.destroy()
+        return ms, block
+
+    ms, block = run(1)
+    ms_nf, _ = run(0)
     mlups = W * H * steps / (ms * 1e3)
+    copy_us = min(W * H * 76 / (copy_roofline(W * H, 9, 4, iters=2000, workgroup=wg) * 1e9) * 1e6 for wg in (64, 128, 256))
     return dict(grid=[W, H], steps=steps, mlups=mlups, ms_per_step=ms / steps, algorithmic_gbs=mlups * 76e-3,
-                bulk_block=block)
+                bulk_block=block, step_us=ms / steps * 1e3, step_without_force_samples_us=ms_nf / steps * 1e3,
+                copy_kernel_us_same_size=copy_us, copy_kernel_mlups_same_size=W * H / copy_us)
 
 
 PRECISION_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",

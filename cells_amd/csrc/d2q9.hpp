@@ -161,10 +161,18 @@ template <bool REC>
 __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
                                            const Params2& p, uint32_t x, int y, uint32_t idx, const Record& rc) {
-  const uint32_t fl = flags[idx];
-  const bool inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
-  const bool outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
-  if (fl != 0u && !inlet && !outlet) return;
+  // On a row with a row above and below, every plain pull address lies in the buffer (at x = 0 and
+  // x = W - 1 they reach into the neighbouring rows), so the loads go out before the flag word is
+  // known and their latency overlaps its; the boundary rows read the flag word first.
+  const bool interior = y > 0 && y + 1 < p.H;
+  uint32_t fl = 0u;
+  bool inlet = false, outlet = false;
+  if (!interior) {
+    fl = flags[idx];
+    inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
+    outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
+    if (fl != 0u && !inlet && !outlet) return;
+  }
   const size_t N = p.N;
   const uint32_t off = idx * 4u;
   const char* const sb = reinterpret_cast<const char*>(src);
@@ -175,9 +183,16 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
     const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * 4;
     uint32_t o = off;
     // The populations entering from off the grid are placeholders (step.ts:173-174) that Zou-He
-    // overwrites; read the node's own population instead so the address stays on the grid.
+    // overwrites; on the boundary rows read the node's own population instead so the address stays
+    // on the grid (on interior rows the plain address is in the buffer and its value is overwritten).
     if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) o = off + uint32_t(delta) * 4u;
     f[i] = *reinterpret_cast<const float*>(base + o);
+  }
+  if (interior) {
+    fl = flags[idx];
+    inlet = fl == (FLAG_INLET2 | INLET_DIRS2);
+    outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
+    if (fl != 0u && !inlet && !outlet) return;
   }
   if (inlet) zou_he_inlet(f, y, p);
   if (outlet) zou_he_outlet(f);

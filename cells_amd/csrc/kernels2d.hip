@@ -33,9 +33,13 @@ __global__ __launch_bounds__(1024) void step2_kernel(const float* __restrict__ s
       return;
     }
     if (lb >= nlb + nbb) return;
+    // a sample summed in this one block: its ring count is read now, beside the node's own loads
+    const uint32_t st0 = (fs.sample && nbb == 1 && threadIdx.x == 0)
+                             ? __hip_atomic_load(fs.state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                             : ~0u;
     const uint32_t kb = (lb - nlb) * blockDim.x + threadIdx.x;
     if (kb < lt.n_body) list_cell2<true, REC>(src, dst, flags, xtab, lt, lt.n_link + kb, p, rc, force);
-    if (fs.sample) force_sample<2>(force, fs, lb - nlb, nbb);
+    if (fs.sample) force_sample<2>(force, fs, lb - nlb, nbb, st0);
     return;
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;

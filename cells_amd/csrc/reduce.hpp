@@ -4,8 +4,9 @@
 // src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every wave of the group sums its
 // cells' momentum-exchange forces in double with wave64 butterfly shuffles and publishes one
 // partial, and the last block to finish (completion counter) sums the partials in wave order and
-// appends the sample to the history ring. The summation order is fixed, so a sample is
-// bit-reproducible run to run.
+// appends the sample to the history ring. When one block holds every node of the sample, the
+// partials meet in LDS instead and that block appends the sample, with the same summation order.
+// The summation order is fixed, so a sample is bit-reproducible run to run.
 //
 // Publication uses device-scope (agent) atomic stores and loads, which are coherent across the
 // XCDs' separate L2 caches, and a wait for the stores' completion before the counter increment.
@@ -29,14 +30,41 @@ __device__ __forceinline__ void wave_sum(double (&v)[DIM]) {
 // nodes of one tunnel): block is this block's index inside the group, nblocks the group's size.
 // Partials are per wave (64 consecutive nodes), and the last block sums them in wave order with a
 // fixed lane assignment, so the sample does not depend on the block size.
+// st0: the ring's sample count read by the caller before its own loads (only the appending thread of
+// the sample writes it, so it cannot change within the launch), or ~0u to read it here.
 template <int DIM>
-__device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uint32_t block, uint32_t nblocks) {
+__device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uint32_t block, uint32_t nblocks,
+                             uint32_t st0 = ~0u) {
   __shared__ int is_last;
+  __shared__ double part[16][DIM];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, wpb = blockDim.x >> 6;
   double v[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; d++) v[d] = double(force[d]);
   wave_sum<DIM>(v);
+  if (nblocks == 1) {
+    // One block holds every node of the sample: its waves' partials meet in LDS and its first wave
+    // sums them exactly as the last block does below (lane l takes partial l, then the butterfly),
+    // without the round trips through device memory and the completion counter.
+    if (lane == 0)
+#pragma unroll
+      for (int d = 0; d < DIM; d++) part[wave][d] = v[d];
+    __syncthreads();
+    if (wave != 0) return;
+    double acc[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) acc[d] = 0.0 + (lane < wpb ? part[lane][d] : 0.0);
+    wave_sum<DIM>(acc);
+    if (lane == 0) {
+      const uint32_t st = st0 != ~0u ? st0 : __hip_atomic_load(fs.state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t slot = st % HISTORY_LEN;
+#pragma unroll
+      for (int d = 0; d < DIM; d++) fs.ring[size_t(slot) * DIM + d] = acc[d];
+      note_divergence<DIM>(acc, fs.state, fs.step);
+      __hip_atomic_store(fs.state, st + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
   if (lane == 0) {
     const size_t part = size_t(block) * wpb + wave;
 #pragma unroll
