@@ -237,7 +237,7 @@ def sweep_bench(steps):
         v != 0 for v in ib["diverged_at"]) == len(cmp["diverged"]))
     bm, sm = sweep.mlups(cases, steps, tb), sweep.mlups(cases, steps, ts)
     cells = sum(c["config"]["width"] * c["config"]["height"] for c in cases)
-    return dict(tunnels=len(cases), cells=cells, steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
+    return dict(cells=cells, steps=steps, batched_mlups=bm, sequential_mlups=sm, speedup=bm / sm,
                 batched_s=tb, sequential_s=ts, launch=dict(tile=ib["tile"], order=ib["order"]),
                 batched_algorithmic_gbs=bm * 76e-3, batched_roofline_frac=bm * 76e-3 / HBM_PEAK_GBS, **cmp)
 
@@ -472,12 +472,18 @@ ONLY = {
 
 
 def copy_rooflines():
-    from cells_amd.lbm import copy_roofline
+    """Copy-roofline kernels (src/bench.ts:61-95: Q populations and the flag word read, Q written, no
+    arithmetic, aligned streams) and the pull copy (the same bytes with a D3Q19 step's shifted streams)."""
+    from cells_amd.lbm import copy_roofline, pull_copy_roofline
 
     n1, n4 = 192 * 96 * 96, 256 * 128 * 128
     return {"fp32_192x96x96_gbs": copy_roofline(n1, 19, 4), "fp16_192x96x96_gbs": copy_roofline(n1, 19, 2),
             "fp32_256x128x128_gbs": copy_roofline(n4, 19, 4), "fp16_256x128x128_gbs": copy_roofline(n4, 19, 2),
-            "d2q9_fp32_7430144_gbs": copy_roofline(7430144, 9, 4)}
+            "d2q9_fp32_7430144_gbs": copy_roofline(7430144, 9, 4),
+            "pull_fp32_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 4),
+            "pull_fp16_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 2),
+            "pull_fp32_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 4),
+            "pull_fp16_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 2)}
 
 
 def main():

@@ -59,7 +59,76 @@ static int run_copy(int64_t n, int block, int iters, double* ms_out) {
   return LBM_OK;
 }
 
+// Pull copy: the same traffic with the D3Q19 step's access pattern, every population read from the
+// upstream node x - c_i (the shifted, partly unaligned streams of a pull step), over the rows with a
+// row above and below in y and z (so every address lies in the buffer), one block per row.
+template <class T>
+__global__ __launch_bounds__(1024) void pull_copy_kernel(const T* __restrict__ src, T* __restrict__ dst,
+                                                         const uint32_t* __restrict__ flags, uint32_t W,
+                                                         uint32_t H, uint32_t N) {
+  const uint32_t x = threadIdx.x;
+  if (x >= W) return;
+  const uint32_t y = 1u + blockIdx.x, z = 1u + blockIdx.y;
+  const uint32_t idx = x + W * (y + H * z);
+  const int64_t WH = int64_t(W) * H;
+  const T k = T(flags[idx]);
+  T v[19];
+#pragma unroll
+  for (int i = 0; i < 19; i++) v[i] = src[int64_t(i) * N + idx - (CX3[i] + int64_t(W) * CY3[i] + WH * CZ3[i])];
+#pragma unroll
+  for (int i = 0; i < 19; i++) dst[size_t(i) * N + idx] = v[i] + k;
+}
+
+template <class T>
+static int run_pull_copy(int W, int H, int D, int iters, double* ms_out, int64_t* cells) {
+  const uint32_t N = uint32_t(W) * uint32_t(H) * uint32_t(D);
+  T *a = nullptr, *b = nullptr;
+  uint32_t* fl = nullptr;
+  const size_t bytes = sizeof(T) * 19 * size_t(N);
+  HIP_TRY(hipMalloc(&a, bytes));
+  HIP_TRY(hipMalloc(&b, bytes));
+  HIP_TRY(hipMalloc(&fl, sizeof(uint32_t) * N));
+  HIP_TRY(hipMemset(a, 0, bytes));
+  HIP_TRY(hipMemset(b, 0, bytes));
+  HIP_TRY(hipMemset(fl, 0, sizeof(uint32_t) * N));
+  hipStream_t st;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  const dim3 grid(uint32_t(H - 2), uint32_t(D - 2));
+  const uint32_t block = (uint32_t(W) + 63u) / 64u * 64u;
+  for (int k = 0; k < 50; k++) pull_copy_kernel<T><<<grid, block, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, W, H, N);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, st));
+  for (int k = 0; k < iters; k++) pull_copy_kernel<T><<<grid, block, 0, st>>>(k & 1 ? b : a, k & 1 ? a : b, fl, W, H, N);
+  HIP_TRY(hipEventRecord(e1, st));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms;
+  *cells = int64_t(W) * (H - 2) * (D - 2);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(fl);
+  return LBM_OK;
+}
+
 }  // namespace lbm
+
+extern "C" int lbm_pull_copy_roofline(int device, int32_t width, int32_t height, int32_t depth, int32_t elem_bytes,
+                                      int32_t iters, double* ms_total, int64_t* cells) {
+  using namespace lbm;
+  if (width < 1 || width > 1024 || height < 3 || depth < 3 || iters <= 0 || !ms_total || !cells ||
+      19ull * uint64_t(width) * uint64_t(height) * uint64_t(depth) >= (1ull << 32))
+    return fail(LBM_ERR_INVALID, "bad pull-copy roofline arguments (width 1-1024, height and depth >= 3)");
+  HIP_TRY(hipSetDevice(device));
+  if (elem_bytes == 4) return run_pull_copy<float>(width, height, depth, iters, ms_total, cells);
+  if (elem_bytes == 2) return run_pull_copy<_Float16>(width, height, depth, iters, ms_total, cells);
+  return fail(LBM_ERR_UNSUPPORTED, "pull-copy roofline: elem_bytes must be 4 or 2");
+}
 
 extern "C" int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t workgroup,
                                  int32_t iters, double* ms_total) {

@@ -178,6 +178,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
                               C.c_int),
+        "lbm_pull_copy_roofline": ([C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
         "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
         "lbm_adjoint_backward": ([P, C.c_double, C.c_void_p, C.c_size_t], C.c_int),
         "lbm_adjoint_info": ([P] + [C.POINTER(C.c_int64)] * 5, C.c_int),
@@ -593,6 +595,15 @@ def copy_roofline(n_cells: int, q: int = 19, elem_bytes: int = 4, iters: int = 2
     _check(lib.lbm_copy_roofline(int(device), int(n_cells), int(q), int(elem_bytes), int(workgroup), int(iters),
                                  C.byref(ms)))
     return n_cells * (q * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
+
+
+def pull_copy_roofline(W: int, H: int, D: int, elem_bytes: int = 4, iters: int = 200, device: int = 0) -> float:
+    """GB/s of the pull-copy kernel (a D3Q19 step's bytes and shifted streams, no arithmetic)."""
+    lib = load_library()
+    ms, cells = C.c_double(), C.c_int64()
+    _check(lib.lbm_pull_copy_roofline(int(device), int(W), int(H), int(D), int(elem_bytes), int(iters), C.byref(ms),
+                                      C.byref(cells)))
+    return cells.value * (19 * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
 
 
 def bytes_per_cell(q: int, precision: int) -> int:

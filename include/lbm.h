@@ -247,6 +247,14 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
  * 256); returns their total milliseconds. The memory-only ceiling of a step. */
 int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t workgroup, int32_t iters,
                       double* ms_total);
+/* No counterpart in the reference: the copy roofline with a pull step's access pattern. A D3Q19 grid of
+ * width x height x depth (width <= 1024), elem_bytes 4 or 2; one launch reads, for every node of the
+ * rows with a neighbour row on both sides in y and z, population i from the upstream node x - c_i and
+ * the node's flag word, and writes the 19 populations at the node: the bytes of a step, no arithmetic,
+ * the shifted streams of the pull. iters launches after 50 untimed ones; *cells receives the nodes
+ * one launch updates. */
+int lbm_pull_copy_roofline(int device, int32_t width, int32_t height, int32_t depth, int32_t elem_bytes,
+                           int32_t iters, double* ms_total, int64_t* cells);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 
