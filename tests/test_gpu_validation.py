@@ -1,8 +1,8 @@
 """The reference's validation suite (src/cases.ts:61-427, src/cases3d.ts:11-81) on the gfx950 solver,
-with the reference's acceptance ranges (cells_amd/validate.py restates each case unchanged)."""
+with the reference's acceptance ranges (tests/validation/validate.py restates each case unchanged)."""
 import pytest
 
-from cells_amd import validate
+from tests.validation import validate
 
 pytestmark = pytest.mark.gpu
 

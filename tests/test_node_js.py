@@ -78,7 +78,7 @@ def test_validation_suite_from_node():
         pytest.skip("node is not installed")
     if not os.path.exists(ADDON):
         pytest.skip("the N-API addon has not been built")
-    script = os.path.join(os.path.dirname(HERE), "cells_amd", "node", "validate.js")
+    script = os.path.join(HERE, "validation", "validate.js")
     r = subprocess.run([node, script], capture_output=True, text=True, timeout=240)
     print(r.stdout, r.stderr)
     results = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]

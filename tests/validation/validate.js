@@ -6,10 +6,10 @@
  * ranges (fixed before a case runs, AGENTS.md:90). A case resolves to
  * { name, metrics: [{ name, value, ref, accept, pass }], pass, steps, cells, seconds, notes }.
  *
- *   node cells_amd/node/validate.js [case ...]     (default: every case; one JSON line per case)
+ *   node tests/validation/validate.js [case ...]     (default: every case; one JSON line per case)
  */
-const { Solver, Solver3D, XMode, YMode } = require('./solver');
-const g = require('./geometry');
+const { Solver, Solver3D, XMode, YMode } = require('../../cells_amd/node/solver');
+const g = require('../../cells_amd/node/geometry');
 
 /** metric (src/cases.ts:31-33). */
 function metric(name, value, ref, accept) {

@@ -6,7 +6,7 @@ AGENTS.md:90); the solver underneath is the gfx950 library through cells_amd.lbm
 {"name", "metrics": [{"name", "value", "ref", "accept", "pass"}], "pass", "steps", "cells",
 "seconds", "notes"}.
 
-Run from the repository root: python -m cells_amd.validate [case ...]   (default: every case)
+Run from the repository root: python -m tests.validation.validate [case ...]   (default: every case)
 """
 from __future__ import annotations
 
@@ -17,9 +17,9 @@ import time
 
 import numpy as np
 
-from . import geometry as g
-from .analysis import mean, period_from_crossings, primary_vortex, stream_function
-from .lbm import X_OPEN, X_PERIODIC, X_WALL, Y_MOVING, Y_NOSLIP, Y_PERIODIC, Y_SLIP, Solver, Solver3D
+from cells_amd import geometry as g
+from cells_amd.analysis import mean, period_from_crossings, primary_vortex, stream_function
+from cells_amd.lbm import X_OPEN, X_PERIODIC, X_WALL, Y_MOVING, Y_NOSLIP, Y_PERIODIC, Y_SLIP, Solver, Solver3D
 
 
 def metric(name, value, ref, accept):
