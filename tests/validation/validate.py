@@ -324,7 +324,7 @@ def sphere_fp16():
     """src/cases3d.ts:11-81: sphere C_D at Re 100 in FP32 and FP16 storage, 192 x 96 x 96."""
     W, H, D = 192, 96, 96
     d, u = 16, 0.1
-    from .analysis import derive_tau
+    from cells_amd.analysis import derive_tau
 
     tau = derive_tau(100, u, d).tau
     sdf = g.empty_sdf3(W, H, D)
