@@ -387,7 +387,7 @@ int lbm_batch_launch_info(lbm_batch* b, int32_t* tile, int32_t* order) {
 }
 
 // Times every tile width with both tunnel orders over trial_steps launches (after two untimed ones;
-// every candidate twice, interleaved, keeping its faster time) on the tunnels' own state, keeps the
+// every candidate three times, interleaved, keeping its fastest time) on the tunnels' own state, keeps the
 // fastest, and restores every tunnel's populations and force ring (the batch's and tunnels' step
 // counters are not advanced). Neither the tile width nor the order changes a result: cells are
 // independent within a step and each force sample is summed per wave of body nodes in node order.
@@ -460,7 +460,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   const std::pair<uint32_t, int> cand[] = {{64, 0}, {128, 0}, {256, 0}, {512, 0}, {64, 1}, {128, 1}, {256, 1}, {512, 1}};
   const size_t nc = sizeof(cand) / sizeof(cand[0]);
   std::vector<double> best_of(nc, 1e300);
-  for (int rep = 0; rep < 2; rep++) {
+  for (int rep = 0; rep < 3; rep++) {
     for (size_t c = 0; c < nc; c++) {
       b->tile = cand[c].first;
       b->order = cand[c].second;

@@ -78,7 +78,7 @@ def run_batched(cases, steps: int, chunk: int = 4096, ramp_steps: float = 3000, 
     if launch is not None:
         b.set_launch(*launch)
     elif autotune:
-        b.autotune(10)
+        b.autotune(40)
     traces = [([], []) for _ in cases]
     elapsed = 0.0
     done = 0
