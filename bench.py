@@ -62,6 +62,18 @@ def tunnel_sdf(W, H, D):
     return sdf
 
 
+# Launch shapes pinned from the command line (--launch / --batch-launch) instead of autotuned, so that
+# counter passes, which slow every dispatch and can change the autotuner's choice, measure one shape.
+PIN = {"launch": None, "batch": None}
+
+
+def tune_or_pin(s, trial_steps):
+    if PIN["launch"] is not None and s.Q == 19:
+        s.set_launch(*PIN["launch"])
+        return s.launch_info()[0]
+    return s.autotune(trial_steps)
+
+
 def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
     from cells_amd.lbm import Solver3D, bytes_per_cell
 
@@ -69,7 +81,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
     s.set_sdf(tunnel_sdf(W, H, D))
     s.init_field()
     s.set_inlet_ramp("smoothstep", 0.1, 3000)
-    block = s.autotune(200)
+    block = tune_or_pin(s, 200)
     if warmup:
         s.time_steps(warmup)
     s.sync()
@@ -123,7 +135,7 @@ def d2q9_config0(steps=5000):
         s.set_sdf(sdf)
         s.init_field()
         s.set_inlet_ramp("linear", 0.1, 3000)
-        block = s.autotune(200)
+        block = tune_or_pin(s, 200)
         s.time_steps(500)
         ms = s.time_steps(steps)
         s.destroy()
@@ -228,7 +240,7 @@ def sweep_bench(steps):
 
     cases = sweep.sweep_cases(64, seed=0)
     ib, iseq = {}, {}
-    tb, rb = sweep.run_batched(cases, steps, info=ib)
+    tb, rb = sweep.run_batched(cases, steps, info=ib, launch=PIN["batch"])
     ts, rs = sweep.run_sequential(cases, steps, info=iseq)
     cmp = sweep.compare_runs(cases, rb, rs, min(2000, steps // 2))
     # the device-side divergence records agree with the traces and between the two runs
@@ -373,7 +385,7 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
     sdf0, dsdf = sdf_of(coeffs)
     s = Solver3D.create(cfg("fp16"))
     s.set_sdf(sdf0)
-    s.autotune(20)
+    tune_or_pin(s, 20)
     s.set_inlet_ramp("smoothstep", u, 1000)
 
     def plain_forward():
@@ -497,7 +509,13 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 2D and 3D shape-gradient runs")
     ap.add_argument("--only", choices=sorted(ONLY), help="run one workload and print its own JSON line "
                     "(for kernel traces and counter passes of that workload)")
+    ap.add_argument("--launch", help="pin the D2Q9 / D3Q19 launch shape: BLOCK,VARIANT (lbm_set_launch)")
+    ap.add_argument("--batch-launch", help="pin the batch launch shape: TILE,ORDER (lbm_batch_set_launch)")
     args = ap.parse_args()
+    if args.launch:
+        PIN["launch"] = tuple(int(v) for v in args.launch.split(","))
+    if args.batch_launch:
+        PIN["batch"] = tuple(int(v) for v in args.batch_launch.split(","))
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
               f"across GPUs; see the sharding design notes).", file=sys.stderr)
