@@ -647,7 +647,7 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials,
     const int order = s->sweep & 3;
     const bool descend = order == 1 ? (src > dst) : order == 2 ? (src < dst) : false;
     launch_step3(s->prec, src, dst, s->flags, lt, s->p3, fin, s->xtab, fs, s->bulk_block, descend,
-                 (s->sweep & SWEEP_MERGE) != 0, s->row_clean,
+                 (s->sweep & SWEEP_MERGE) != 0, (s->sweep & SWEEP_EARLY) != 0, s->row_clean,
                  s->stream, rec);
   }
   s->parity ^= 1;
@@ -1210,13 +1210,13 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 }
 
 // Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
-// for this W (and, in 3D, the three row orders and two store variants) is timed over trial_steps
+// for this W (and, in 3D, the three row orders, two store variants and two pull orders) is timed over trial_steps
 // steps (after two warm-up steps) on a copy of the solver's own state, the fastest is kept, and the
 // populations, step, parity and force ring are restored. Results are cached per (lattice,
 // precision, W, H, D, open-face kind, subgrid model on/off, force cadence) for the life of the
-// process. Block width and row order do not change a result
-// (cells are independent within a step and the force sum is ordered by node, not by block); the
-// store variants are separate compilations and may differ in the last bits.
+// process. Block width and row order do not change a result (cells are independent within a step
+// and the force sum is ordered by node, not by block); the store variants, and the pull orders of
+// the shared-store variant, are separate compilations and may differ in the last bits.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
   if (int rc = check_usable(s)) return rc;
@@ -1315,7 +1315,9 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     tried.push_back(bx);
     if (s->nd == 3) {
       for (int sw : {1, 2, 0})
-        for (int mg : {0, SWEEP_MERGE}) cand.push_back({bx, sw | mg});
+        for (int mg : {0, SWEEP_MERGE})
+          for (int ea : {0, SWEEP_EARLY})
+            if (!(ea && smag)) cand.push_back({bx, sw | mg | ea});  // the subgrid-model kernels have one pull order
     } else {
       cand.push_back({bx, s->sweep});
     }
@@ -1367,8 +1369,8 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   const int bb = bulk_block_for(block);
   if (bb < 0 || block <= 0) return fail(LBM_ERR_INVALID, "block must be a multiple of 64 up to 1024");
-  if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE) || (sweep & 3) == 3) : sweep != 0)
-    return fail(LBM_ERR_INVALID, "variant must be 0-2 or 4-6 (3D), 0 (2D)");
+  if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE | SWEEP_EARLY) || (sweep & 3) == 3) : sweep != 0)
+    return fail(LBM_ERR_INVALID, "variant must be 0-2, 4-6, 8-10 or 12-14 (3D), 0 (2D)");
   if (s->Q == 27) return fail(LBM_ERR_UNSUPPORTED, "D3Q27 has one launch shape");
   s->bulk_block = bb;
   if (s->nd == 3) s->sweep = sweep;

@@ -59,11 +59,13 @@ __device__ __forceinline__ void put_moments(void* mom, uint32_t idx, float drho,
 // One time step in one launch: bulk rows (plain fluid, slip-face rows and plain inlet/outlet nodes,
 // one thread per cell) and boundary-list blocks (every other non-solid node, plus the force sample).
 // descend: walk the rows top-down this step (alternated step by step for cache reuse). merge: the
-// inlet and outlet nodes share their wave's stores (same values either way). row_clean: one byte per
-// row (y + H z), 1 where the row's node kinds follow from x alone.
+// inlet and outlet nodes share their wave's stores (same values either way). early: the pulls are
+// issued before the node's kind is known (same values; not for recording or subgrid-model steps).
+// row_clean: one byte per row (y + H z), 1 where the row's node kinds follow from x alone.
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  bool descend, bool merge, const uint8_t* row_clean, hipStream_t st, const Record* rec = nullptr);
+                  bool descend, bool merge, bool early, const uint8_t* row_clean, hipStream_t st,
+                  const Record* rec = nullptr);
 // Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
 void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
 uint32_t bulk_block_x(int W, int requested);

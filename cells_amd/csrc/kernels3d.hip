@@ -310,7 +310,15 @@ __device__ __forceinline__ void list_part3(const typename S::elem* __restrict__ 
 // shares one store per population (whole lines per store instead of a line with a one-node hole
 // plus a one-node store, which costs read-modify-writes where the lines go to HBM rather than stay
 // in cache). Both give the same values; the autotuner picks per grid.
-template <class S, bool REC, bool SMAG, bool MERGE, bool ZH>
+//
+// EARLY issues the 19 pulls before the node's kind is known: the row's clean byte (and, on a row
+// that is not clean, the node's flag word, a second dependent load) is then waited for while the
+// pulls are in flight instead of ahead of them. Every node pulls, also one that turns out to be
+// handled elsewhere (boundary list, solid) or to be the reference's inlet plane, which use none of
+// the values. Each node computes what it does in the other order (bit for bit with per-node stores;
+// with MERGE the compiler may fuse different multiply-adds); which order is faster depends on the
+// grid (measured both ways), so the autotuner picks.
+template <class S, bool REC, bool SMAG, bool MERGE, bool ZH, bool EARLY = false>
 __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
@@ -338,10 +346,55 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     if (outY || outZ) M |= 1u << i;
   }
   const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
+  const size_t N = p.N;
+  const uint32_t off = idx * uint32_t(sizeof(E));
+  const char* const sb = reinterpret_cast<const char*>(src);
+  float f[19];
+  // The 19 pulls; at_outlet selects the outlet copy rule for the directions entering from x = W.
+  auto pull = [&](bool at_outlet) {
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+      const ptrdiff_t delta = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W + CX3[i];
+      const bool outY = (loY && CY3[i] > 0) || (hiY && CY3[i] < 0);
+      const bool outZ = (loZ && CZ3[i] > 0) || (hiZ && CZ3[i] < 0);
+      int sd = i;
+      ptrdiff_t sdelta = delta;
+      if (outY && outZ) {
+        sd = opp3(i);
+        sdelta = 0;
+      } else if (outY) {
+        sd = mirror_y3(i);
+        sdelta = ptrdiff_t(CZ3[i]) * p.WH + CX3[i];
+      } else if (outZ) {
+        sd = mirror_z3(i);
+        sdelta = ptrdiff_t(CY3[i]) * p.W + CX3[i];
+      }
+      const char* base = sb + (ptrdiff_t(sd) * ptrdiff_t(N) - sdelta) * ptrdiff_t(sizeof(E));
+      uint32_t o = off;
+      // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z),
+      // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements (rows off the faces
+      // only, where base is the plain one).
+      if (CX3[i] < 0 && at_outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
+      f[i] = S::load_at(base, o);
+    }
+  };
+  const uint8_t* const clean_byte = row_clean + (y + uint32_t(p.H) * z);
+  uint8_t clean = 0;
+  if constexpr (EARLY) {
+    // Only x = W-1 off the faces can hold an outlet node (face-row outlet nodes are on the boundary
+    // list). Every address lies inside the population buffer: a pull that leaves the grid through
+    // x = 0 or W-1 reads the neighbouring row's end of the same population, at worst element -1 or
+    // N of it, the previous / next population's end; the directions with c_x != 0 are numbered 1-16,
+    // so that stays within elements 0 .. 19 N - 1.
+    clean = *clean_byte;
+    pull(!face && x == uint32_t(p.W - 1));
+  } else {
+    clean = *clean_byte;
+  }
   // A clean row (no solid or boundary-list node besides the outlet node of a face row) is known
   // from its position alone, so its flag words are not read.
   uint32_t fl;
-  if (row_clean[y + uint32_t(p.H) * z])
+  if (clean)
     fl = x == 0u ? (FLAG_INLET3 | INLET_DIRS3 | M)
                  : x == uint32_t(p.W - 1) ? (face ? FLAG_LIST3 : (FLAG_OUTLET3 | OUTLET_DIRS3)) : M;
   else
@@ -349,8 +402,6 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   const bool inlet = fl == (FLAG_INLET3 | INLET_DIRS3 | M);
   const bool outlet = !face && fl == (FLAG_OUTLET3 | OUTLET_DIRS3);
   if (fl != M && !inlet && !outlet) return;  // handled by the boundary list (or solid)
-  const size_t N = p.N;
-  const uint32_t off = idx * uint32_t(sizeof(E));
   char* const d = reinterpret_cast<char*>(dst);
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
   constexpr bool zh = ZH;  // Zou-He faces (p.faces != 0), compiled apart so the default kernels carry none of it
@@ -361,34 +412,8 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     return;
   }
   // (MERGE or Zou-He faces: the inlet node's c_x = +1 pulls reach x = -1, the previous row's last
-  // node: in bounds, never used.)
-  const char* const sb = reinterpret_cast<const char*>(src);
-  float f[19];
-#pragma unroll
-  for (int i = 0; i < 19; i++) {
-    const ptrdiff_t delta = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W + CX3[i];
-    const bool outY = (loY && CY3[i] > 0) || (hiY && CY3[i] < 0);
-    const bool outZ = (loZ && CZ3[i] > 0) || (hiZ && CZ3[i] < 0);
-    int sd = i;
-    ptrdiff_t sdelta = delta;
-    if (outY && outZ) {
-      sd = opp3(i);
-      sdelta = 0;
-    } else if (outY) {
-      sd = mirror_y3(i);
-      sdelta = ptrdiff_t(CZ3[i]) * p.WH + CX3[i];
-    } else if (outZ) {
-      sd = mirror_z3(i);
-      sdelta = ptrdiff_t(CY3[i]) * p.W + CX3[i];
-    }
-    const char* base = sb + (ptrdiff_t(sd) * ptrdiff_t(N) - sdelta) * ptrdiff_t(sizeof(E));
-    uint32_t o = off;
-    // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z),
-    // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements (rows off the faces
-    // only, where base is the plain one).
-    if (CX3[i] < 0 && outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
-    f[i] = S::load_at(base, o);
-  }
+  // node: in bounds, replaced by the Zou-He rule or unused.)
+  if constexpr (!EARLY) pull(outlet);
   if (zh && inlet) zou_he_face3<true>(f, p.uIn);
   else if (zh && outlet) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
@@ -669,7 +694,7 @@ uint32_t bulk_block_x(int W, int requested) {
 template <class S>
 static void step3_t(const void* src, void* dst, const uint32_t* flags, const ListTables& lt, const Params3& p,
                     const float* feq_in, const float4* xtab, const ForceSink& fs, int bb, bool descend, bool merge,
-                    const uint8_t* row_clean, hipStream_t st, const Record* rec) {
+                    bool early, const uint8_t* row_clean, hipStream_t st, const Record* rec) {
   using E = typename S::elem;
   FeqIn fin;
   for (int i = 0; i < 19; i++) fin.v[i] = feq_in[i];
@@ -684,48 +709,52 @@ static void step3_t(const void* src, void* dst, const uint32_t* flags, const Lis
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   const Record none{nullptr, nullptr};
-  if (rec && merge)  // recording steps (the adjoint) never use the subgrid model or the Zou-He faces
-    step3_kernel<S, true, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                    *rec, desc, row_clean);
+#define LBM_STEP3(...) \
+  __VA_ARGS__<<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs, rec ? *rec : none, desc, row_clean)
+  // Recording steps (the adjoint) never use the subgrid model or the Zou-He faces; the early-pull
+  // order is compiled for the kernels without recording and without the subgrid model.
+  if (rec && merge)
+    LBM_STEP3(step3_kernel<S, true, false, true, false>);
   else if (rec)
-    step3_kernel<S, true, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                     *rec, desc, row_clean);
-  else if (p.faces && p.smagC2 > 0.0f)
-    step3_kernel<S, false, true, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                    none, desc, row_clean);
-  else if (p.faces && merge)
-    step3_kernel<S, false, false, true, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                    none, desc, row_clean);
-  else if (p.faces)
-    step3_kernel<S, false, false, false, true><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                     none, desc, row_clean);
+    LBM_STEP3(step3_kernel<S, true, false, false, false>);
+  else if (p.smagC2 > 0.0f && p.faces)
+    LBM_STEP3(step3_kernel<S, false, true, false, true>);
   else if (p.smagC2 > 0.0f)
-    step3_kernel<S, false, true, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                     none, desc, row_clean);
-  else if (merge)
-    step3_kernel<S, false, false, true, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                     none, desc, row_clean);
-  else
-    step3_kernel<S, false, false, false, false><<<grid, bx, 0, st>>>(s0, d0, flags, xtab, p, fin, lt, zl, nlb, nbb, fs,
-                                                                      none, desc, row_clean);
+    LBM_STEP3(step3_kernel<S, false, true, false, false>);
+  else if (p.faces && merge) {
+    if (early)
+      LBM_STEP3(step3_kernel<S, false, false, true, true, true>);
+    else
+      LBM_STEP3(step3_kernel<S, false, false, true, true>);
+  } else if (p.faces) {
+    if (early)
+      LBM_STEP3(step3_kernel<S, false, false, false, true, true>);
+    else
+      LBM_STEP3(step3_kernel<S, false, false, false, true>);
+  } else if (merge) {
+    if (early)
+      LBM_STEP3(step3_kernel<S, false, false, true, false, true>);
+    else
+      LBM_STEP3(step3_kernel<S, false, false, true, false>);
+  } else {
+    if (early)
+      LBM_STEP3(step3_kernel<S, false, false, false, false, true>);
+    else
+      LBM_STEP3(step3_kernel<S, false, false, false, false>);
+  }
+#undef LBM_STEP3
 }
 
 void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, const ListTables& lt,
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
-                  bool descend, bool merge, const uint8_t* row_clean, hipStream_t st, const Record* rec) {
+                  bool descend, bool merge, bool early, const uint8_t* row_clean, hipStream_t st,
+                  const Record* rec) {
+  const int bb = bulk_block;
   switch (prec) {
-    case 1:
-      step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
-      break;
-    case 2:
-      step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
-      break;
-    case 3:
-      step3_t<StoreF16SR>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
-      break;
-    default:
-      step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bulk_block, descend, merge, row_clean, st, rec);
-      break;
+    case 1: step3_t<StoreF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bb, descend, merge, early, row_clean, st, rec); break;
+    case 2: step3_t<StoreBF16>(src, dst, flags, lt, p, feq_in, xtab, fs, bb, descend, merge, early, row_clean, st, rec); break;
+    case 3: step3_t<StoreF16SR>(src, dst, flags, lt, p, feq_in, xtab, fs, bb, descend, merge, early, row_clean, st, rec); break;
+    default: step3_t<StoreF32>(src, dst, flags, lt, p, feq_in, xtab, fs, bb, descend, merge, early, row_clean, st, rec); break;
   }
 }
 

@@ -49,6 +49,7 @@ struct Ticket {
 }  // namespace lbm
 
 constexpr int SWEEP_MERGE = 4;
+constexpr int SWEEP_EARLY = 8;
 
 struct lbm_solver {
   int nd = 2;  // 2 or 3
@@ -69,7 +70,8 @@ struct lbm_solver {
   int bulk_block = 256;
   // 3D launch variant (autotuned). Bits 0-1, row order: 0 = always bottom-up; 1 / 2 = top-down on
   // the steps that read the buffer at the higher / lower address, bottom-up on the others. Bit 2
-  // (SWEEP_MERGE): inlet and outlet nodes share their wave's stores.
+  // (SWEEP_MERGE): inlet and outlet nodes share their wave's stores. Bit 3 (SWEEP_EARLY): the pulls
+  // are issued before the node's kind is known.
   int sweep = 1;
 
   void* f[2] = {nullptr, nullptr};

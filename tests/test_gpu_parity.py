@@ -357,13 +357,15 @@ def test_d2q9_brush_strokes_match_oracle():
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_d3q19_launch_shapes_give_identical_bits(precision):
     """The launch shapes the autotuner picks from (a row of 150 nodes, so the block widths split it
-    differently): the block width and the row order never change a stored bit or a force sample; the
-    two store variants (inlet and outlet nodes in a branch of their own, or sharing their wave's
-    stores) are compiled into different branch structures, so the compiler's choice of fused
-    multiply-adds can differ between them, and their results agree to rounding (the FP32 parity
-    bounds) rather than bit for bit."""
+    differently): the block width and the row order never change a stored bit or a force sample, nor
+    does the pull order (bit 3: loads issued before the node kind is known) with the per-node stores;
+    the two store variants (inlet and outlet nodes in a branch of their own, or sharing their wave's
+    stores), and the shared-store variant in the two pull orders, are compiled into different branch
+    structures, so the compiler's choice of fused multiply-adds can differ between them, and their
+    results agree to rounding (the FP32 parity bounds) rather than bit for bit."""
     cfg, sdf = case3(precision, W=150)
-    shapes = ((64, 0), (128, 0), (256, 0), (64, 1), (64, 2), (64, 4), (128, 5), (256, 6))
+    shapes = ((64, 0), (128, 0), (256, 0), (64, 1), (64, 2), (64, 4), (128, 5), (256, 6), (64, 8), (128, 9),
+              (64, 12), (256, 14))
     outs = {}
     for block, variant in shapes:
         s = Solver3D.create(dict(cfg))
@@ -375,17 +377,18 @@ def test_d3q19_launch_shapes_give_identical_bits(precision):
         s.destroy()
     report = {}
     for k, v in outs.items():
-        ref = outs[(64, 4)] if k[1] & 4 else outs[(64, 0)]
+        ref = outs[(64, 12)] if k[1] & 12 == 12 else outs[(64, 4)] if k[1] & 4 else outs[(64, 0)]
         report[k] = (int(np.count_nonzero(v[0] != ref[0])), int(np.count_nonzero(v[1] != ref[1])))
     print(report)
     assert all(v == (0, 0) for v in report.values()), report
-    a, b = outs[(64, 0)], outs[(64, 4)]
     tol = TOL32[100] if precision == "fp32" else TOL16[100]
-    assert _err(a[2], b[2]) <= tol
-    assert np.abs(a[1] - b[1]).max() <= 1e-4 * np.abs(a[1]).max()
+    for other in ((64, 4), (64, 12)):
+        a, b = outs[(64, 0)], outs[other]
+        assert _err(a[2], b[2]) <= tol
+        assert np.abs(a[1] - b[1]).max() <= 1e-4 * np.abs(a[1]).max()
 
 
-@pytest.mark.parametrize("precision,variant", [("fp32", 0), ("fp32", 4), ("fp16", 1)])
+@pytest.mark.parametrize("precision,variant", [("fp32", 0), ("fp32", 4), ("fp16", 1), ("fp32", 12), ("fp16", 9)])
 def test_d3q19_zou_he_faces_match_oracle(precision, variant):
     """Hecht-Harting Zou-He inlet and outlet faces (lbm_set_open_faces): GPU against the oracle to the
     bounds of the default planes, in both store variants, with a body two cells from the inlet plane
