@@ -9,9 +9,13 @@
 // The summation order is fixed, so a sample is bit-reproducible run to run.
 //
 // Publication uses device-scope (agent) atomic stores and loads, which are coherent across the
-// XCDs' separate L2 caches, and a wait for the stores' completion before the counter increment.
-// A device-scope release fence would also be correct but writes back every dirty L2 line of the
-// issuing XCD, i.e. much of what the step just stored, once per block.
+// XCDs' separate L2 caches, and a wait for the stores' completion (s_waitcnt vmcnt(0)) before the
+// counter increment. This ordering rests on the hardware's wait, not on a release/acquire pair of
+// the HIP memory model: a device-scope release fence would say it in the model's terms but writes
+// back every dirty L2 line of the issuing XCD, i.e. much of what the step just stored, once per
+// block. tests/test_gpu_parity.py::test_force_reduction_across_many_blocks runs the multi-block path
+// over 13 to 54 blocks and checks every sample against the oracle and for identical bits in two
+// runs, so a partial read before it landed would show there.
 #pragma once
 #include "lbm_device.hpp"
 

@@ -492,3 +492,42 @@ def test_d3q27_uniform_flow_and_unsupported_paths():
     with pytest.raises(LbmError):
         s.set_launch(64, 1)
     s.destroy()
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_force_reduction_across_many_blocks(block):
+    """The multi-block path of the force reduction (reduce.hpp force_sample: every wave publishes an
+    FP64 partial with device-scope stores, waits for them to land, and the last block to bump the
+    completion counter sums all partials in wave order): a sphere of radius 14 has a few thousand
+    body-link nodes, i.e. 13 to 54 blocks of them at 256 and 64 threads per block. Samples equal the oracle's to the FP32
+    bounds of the other force checks at every step, and two solvers give the same bits at every
+    sample (a partial read before it landed, or a counter that did not reset, would change a sum or
+    drop a sample). The small parity cases fit one block and use the one-block path."""
+    W, H, D = 64, 48, 48
+    cfg = dict(width=W, height=H, depth=D, precision="fp32", tau=0.56, uIn=0.05, spongeFraction=0.15,
+               spongeTau=1, absorb=0.02, smagorinsky=0.0)
+    sdf = g.empty_sdf3(W, H, D)
+    g.add_sphere(sdf, W, H, D, 24.3, 23.6, 24.2, 14.0)
+    sims = [Solver3D.create(dict(cfg)) for _ in range(2)]
+    o32 = OracleSolver3D(dict(cfg), 32)
+    for s in sims + [o32]:
+        s.set_sdf(sdf)
+        s.init_field()
+    for s in sims:
+        s.set_launch(block, 0)
+    n_list = sims[0].node_counts()[1]
+    assert n_list > 10 * block, n_list  # many blocks of boundary-list nodes
+    for s in sims:
+        s.run(40)
+    o32.steps(40)
+    (sa, va), (sb, vb) = sims[0].read_forces_raw(), sims[1].read_forces_raw()
+    so, vo = o32.read_forces_raw()
+    np.testing.assert_array_equal(sa, so)
+    np.testing.assert_array_equal(sa, sb)
+    np.testing.assert_array_equal(va.view(np.uint64), vb.view(np.uint64))
+    rel = np.abs(va - vo).max() / np.abs(vo).max()
+    print(f"block {block}: {n_list} list nodes, {len(sa)} samples, max rel force diff vs oracle {rel:.2e}")
+    assert rel < 1e-4
+    for s in sims:
+        s.destroy()
+    o32.destroy()
