@@ -59,6 +59,8 @@ declare class SolverBase {
   readFlags(): Uint32Array;
   timeSteps(n: number): number;
   autotune(trialSteps?: number): { block: number; sweep: number };
+  /** First step whose force sample was non-finite since initField / resetForces, 0 while none was. */
+  divergedAt(): number;
   destroy(): void;
 }
 

@@ -298,6 +298,19 @@ static napi_value js_autotune(napi_env env, napi_callback_info info) {
   return out;
 }
 
+/* divergedAt(handle) -> number: lbm_diverged (waits for the work enqueued before it) */
+static napi_value js_diverged_at(napi_env env, napi_callback_info info) {
+  napi_value argv[1], out;
+  if (!get_args(env, info, 1, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  int64_t first = 0;
+  int rc = lbm_diverged(s, &first);
+  if (rc) return throw_status(env, rc);
+  CHECK_NAPI(env, napi_create_int64(env, first, &out));
+  return out;
+}
+
 static napi_value js_rebuild_flags(napi_env env, napi_callback_info info) {
   napi_value argv[1];
   if (!get_args(env, info, 1, argv)) return NULL;
@@ -735,6 +748,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("deviceName", js_device_name),
       EXPORT("copyRoofline", js_copy_roofline),
       EXPORT("autotune", js_autotune),
+      EXPORT("divergedAt", js_diverged_at),
       EXPORT("brush", js_brush),
       EXPORT("brushStroke", js_brush_stroke),
       EXPORT("initField", js_init_field),

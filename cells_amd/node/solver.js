@@ -157,6 +157,15 @@ class Base {
   timeSteps(n) {
     return addon().timeSteps(this.handle, n);
   }
+  /**
+   * hasDiverged (src/tunnel3d.ts:97-99) kept on the device: the number of the first step whose force
+   * sample had a non-finite component since the last initField / resetForces, 0 while none had.
+   * Checked as the samples are written, so a blow-up between two readForces calls is not missed.
+   * Waits for the work enqueued before it.
+   */
+  divergedAt() {
+    return addon().divergedAt(this.handle);
+  }
   /** Picks the fastest launch shape for this grid (results do not change); returns { block, sweep }. */
   autotune(trialSteps) {
     return addon().autotune(this.handle, trialSteps === undefined ? 200 : trialSteps);

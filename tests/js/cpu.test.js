@@ -101,7 +101,7 @@ test('addon exports every entry point', () => {
   const a = m.addon();
   for (const k of ['create2d', 'create3d', 'destroy', 'update2d', 'update3d', 'setSdf', 'initField', 'resetForces',
     'enqueueSteps', 'enqueueMacro', 'syncAsync', 'readMacroAsync', 'readForcesAsync', 'readPopulations', 'readFlags',
-    'info', 'timeSteps', 'setInletRamp', 'adjointForwardAsync', 'adjointBackwardAsync', 'rebuildFlags', 'setOpenFaces', 'brush', 'brushStroke', 'deviceName', 'copyRoofline', 'autotune']) assert.strictEqual(typeof a[k], 'function', k);
+    'info', 'timeSteps', 'setInletRamp', 'adjointForwardAsync', 'adjointBackwardAsync', 'rebuildFlags', 'setOpenFaces', 'brush', 'brushStroke', 'deviceName', 'copyRoofline', 'autotune', 'divergedAt']) assert.strictEqual(typeof a[k], 'function', k);
   assert.strictEqual(a.abiVersion(), 2);
 });
 

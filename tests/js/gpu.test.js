@@ -24,7 +24,28 @@ async function main() {
   const mac = await s3.readMacro();
   assert.strictEqual(mac.length, W * H * D * 4);
   assert.ok(!m.fieldDiverged(mac));
+  assert.strictEqual(s3.divergedAt(), 0);
   console.log(`ok 3d run: C_D-like fx ${f[49].fx.toExponential(4)}`);
+
+  // --- blow-up record (hasDiverged kept on the device): a 2D tunnel driven unstable records the step
+  // of its first non-finite force sample, the same step hasDiverged finds in the samples
+  {
+    const u = await m.Solver.create(0, { width: 64, height: 24, left: m.XMode.Open, right: m.XMode.Open,
+      bottom: m.YMode.Slip, top: m.YMode.Slip, tau: 0.5005, uIn: 0.3, spongeFraction: 0, absorb: 0, forceEvery: 1 });
+    const sdf2 = m.emptySdf(64, 24);
+    m.addCircle(sdf2, 64, 24, 16, 11.5, 4);
+    u.setSdf(sdf2);
+    u.initField();
+    await u.run(300, 100);
+    const samples = await u.readForces();
+    assert.ok(m.hasDiverged(samples));
+    const firstBad = samples.find((q) => !Number.isFinite(q.fx) || !Number.isFinite(q.fy)).step;
+    assert.strictEqual(u.divergedAt(), firstBad);
+    u.resetForces();
+    assert.strictEqual(u.divergedAt(), 0);
+    u.destroy();
+    console.log(`ok blow-up record at step ${firstBad}`);
+  }
 
   // --- Zou-He faces: the inlet plane carries u = (uIn, 0, 0) (no absorbing source at x = 0)
   s3.setOpenFaces('zouHe');
