@@ -14,6 +14,8 @@
 //     force reduction in FP64 (reduce.hpp).
 // Storage is structure-of-arrays f[i * N + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16;
 // arithmetic is FP32 throughout, as in the reference.
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -271,6 +273,29 @@ __device__ __forceinline__ void collide3(const float (&f)[19], const Mom3& m, co
   }
 }
 
+// The outlet plane's equilibrium at rho = 1 with the node's own velocity (src/shaders/step3d.ts:205-209,
+// feq of common3d.ts:78-81 at rho = 1) for all 19 directions, in the pair form the collision uses:
+// feq_a = w (S + cu), feq_b = w (S - cu), S = cu^2 / 2 - usq, for the pair (a, b) with cu = 3 c_a . u.
+// Algebraically feq3(i, 1, u) (its rounding differs by a few ulps) with fewer live values.
+__device__ __forceinline__ void outlet_feq3(const Mom3& m, float (&out)[19]) {
+  const float u3x = 3.0f * m.ux, u3y = 3.0f * m.uy, u3z = 3.0f * m.uz;
+  out[0] = W3[0] * -m.usq;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const int a = 2 * k + 1;
+    float c = 0.0f;
+    if (CX3[a] > 0) c = c + u3x;
+    if (CX3[a] < 0) c = c - u3x;
+    if (CY3[a] > 0) c = c + u3y;
+    if (CY3[a] < 0) c = c - u3y;
+    if (CZ3[a] > 0) c = c + u3z;
+    if (CZ3[a] < 0) c = c - u3z;
+    const float S = 0.5f * c * c - m.usq;
+    out[a] = W3[a] * (S + c);
+    out[a + 1] = W3[a] * (S - c);
+  }
+}
+
 // Inlet equilibrium feq(i, 1, (uIn, 0, 0)) for all i, computed on the host per step (it is uniform
 // over the inlet plane) and passed by value.
 struct FeqIn {
@@ -420,8 +445,10 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   if (REC && !inlet && rc.mom) put_moments<sizeof(E)>(rc.mom, idx, m.drho, m.ux, m.uy, m.uz);
   if (!MERGE && outlet && !zh) {
     // src/shaders/step3d.ts:205-209: equilibrium at rho = 1 with the node's own velocity.
+    float eq[19];
+    outlet_feq3(m, eq);
 #pragma unroll
-    for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
+    for (int i = 0; i < 19; i++) put_at<S>(d + i * N * sizeof(E), off, eq[i], nw, i);
     return;
   }
   const float4 t = xtab[x];
@@ -433,8 +460,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   float out[19];
   collide3<SMAG>(f, m, r, p, out);
   if (MERGE && outlet && !zh) {
-#pragma unroll
-    for (int i = 0; i < 19; i++) out[i] = feq3(i, 1.0f, m.ux, m.uy, m.uz);
+    outlet_feq3(m, out);
   } else if (MERGE && inlet && !zh) {
 #pragma unroll
     for (int i = 0; i < 19; i++) out[i] = fin.v[i];
@@ -457,6 +483,56 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
 // replaces the gathered f_j with the interpolation against its record's second operand and adds
 // c_j (f_j + f_i) to the force; a slip face mirroring a solid node (step3d.ts:139-146) keeps f_j and
 // adds c_j 2 f_j.
+// The body links of a list node for directions LO .. LO + 8: Bouzidi interpolation of the gathered
+// f_j against the record's second operand (src/shaders/step3d.ts:147-166) and the momentum
+// exchange c_j (f_j + f_i) into force; a slip face mirroring a solid node (step3d.ts:139-146) keeps
+// f_j and adds c_j 2 f_j. Directions are taken in increasing order, as one loop over all would.
+template <class S, bool REC, int LO>
+__device__ __forceinline__ void link_half3(const typename S::elem* __restrict__ src, const ListTables& lt, uint32_t k,
+                                            uint32_t bm, uint32_t xm, const BzLink* __restrict__ rec,
+                                            const Record& rc, float (&f)[19], float (&force)[3],
+                                            std::integral_constant<int, LO>) {
+  float q[9], b[9];
+  uint32_t gb[9];
+#pragma unroll
+  for (int h = 0; h < 9; h++) {
+    const int i = LO + h;
+    q[h] = 1.0f;
+    gb[h] = NO_OPERAND;
+    if (bm & (1u << i)) {
+      const BzLink r = rec[__popc(bm & ((1u << i) - 1u))];
+      q[h] = r.q;
+      gb[h] = r.gb;
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 9; h++) b[h] = gb[h] != NO_OPERAND ? S::load(src, gb[h]) : 0.0f;
+#pragma unroll
+  for (int h = 0; h < 9; h++) {
+    const int i = LO + h;
+    const int j = opp3(i);
+    const float fj = f[i];
+    if (bm & (1u << i)) {
+      const float qi = q[h];
+      float fi = fj;
+      if (qi < 0.5f) {
+        if (gb[h] != NO_OPERAND) fi = 2.0f * qi * fj + (1.0f - 2.0f * qi) * b[h];
+      } else {
+        fi = fj / (2.0f * qi) + (2.0f * qi - 1.0f) / (2.0f * qi) * b[h];
+      }
+      force[0] += float(CX3[j]) * (fj + fi);
+      force[1] += float(CY3[j]) * (fj + fi);
+      force[2] += float(CZ3[j]) * (fj + fi);
+      if (REC && rc.dq) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[h], gb[h] != NO_OPERAND);
+      f[i] = fi;
+    } else if (xm & (1u << i)) {
+      force[0] += float(CX3[j]) * (2.0f * fj);
+      force[1] += float(CY3[j]) * (2.0f * fj);
+      force[2] += float(CZ3[j]) * (2.0f * fj);
+    }
+  }
+}
+
 template <class S, bool BODY, bool REC, bool SMAG, bool ZH>
 __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
@@ -481,51 +557,25 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
     const uint32_t bm = lt.bmask[k];
     const uint32_t xm = lt.xmask[k];
     const BzLink* __restrict__ rec = lt.links + lt.lbase[k];
-    float q[19], b[19];
-    uint32_t gb[19];
-#pragma unroll
-    for (int i = 1; i < 19; i++) {
-      q[i] = 1.0f;
-      gb[i] = NO_OPERAND;
-      if (bm & (1u << i)) {
-        const BzLink r = rec[__popc(bm & ((1u << i) - 1u))];
-        q[i] = r.q;
-        gb[i] = r.gb;
-      }
-    }
-#pragma unroll
-    for (int i = 1; i < 19; i++) b[i] = gb[i] != NO_OPERAND ? S::load(src, gb[i]) : 0.0f;
-#pragma unroll
-    for (int i = 1; i < 19; i++) {
-      const int j = opp3(i);
-      const float fj = f[i];
-      if (bm & (1u << i)) {
-        const float qi = q[i];
-        float fi = fj;
-        if (qi < 0.5f) {
-          if (gb[i] != NO_OPERAND) fi = 2.0f * qi * fj + (1.0f - 2.0f * qi) * b[i];
-        } else {
-          fi = fj / (2.0f * qi) + (2.0f * qi - 1.0f) / (2.0f * qi) * b[i];
-        }
-        force[0] += float(CX3[j]) * (fj + fi);
-        force[1] += float(CY3[j]) * (fj + fi);
-        force[2] += float(CZ3[j]) * (fj + fi);
-        if (REC && rc.dq) rc.dq[lt.lbase[k] + __popc(bm & ((1u << i) - 1u))] = bouzidi_dq(qi, fj, b[i], gb[i] != NO_OPERAND);
-        f[i] = fi;
-      } else if (xm & (1u << i)) {
-        force[0] += float(CX3[j]) * (2.0f * fj);
-        force[1] += float(CY3[j]) * (2.0f * fj);
-        force[2] += float(CZ3[j]) * (2.0f * fj);
-      }
-    }
+    // The links in two halves of the directions (1-9, 10-18), each with its records and second
+    // operands loaded together: a body node waits for two rounds of those loads instead of one and
+    // keeps half the operands in registers. The kernel's registers are sized for its heaviest path,
+    // and this one would otherwise set them for the bulk rows too (with 16-bit storage 7 instead of 8
+    // waves per SIMD). Body nodes are a few thousand of millions and their blocks run first,
+    // alongside the bulk rows.
+    link_half3<S, REC>(src, lt, k, bm, xm, rec, rc, f, force, std::integral_constant<int, 1>{});
+    asm volatile("" ::: "memory");
+    link_half3<S, REC>(src, lt, k, bm, xm, rec, rc, f, force, std::integral_constant<int, 10>{});
   }
   if (zh && (fl & FLAG_INLET3)) zou_he_face3<true>(f, p.uIn);
   else if (zh && (fl & FLAG_OUTLET3)) zou_he_face3<false>(f, p.uIn);
   const Mom3 m = moments3(f);
   if (REC && rc.mom) put_moments<sizeof(typename S::elem)>(rc.mom, idx, m.drho, m.ux, m.uy, m.uz);
   if ((fl & FLAG_OUTLET3) && !zh) {
+    float eq[19];
+    outlet_feq3(m, eq);
 #pragma unroll
-    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, feq3(i, 1.0f, m.ux, m.uy, m.uz), nw, i);
+    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, eq[i], nw, i);
     return;
   }
   const float4 t = xtab[idx % uint32_t(p.W)];
