@@ -483,19 +483,32 @@ ONLY = {
 }
 
 
+def filled_pull_key(variant):
+    """The filled pull-copy entry of copy_rooflines() with the element width and grid of a variant."""
+    prec, size = variant.split("_", 1)
+    return f"filled_pull_{'fp32' if prec == 'fp32' else 'fp16'}_{size}_gbs"
+
+
 def copy_rooflines():
     """Copy-roofline kernels (src/bench.ts:61-95: Q populations and the flag word read, Q written, no
-    arithmetic, aligned streams) and the pull copy (the same bytes with a D3Q19 step's shifted streams)."""
+    arithmetic, aligned streams) and the pull copy (the same bytes with a D3Q19 step's shifted streams).
+    The reference page copies zeroed buffers; the memory system moves zeros faster than the values a
+    step moves, so each is also measured over flow-like values ("filled"), the ceiling for a step."""
     from cells_amd.lbm import copy_roofline, pull_copy_roofline
 
     n1, n4 = 192 * 96 * 96, 256 * 128 * 128
-    return {"fp32_192x96x96_gbs": copy_roofline(n1, 19, 4), "fp16_192x96x96_gbs": copy_roofline(n1, 19, 2),
-            "fp32_256x128x128_gbs": copy_roofline(n4, 19, 4), "fp16_256x128x128_gbs": copy_roofline(n4, 19, 2),
-            "d2q9_fp32_7430144_gbs": copy_roofline(7430144, 9, 4),
-            "pull_fp32_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 4),
-            "pull_fp16_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 2),
-            "pull_fp32_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 4),
-            "pull_fp16_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 2)}
+    out = {}
+    for tag, filled in (("", False), ("filled_", True)):
+        out.update({f"{tag}fp32_192x96x96_gbs": copy_roofline(n1, 19, 4, filled=filled),
+                    f"{tag}fp16_192x96x96_gbs": copy_roofline(n1, 19, 2, filled=filled),
+                    f"{tag}fp32_256x128x128_gbs": copy_roofline(n4, 19, 4, filled=filled),
+                    f"{tag}fp16_256x128x128_gbs": copy_roofline(n4, 19, 2, filled=filled),
+                    f"{tag}d2q9_fp32_7430144_gbs": copy_roofline(7430144, 9, 4, filled=filled),
+                    f"{tag}pull_fp32_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 4, filled=filled),
+                    f"{tag}pull_fp16_192x96x96_gbs": pull_copy_roofline(192, 96, 96, 2, filled=filled),
+                    f"{tag}pull_fp32_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 4, filled=filled),
+                    f"{tag}pull_fp16_256x128x128_gbs": pull_copy_roofline(256, 128, 128, 2, filled=filled)})
+    return out
 
 
 def main():
@@ -577,6 +590,12 @@ def main():
         "fp16sr_over_fp32": variants.get("fp16sr_192x96x96", main_run)["mlups"] / main_run["mlups"],
         "bulk_block": main_run["bulk_block"],
         "copy_roofline": copy,
+        # each step against the pull copy of its size and element width over flow-like values (the
+        # same bytes and shifted streams with no arithmetic)
+        "step_over_filled_pull_copy": {
+            k: variants[k]["algorithmic_gbs"] / copy[filled_pull_key(k)]
+            for k in ("fp32_192x96x96", "fp16_192x96x96", "fp16sr_192x96x96", "bf16_192x96x96", "fp32_256x128x128",
+                      "fp16_256x128x128") if k in variants},
         "sweep": sweep_res,
         "precision_vs_fp64": precision,
         "d2q9_512x128_cylinder": d2q9,

@@ -178,7 +178,9 @@ def load_library(path: str = LIB_PATH):
         "lbm_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
                               C.c_int),
-        "lbm_pull_copy_roofline": ([C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+        "lbm_copy_roofline_filled": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_double)], C.c_int),
+        "lbm_pull_copy_roofline": ([C.c_int, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int),
         "lbm_adjoint_forward": ([P, C.POINTER(AdjointConfig), C.POINTER(C.c_double)], C.c_int),
         "lbm_adjoint_backward": ([P, C.c_double, C.c_void_p, C.c_size_t], C.c_int),
@@ -589,21 +591,24 @@ def device_name(device: int = 0) -> str:
 
 
 def copy_roofline(n_cells: int, q: int = 19, elem_bytes: int = 4, iters: int = 200, device: int = 0,
-                  workgroup: int = 256) -> float:
-    """GB/s of the copy-roofline kernel: q populations + one flag word read, q populations written."""
+                  workgroup: int = 256, filled: bool = False) -> float:
+    """GB/s of the copy-roofline kernel: q populations + one flag word read, q populations written; over
+    zeroed buffers as the reference page does, or (filled) over flow-like values."""
     lib = load_library()
     ms = C.c_double()
-    _check(lib.lbm_copy_roofline(int(device), int(n_cells), int(q), int(elem_bytes), int(workgroup), int(iters),
-                                 C.byref(ms)))
+    _check(lib.lbm_copy_roofline_filled(int(device), int(n_cells), int(q), int(elem_bytes), int(workgroup),
+                                        int(bool(filled)), int(iters), C.byref(ms)))
     return n_cells * (q * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
 
 
-def pull_copy_roofline(W: int, H: int, D: int, elem_bytes: int = 4, iters: int = 200, device: int = 0) -> float:
-    """GB/s of the pull-copy kernel (a D3Q19 step's bytes and shifted streams, no arithmetic)."""
+def pull_copy_roofline(W: int, H: int, D: int, elem_bytes: int = 4, iters: int = 200, device: int = 0,
+                       filled: bool = True) -> float:
+    """GB/s of the pull-copy kernel (a D3Q19 step's bytes and shifted streams, no arithmetic), over
+    flow-like values (filled, the step's ceiling) or zeroed buffers."""
     lib = load_library()
     ms, cells = C.c_double(), C.c_int64()
-    _check(lib.lbm_pull_copy_roofline(int(device), int(W), int(H), int(D), int(elem_bytes), int(iters), C.byref(ms),
-                                      C.byref(cells)))
+    _check(lib.lbm_pull_copy_roofline(int(device), int(W), int(H), int(D), int(elem_bytes), int(bool(filled)),
+                                      int(iters), C.byref(ms), C.byref(cells)))
     return cells.value * (19 * elem_bytes * 2 + 4) * iters / (ms.value * 1e-3) / 1e9
 
 

@@ -253,10 +253,17 @@ int lbm_copy_roofline(int device, int64_t n_cells, int32_t q, int32_t elem_bytes
  * width x height x depth (width <= 1024), elem_bytes 4 or 2; one launch reads, for every node of the
  * rows with a neighbour row on both sides in y and z, population i from the upstream node x - c_i and
  * the node's flag word, and writes the 19 populations at the node: the bytes of a step, no arithmetic,
- * the shifted streams of the pull. iters launches after 50 untimed ones; *cells receives the nodes
- * one launch updates. */
+ * the shifted streams of the pull. fill 0 copies zeroed buffers (as the reference page's copy does);
+ * fill 1 first fills them with the kind of values a step moves (f - w of a uniform stream plus a
+ * small hashed perturbation), which the memory system moves measurably more slowly than zeros: the
+ * ceiling for a step. iters launches after 50 untimed ones; *cells receives the nodes one launch
+ * updates. */
 int lbm_pull_copy_roofline(int device, int32_t width, int32_t height, int32_t depth, int32_t elem_bytes,
-                           int32_t iters, double* ms_total, int64_t* cells);
+                           int32_t fill, int32_t iters, double* ms_total, int64_t* cells);
+/* No counterpart in the reference: lbm_copy_roofline with the fill choice of lbm_pull_copy_roofline
+ * (fill 0 is lbm_copy_roofline). */
+int lbm_copy_roofline_filled(int device, int64_t n_cells, int32_t q, int32_t elem_bytes, int32_t workgroup,
+                             int32_t fill, int32_t iters, double* ms_total);
 /* Counts of the node classes the step kernels see: plain (bulk), boundary list, solid. */
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid);
 

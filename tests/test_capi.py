@@ -51,8 +51,10 @@ def test_host_only_calls():
     assert lib.lbm_adjoint_forward(None, None, None) != 0
     assert lib.lbm_copy_roofline(0, -1, 19, 4, 0, 1, C.byref(C.c_double())) != 0
     assert lib.lbm_copy_roofline(0, 1024, 19, 4, 96, 1, C.byref(C.c_double())) != 0
-    assert lib.lbm_pull_copy_roofline(0, 2048, 8, 8, 4, 1, C.byref(C.c_double()), C.byref(C.c_int64())) != 0
-    assert lib.lbm_pull_copy_roofline(0, 64, 2, 8, 4, 1, C.byref(C.c_double()), C.byref(C.c_int64())) != 0
+    assert lib.lbm_pull_copy_roofline(0, 2048, 8, 8, 4, 1, 1, C.byref(C.c_double()), C.byref(C.c_int64())) != 0
+    assert lib.lbm_pull_copy_roofline(0, 64, 2, 8, 4, 1, 1, C.byref(C.c_double()), C.byref(C.c_int64())) != 0
+    assert lib.lbm_pull_copy_roofline(0, 64, 8, 8, 4, 2, 1, C.byref(C.c_double()), C.byref(C.c_int64())) != 0
+    assert lib.lbm_copy_roofline_filled(0, 1024, 19, 4, 256, 2, 1, C.byref(C.c_double())) != 0
     assert lib.lbm_enqueue_steps_noalloc(None, 1) != 0
     assert lib.lbm_set_launch(None, 64, 0) != 0
     assert lib.lbm_rebuild_flags(None) != 0
