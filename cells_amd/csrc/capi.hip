@@ -1210,8 +1210,9 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 }
 
 // Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
-// for this W (and, in 3D, the three row orders, two store variants and two pull orders) is timed over trial_steps
-// steps (after two warm-up steps) on a copy of the solver's own state, the fastest is kept, and the
+// for this W (and, in 3D, the three row orders, two store variants and two pull orders) is timed over
+// trial_steps steps (after eight untimed ones; three interleaved rounds, the fastest round counting)
+// on a copy of the solver's own state, the fastest is kept, and the
 // populations, step, parity and force ring are restored. Results are cached per (lattice,
 // precision, W, H, D, open-face kind, subgrid model on/off, force cadence) for the life of the
 // process. Block width and row order do not change a result (cells are independent within a step
@@ -1307,7 +1308,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   std::vector<int> tried;
   int best = s->bulk_block, best_sweep = s->sweep;
   double best_ms = 1e300;
-  // every candidate is timed twice, interleaved with the others, and keeps its faster time
+  // every candidate is timed three times, interleaved with the others, and keeps its fastest time
   std::vector<std::pair<int, int>> cand;
   for (int req = 64; req <= 1024; req += 64) {
     const int bx = int(bulk_block_x(s->W, req));
@@ -1323,12 +1324,13 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     }
   }
   std::vector<double> best_of(cand.size(), 1e300);
-  for (int rep = 0; rep < 2; rep++) {
+  for (int rep = 0; rep < 3; rep++) {
     for (size_t c = 0; c < cand.size(); c++) {
       s->bulk_block = cand[c].first;
       s->sweep = cand[c].second;
-      enqueue_one(s);
-      enqueue_one(s);
+      // untimed steps first: the previous candidate's row order leaves the Infinity Cache holding
+      // other rows than this one's steady state does
+      for (int k = 0; k < 8; k++) enqueue_one(s);
       HIP_TRY(hipEventRecord(g.a, s->stream));
       for (int k = 0; k < trial_steps; k++) enqueue_one(s);
       HIP_TRY(hipGetLastError());
