@@ -634,8 +634,10 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials,
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[27];
     inlet_feq27(s->p3.uIn, fin);
+    const int order = s->sweep & 3;  // row order, as for D3Q19 below
+    const bool descend = order == 1 ? (src > dst) : order == 2 ? (src < dst) : false;
     launch_step27(s->prec, src, dst, s->flags, s->row_clean, s->sdf, s->xtab, s->p3, fin, fs, s->list, s->n_link,
-                  s->stream);
+                  descend, s->stream);
   } else {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[19];
@@ -1210,27 +1212,24 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 }
 
 // Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
-// for this W (and, in 3D, the three row orders, two store variants and two pull orders) is timed over
+// for this W (and, in 3D, the three row orders, two store variants and two pull orders; D3Q27: its one
+// block width in the three row orders) is timed over
 // trial_steps steps (after eight untimed ones; three interleaved rounds, the fastest round counting)
 // on a copy of the solver's own state, the fastest is kept, and the
-// populations, step, parity and force ring are restored. Results are cached per (lattice,
-// precision, W, H, D, open-face kind, subgrid model on/off, force cadence) for the life of the
+// populations, step, parity and force ring are restored. Results are cached per (lattice, velocity
+// set, precision, W, H, D, open-face kind, subgrid model on/off, force cadence) for the life of the
 // process. Block width and row order do not change a result (cells are independent within a step
 // and the force sum is ordered by node, not by block); the store variants, and the pull orders of
 // the shared-store variant, are separate compilations and may differ in the last bits.
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
   if (int rc = check_usable(s)) return rc;
-  if (s->Q == 27) {  // one launch shape (whole rows of at most 256 threads)
-    if (chosen) *chosen = int32_t(step27_block_x(s->W));
-    return LBM_OK;
-  }
   static std::mutex cache_mu;
   static std::map<std::string, std::pair<int, int>> cache;
   // Everything that selects a different kernel instantiation or changes the work of a step is part
   // of the key: lattice, precision, grid, open-face kind, subgrid model on/off, force cadence.
   const bool smag = (s->nd == 2 ? s->cfg2.smagorinsky : s->cfg3.smagorinsky) > 0;
-  const std::string key = std::to_string(s->nd) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
+  const std::string key = std::to_string(s->nd) + ":Q" + std::to_string(s->Q) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
                           std::to_string(s->H) + "x" + std::to_string(s->D) + ":faces" + std::to_string(s->faces) +
                           ":smag" + std::to_string(int(smag)) + ":every" + std::to_string(s->force_every());
   {
@@ -1239,7 +1238,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     if (it != cache.end()) {
       s->bulk_block = it->second.first;
       s->sweep = it->second.second;
-      if (chosen) *chosen = s->bulk_block;
+      if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : s->bulk_block;
       return LBM_OK;
     }
   }
@@ -1310,7 +1309,9 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   double best_ms = 1e300;
   // every candidate is timed three times, interleaved with the others, and keeps its fastest time
   std::vector<std::pair<int, int>> cand;
-  for (int req = 64; req <= 1024; req += 64) {
+  if (s->Q == 27)  // one block width (whole rows of at most 256 threads); the three row orders
+    for (int sw : {1, 2, 0}) cand.push_back({s->bulk_block, sw});
+  for (int req = 64; req <= 1024 && s->Q != 27; req += 64) {
     const int bx = int(bulk_block_x(s->W, req));
     if (std::find(tried.begin(), tried.end(), bx) != tried.end()) continue;
     tried.push_back(bx);
@@ -1356,7 +1357,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     std::lock_guard<std::mutex> lk(cache_mu);
     cache[key] = {best, best_sweep};
   }
-  if (chosen) *chosen = best;
+  if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : best;
   return LBM_OK;
 }
 
@@ -1373,7 +1374,13 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
   if (bb < 0 || block <= 0) return fail(LBM_ERR_INVALID, "block must be a multiple of 64 up to 1024");
   if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE | SWEEP_EARLY) || (sweep & 3) == 3) : sweep != 0)
     return fail(LBM_ERR_INVALID, "variant must be 0-2, 4-6, 8-10 or 12-14 (3D), 0 (2D)");
-  if (s->Q == 27) return fail(LBM_ERR_UNSUPPORTED, "D3Q27 has one launch shape");
+  if (s->Q == 27) {  // one block width; the row order is the only variant
+    if (block != int32_t(step27_block_x(s->W)) || (sweep & ~3) != 0)
+      return fail(LBM_ERR_UNSUPPORTED, "D3Q27: block must be " + std::to_string(step27_block_x(s->W)) +
+                                           " for this width and the variant a row order (0-2)");
+    s->sweep = sweep;
+    return LBM_OK;
+  }
   s->bulk_block = bb;
   if (s->nd == 3) s->sweep = sweep;
   return LBM_OK;

@@ -84,10 +84,10 @@ uint32_t body_list_blocks(uint32_t n_body, uint32_t bx, bool sample);
 // 1 where the row holds no solid or FLAG_SLOW27 node. Partials: 3 doubles per
 // wave index, up to step27_partials(W, H, D) waves; force_waves lists, ascending, the n_force_waves
 // wave indices holding a FLAG_SLOW27 node (wave = (block (z H + y) gx + x / bx) (bx / 64) + x % bx / 64,
-// bx = step27_block_x(W)).
+// bx = step27_block_x(W)). descend: rows walked from the top (z and y reversed) in this step.
 void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean,
                    const float* sdf, const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs,
-                   const uint32_t* force_waves, uint32_t n_force_waves, hipStream_t st);
+                   const uint32_t* force_waves, uint32_t n_force_waves, bool descend, hipStream_t st);
 uint32_t step27_block_x(int W);
 size_t step27_partials(int W, int H, int D);
 void launch_macro27(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);

@@ -260,18 +260,22 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
   return slow;
 }
 
-// Grid (ceil(W / blockDim.x), H, D): x from the block column and thread, y and z from the launch.
+// Grid (ceil(W / blockDim.x), H, D): x from the block column and thread, y and z from the launch,
+// walked from the top (z and y reversed) when descend is set. Alternating the direction every step
+// makes a step begin on the rows the previous one wrote last, which are still in the 256 MB Infinity
+// Cache when the two population buffers do not fit in it together (FP32: 2 x 191 MB at 192x96x96).
 // With a force sample due, every wave holding a FLAG_SLOW27 node writes its FP64 partial at its wave
-// index (block-major over the grid).
+// index (block-major over the grid in cell order, whichever direction the rows are walked in).
 template <class S, bool SMAG>
 __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __restrict__ src,
                                                      typename S::elem* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
                                                      const uint8_t* __restrict__ row_clean, const float* __restrict__ sdf,
                                                      const float4* __restrict__ xtab, Params3 p, FeqIn27 fin,
-                                                     double* __restrict__ partials, int sample) {
+                                                     double* __restrict__ partials, int sample, uint32_t descend) {
   const int x = int(blockIdx.x * blockDim.x + threadIdx.x);
-  const int y = int(blockIdx.y), z = int(blockIdx.z);
+  const int y = descend ? p.H - 1 - int(blockIdx.y) : int(blockIdx.y);
+  const int z = descend ? p.D - 1 - int(blockIdx.z) : int(blockIdx.z);
   float force[3] = {0.0f, 0.0f, 0.0f};
   bool slow = false;
   if (x < p.W) {
@@ -292,7 +296,7 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
   double v[3] = {double(force[0]), double(force[1]), double(force[2])};
   wave_sum<3>(v);
   if ((threadIdx.x & 63u) == 0u) {
-    const size_t block = (size_t(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const size_t block = (size_t(z) * gridDim.y + size_t(y)) * gridDim.x + blockIdx.x;
     const size_t part = block * (blockDim.x >> 6) + (threadIdx.x >> 6);
 #pragma unroll
     for (int d = 0; d < 3; d++) partials[part * 3 + d] = v[d];
@@ -441,7 +445,7 @@ inline uint32_t blocks_for27(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 template <class S>
 void step27_t(const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean, const float* sdf,
               const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs, const uint32_t* force_waves, uint32_t n_force_waves,
-              hipStream_t st) {
+              bool descend, hipStream_t st) {
   FeqIn27 fin;
   for (int i = 0; i < 27; i++) fin.v[i] = feq_in[i];
   using E = typename S::elem;
@@ -450,9 +454,9 @@ void step27_t(const void* src, void* dst, const uint32_t* flags, const uint8_t* 
   const E* s0 = static_cast<const E*>(src);
   E* d0 = static_cast<E*>(dst);
   if (p.smagC2 > 0.0f)
-    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
+    step27_kernel<S, true><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample, descend);
   else
-    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample);
+    step27_kernel<S, false><<<grid, bx, 0, st>>>(s0, d0, flags, row_clean, sdf, xtab, p, fin, fs.partials, fs.sample, descend);
   if (fs.sample) reduce27_kernel<<<1, 64, 0, st>>>(fs.partials, force_waves, n_force_waves, fs.ring, fs.state, fs.step);
 }
 
@@ -467,11 +471,11 @@ size_t step27_partials(int W, int H, int D) {
 
 void launch_step27(int prec, const void* src, void* dst, const uint32_t* flags, const uint8_t* row_clean,
                    const float* sdf, const float4* xtab, const Params3& p, const float* feq_in, const ForceSink& fs,
-                   const uint32_t* force_waves, uint32_t n_force_waves, hipStream_t st) {
+                   const uint32_t* force_waves, uint32_t n_force_waves, bool descend, hipStream_t st) {
   switch (prec) {
-    case 1: step27_t<StoreF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
-    case 2: step27_t<StoreBF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
-    default: step27_t<StoreF32>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, st); break;
+    case 1: step27_t<StoreF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, descend, st); break;
+    case 2: step27_t<StoreBF16>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, descend, st); break;
+    default: step27_t<StoreF32>(src, dst, flags, row_clean, sdf, xtab, p, feq_in, fs, force_waves, n_force_waves, descend, st); break;
   }
 }
 

@@ -476,7 +476,8 @@ def test_d3q27_fields_match_oracle(precision, smag):
 
 def test_d3q27_uniform_flow_and_unsupported_paths():
     """Without a body the inflow equilibrium is a fixed point of the D3Q27 step on the GPU too; the
-    paths written for D3Q19 only (Zou-He faces, pinned launch variants, the adjoint) refuse D3Q27."""
+    paths written for D3Q19 only (Zou-He faces, the store and pull-order variants, other block widths)
+    refuse D3Q27."""
     from cells_amd.lbm import LbmError
 
     cfg = dict(width=40, height=18, depth=14, precision="fp32", tau=0.53, uIn=0.07, spongeFraction=0.3, spongeTau=1,
@@ -490,8 +491,50 @@ def test_d3q27_uniform_flow_and_unsupported_paths():
     with pytest.raises(LbmError):
         s.set_open_faces("zou-he")
     with pytest.raises(LbmError):
-        s.set_launch(64, 1)
+        s.set_launch(64, 4)
+    with pytest.raises(LbmError):
+        s.set_launch(128, 1)
+    s.set_launch(64, 2)
+    assert s.launch_info() == (64, 2)
     s.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_d3q27_row_orders_give_identical_bits(precision):
+    """The D3Q27 row orders the autotuner picks from (rows walked bottom-up, or top-down on alternate
+    steps): cells are independent within a step and the force partials are indexed by the cell's wave,
+    so populations and force samples are the same bits in every order; the autotuner restores the
+    state it times on."""
+    cfg, sdf = case3(precision, W=150)
+    cfg["velocitySet"] = 27
+    W, H, D = cfg["width"], cfg["height"], cfg["depth"]
+    g.add_sphere(sdf, W, H, D, 12, 0.3, D - 0.8, 1.5)
+    outs = {}
+    for variant in (0, 1, 2, "auto"):
+        s = Solver3D.create(dict(cfg))
+        s.set_sdf(sdf)
+        s.init_field()
+        early = (np.zeros(0, np.int64), np.zeros((0, 3)))
+        if variant == "auto":
+            s.run(7)
+            f7 = s.read_populations_raw()
+            early = s.read_forces_raw()
+            s.autotune(5)
+            np.testing.assert_array_equal(s.read_populations_raw(), f7)
+            assert len(s.read_forces_raw()[0]) == 0  # the trial steps leave no samples behind
+            assert s.launch_info()[1] in (0, 1, 2)
+            s.run(23)
+        else:
+            s.set_launch(s.launch_info()[0], variant)
+            s.run(30)
+        late = s.read_forces_raw()
+        outs[variant] = (s.read_populations_raw(),
+                         (np.concatenate([early[0], late[0]]), np.concatenate([early[1], late[1]])))
+        s.destroy()
+    for k, v in outs.items():
+        np.testing.assert_array_equal(v[0], outs[0][0], err_msg=str(k))
+        np.testing.assert_array_equal(v[1][0], outs[0][1][0], err_msg=str(k))
+        np.testing.assert_array_equal(v[1][1], outs[0][1][1], err_msg=str(k))
 
 
 @pytest.mark.parametrize("block", [64, 256])
