@@ -2,9 +2,9 @@
 //
 // Replaces the reference's single-workgroup FP32 reduce pass (src/shaders/aux.ts:144-172,
 // src/shaders/aux3d.ts:155-183) with a fused two-level reduction: every wave of the group sums its
-// cells' momentum-exchange forces in double with wave64 butterfly shuffles and publishes one
-// partial, and the last block to finish (completion counter) sums the partials in wave order and
-// appends the sample to the history ring. When one block holds every node of the sample, the
+// cells' momentum-exchange forces in double over a fixed tree of lane permutes (wave_sum) and
+// publishes one partial, and the last block to finish (completion counter) sums the partials in
+// wave order and appends the sample to the history ring. When one block holds every node of the sample, the
 // partials meet in LDS instead and that block appends the sample, with the same summation order.
 // The summation order is fixed, so a sample is bit-reproducible run to run.
 //
@@ -21,12 +21,37 @@
 
 namespace lbm {
 
+// v + (v of lane CTRL's DPP source): the two 32-bit halves of the double moved by data-parallel
+// primitive lane permutes, which stay in the vector ALU (a __shfl_xor is an LDS permute round trip
+// per half).
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(b)), CTRL, 0xF, 0xF, false));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_update_dpp(0, int(uint32_t(b >> 32)), CTRL, 0xF, 0xF, false));
+  return v + __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), lane));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), lane));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+// Sum over the 64 lanes of the wave, the same tree on every call: lanes l and l ^ 1, then the two
+// pairs of a quad, then lanes i and 7 - i of each half row (two quads), lanes i and 15 - i of each
+// row of 16 (two half rows), then the four row sums in row order, (r0 + r1) + (r2 + r3). Every
+// lane receives the sum. All 64 lanes must be active.
 template <int DIM>
 __device__ __forceinline__ void wave_sum(double (&v)[DIM]) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int d = 0; d < DIM; d++) v[d] += __shfl_xor(v[d], off, 64);
+  for (int d = 0; d < DIM; d++) {
+    double s = dpp_add<0xB1>(v[d]);  // quad_perm [1, 0, 3, 2]
+    s = dpp_add<0x4E>(s);            // quad_perm [2, 3, 0, 1]
+    s = dpp_add<0x141>(s);           // row_half_mirror
+    s = dpp_add<0x140>(s);           // row_mirror
+    v[d] = (lane_value(s, 0) + lane_value(s, 16)) + (lane_value(s, 32) + lane_value(s, 48));
+  }
 }
 
 // Every thread of every block of the group must call this (no early return before it). A group is
@@ -48,7 +73,7 @@ __device__ void force_sample(const float (&force)[DIM], const ForceSink& fs, uin
   wave_sum<DIM>(v);
   if (nblocks == 1) {
     // One block holds every node of the sample: its waves' partials meet in LDS and its first wave
-    // sums them exactly as the last block does below (lane l takes partial l, then the butterfly),
+    // sums them exactly as the last block does below (lane l takes partial l, then wave_sum),
     // without the round trips through device memory and the completion counter.
     if (lane == 0)
 #pragma unroll
