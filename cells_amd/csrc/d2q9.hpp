@@ -165,6 +165,10 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   // x = W - 1 they reach into the neighbouring rows), so the loads go out before the flag word is
   // known and their latency overlaps its; the boundary rows read the flag word first.
   const bool interior = y > 0 && y + 1 < p.H;
+  // The column's relaxation coefficients go out first (x < W): waiting for them after the node's kind
+  // is known would add a second memory round trip to every wave, which small grids (one wave per
+  // SIMD, nothing else to overlap) pay in full.
+  const float4 t = xtab[x];
   uint32_t fl = 0u;
   bool inlet = false, outlet = false;
   if (!interior) {
@@ -197,7 +201,6 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
   if (inlet) zou_he_inlet(f, y, p);
   if (outlet) zou_he_outlet(f);
   if (REC && rc.mom) record2(f, rc, idx);
-  const float4 t = xtab[x];
   Relax2 r;
   r.omBase = t.x;
   r.omM = t.y;
