@@ -52,13 +52,16 @@ __device__ __forceinline__ uint32_t div_w(uint32_t n, uint32_t m, uint32_t l) {
 
 // Block table layout: bulk tiles of every tunnel, then link tiles of every tunnel, then body tiles
 // of every tunnel (each tunnel has at least one body tile, which writes its force samples); tiles of
-// TILE cells or list nodes, one thread each.
+// TILE cells or list nodes, one thread each. With reverse set the bulk tiles are taken from the end
+// of the table (the last tile first): alternating the direction every step makes a step begin on
+// the cells the previous one wrote last, which are still in the 256 MB Infinity Cache when the
+// batch's population buffers do not fit in it.
 template <uint32_t TILE>
 __global__ __launch_bounds__(TILE) void batch2_step(const TunnelDesc* __restrict__ T,
                                                    const uint16_t* __restrict__ block_tunnel, uint32_t n_bulk_blocks,
                                                    uint32_t link_end, int parity, const uint64_t* __restrict__ base_step,
-                                                   uint32_t k) {
-  const uint32_t b = blockIdx.x;
+                                                   uint32_t k, uint32_t reverse) {
+  const uint32_t b = reverse && blockIdx.x < n_bulk_blocks ? n_bulk_blocks - 1u - blockIdx.x : blockIdx.x;
   const TunnelDesc& d = T[block_tunnel[b]];
   Params2 p = d.p;
   const uint64_t step = *base_step + k;
@@ -115,7 +118,8 @@ struct lbm_batch {
   double ramp_steps = 0;
   bool dirty = true;
   uint32_t tile = 256;  // cells (bulk) or list nodes per block
-  int order = 0;        // 0: tunnels in creation order; 1: largest tunnel first
+  int order = 0;        // bit 0: tunnels in creation order (0) or largest first (1); bit 1: bulk tiles
+                        // walked from the end of the table on the steps of parity 1
   static constexpr int GRAPH_STEPS = 32;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
@@ -144,7 +148,7 @@ int refresh(lbm_batch* b) {
   const uint32_t T = b->tile;
   std::vector<size_t> perm(n);
   for (size_t i = 0; i < n; i++) perm[i] = i;
-  if (b->order == 1)
+  if (b->order & 1)
     std::stable_sort(perm.begin(), perm.end(), [&](size_t a, size_t c) { return b->t[a]->N > b->t[c]->N; });
   b->desc.assign(n, TunnelDesc{});
   uint32_t blocks = 0;
@@ -211,9 +215,10 @@ int refresh(lbm_batch* b) {
 }
 
 void launch_step(lbm_batch* b, int parity, uint32_t k) {
+  const uint32_t reverse = (b->order & 2) && parity ? 1u : 0u;
 #define LBM_BATCH_LAUNCH(T)                                                                                      \
   batch2_step<T><<<b->n_blocks, T, 0, b->stream>>>(b->d_desc, b->d_block_tunnel, b->n_bulk_blocks, b->link_end, \
-                                                   parity, b->d_step, k)
+                                                   parity, b->d_step, k, reverse)
   switch (b->tile) {
     case 64: LBM_BATCH_LAUNCH(64); break;
     case 128: LBM_BATCH_LAUNCH(128); break;
@@ -369,7 +374,7 @@ int lbm_batch_reset_clock(lbm_batch* b) {
 int lbm_batch_set_launch(lbm_batch* b, int32_t tile, int32_t order) {
   if (!b) return fail(LBM_ERR_INVALID, "null batch");
   if (tile != 64 && tile != 128 && tile != 256 && tile != 512) return fail(LBM_ERR_INVALID, "tile must be 64, 128, 256 or 512");
-  if (order != 0 && order != 1) return fail(LBM_ERR_INVALID, "order must be 0 or 1");
+  if (order < 0 || order > 3) return fail(LBM_ERR_INVALID, "order must be 0-3");
   if (uint32_t(tile) != b->tile || order != b->order) {
     b->tile = uint32_t(tile);
     b->order = order;
@@ -386,7 +391,8 @@ int lbm_batch_launch_info(lbm_batch* b, int32_t* tile, int32_t* order) {
   return LBM_OK;
 }
 
-// Times every tile width with both tunnel orders over trial_steps launches (after two untimed ones;
+// Times every tile width with both tunnel orders, each with the bulk tiles walked one way or
+// alternating, over trial_steps launches (after two untimed ones;
 // every candidate three times, interleaved, keeping its fastest time) on the tunnels' own state, keeps the
 // fastest, and restores every tunnel's populations and force ring (the batch's and tunnels' step
 // counters are not advanced). Neither the tile width nor the order changes a result: cells are
@@ -457,8 +463,10 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
     HIP_TRY(hipMemcpyAsync(g.sv[i].state, s->state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, b->stream));
   }
   g.saved = true;
-  const std::pair<uint32_t, int> cand[] = {{64, 0}, {128, 0}, {256, 0}, {512, 0}, {64, 1}, {128, 1}, {256, 1}, {512, 1}};
-  const size_t nc = sizeof(cand) / sizeof(cand[0]);
+  std::vector<std::pair<uint32_t, int>> cand;
+  for (int o : {0, 1, 2, 3})
+    for (uint32_t t : {64u, 128u, 256u, 512u}) cand.push_back({t, o});
+  const size_t nc = cand.size();
   std::vector<double> best_of(nc, 1e300);
   for (int rep = 0; rep < 3; rep++) {
     for (size_t c = 0; c < nc; c++) {

@@ -556,8 +556,9 @@ class Batch:
         return float(ms.value)
 
     def autotune(self, trial_steps: int = 20):
-        """Times the tile widths and tunnel orders on the current state (restored); returns the
-        chosen (tile, order)."""
+        """Times the tile widths, tunnel orders and bulk-tile directions on the current state
+        (restored); returns the chosen (tile, order): order bit 0 largest tunnel first, bit 1 bulk
+        tiles walked from the end on every other step."""
         t, o = C.c_int32(), C.c_int32()
         _check(_lib.lbm_batch_autotune(self._h, int(trial_steps), C.byref(t), C.byref(o)))
         return int(t.value), int(o.value)

@@ -331,12 +331,14 @@ int lbm_batch_sync(lbm_batch* b);
 int lbm_batch_time_steps(lbm_batch* b, int64_t n, double* elapsed_ms);
 /* No counterpart in the reference (its page sweeps workgroup sizes by hand, src/bench.ts:110-118):
  * times the tile widths 64, 128, 256 and 512 (cells or list nodes per block) with the tunnels in
- * creation order and largest first, trial_steps launches each on the tunnels' own state, keeps the
+ * creation order and largest first, each with the bulk tiles walked in one direction or in
+ * alternating directions, trial_steps launches each on the tunnels' own state, keeps the
  * fastest and restores every tunnel's populations and force ring; tile and order (may be NULL)
  * receive the choice. Neither changes a result. */
 int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t* order);
-/* The launch shape in use, and setting it by hand (tile 64/128/256/512; order 0 creation order,
- * 1 largest tunnel first). */
+/* The launch shape in use, and setting it by hand (tile 64/128/256/512; order bit 0: 0 creation
+ * order, 1 largest tunnel first; bit 1: bulk tiles walked from the end of the table on every other
+ * step). */
 int lbm_batch_launch_info(lbm_batch* b, int32_t* tile, int32_t* order);
 int lbm_batch_set_launch(lbm_batch* b, int32_t tile, int32_t order);
 

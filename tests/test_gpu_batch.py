@@ -56,14 +56,15 @@ def test_batch_populations_match_separate_run():
 
 
 def test_batch_launch_shapes_give_identical_bits():
-    """Every tile width and tunnel order the batch autotuner picks from stores the same bits and the
-    same force samples (cells are independent within a step; force samples are summed per wave of
-    body nodes in node order, whatever the block size)."""
+    """Every tile width, tunnel order and bulk-tile direction the batch autotuner picks from stores
+    the same bits and the same force samples (cells are independent within a step; force samples are
+    summed per wave of body nodes in node order, whatever the block size or the order blocks run in),
+    also across the graph-captured 32-step loops (250 steps, chunks of 100)."""
     cases = sweep.sweep_cases(n=5, seed=7, sizes=(64, 96, 160))
     ref = None
-    for tile, order in ((256, 0), (64, 0), (128, 1), (512, 1)):
+    for tile, order in ((256, 0), (64, 0), (128, 1), (512, 1), (256, 2), (64, 3), (512, 2)):
         info = {}
-        _, traces = sweep.run_batched(cases, 150, chunk=100, ramp_steps=100, launch=(tile, order), info=info)
+        _, traces = sweep.run_batched(cases, 250, chunk=100, ramp_steps=100, launch=(tile, order), info=info)
         assert (info["tile"], info["order"]) == (tile, order)
         got = (info["checksums"], [v for _, v in traces])
         if ref is None:
