@@ -260,7 +260,7 @@ def adjoint2d_case(fd=True):
     outlet, slip walls, 15% sponge, absorbing layer 0.02; J = mean C_D over steps 4001..6000;
     parameters (alpha, a_1..a_8) with bump width chord / 16; checkpoints every 200 steps."""
     from cells_amd import shapes
-    from cells_amd.adjoint import ShapeAdjoint, parameter_gradient
+    from cells_amd.adjoint import ShapeAdjoint, bouzidi_branch_switches, parameter_gradient
     from cells_amd.lbm import X_OPEN, Y_SLIP, Solver
 
     W, H, chord, u, T, window, K = 384, 128, 64.0, 0.1, 6000, 2000, 200
@@ -336,22 +336,32 @@ def adjoint2d_case(fd=True):
     if fd:
         # central differences of the GPU's FP32 forward at h = 1e-3 and 1e-2 (at 1e-3 the smallest
         # components are within a few FP32 roundings of J); flips counts the nodes whose solid flag
-        # differs between theta +- h e_p and theta, which would make J jump
+        # differs between theta +- h e_p and theta, which would make J jump, and branch_switches per
+        # parameter the Bouzidi links whose q lies on different sides of 1/2 (or of a clamp bound) at
+        # theta + h e_p and theta - h e_p: J's slope changes there, so a difference quotient across one
+        # is not J's derivative at theta. The error over the parameters without a switch is reported
+        # beside the error over all of them.
         solid0 = sdf0.reshape(H, W)[:, 1:-1] < 0
         for h, key in ((1e-3, "fd"), (1e-2, "fd_h1e-2")):
-            fdv, flips = [], 0
+            fdv, flips, switches = [], 0, []
             for p in range(len(theta)):
                 e = np.zeros_like(theta)
                 e[p] = h
                 sp, sm = sdf_of(theta + e)[0], sdf_of(theta - e)[0]
                 flips += int(np.count_nonzero((sp.reshape(H, W)[:, 1:-1] < 0) != solid0))
                 flips += int(np.count_nonzero((sm.reshape(H, W)[:, 1:-1] < 0) != solid0))
+                switches.append(bouzidi_branch_switches(sp, sm, W, H))
                 fdv.append((adj.forward(sp) - adj.forward(sm)) / (2 * h))
             fdv = np.array(fdv)
             out[key] = fdv.tolist()
             out[key + "_solid_flips"] = flips
+            out[key + "_branch_switches"] = switches
             out["max_rel_err_vs_" + key] = float(np.max(np.abs(grad - fdv) / np.abs(fdv)))
             out["max_err_over_max_grad_vs_" + key] = float(np.max(np.abs(grad - fdv)) / np.abs(fdv).max())
+            smooth = [p for p in range(len(theta)) if switches[p] == 0]
+            if smooth:
+                out["max_rel_err_vs_" + key + "_without_switches"] = float(
+                    np.max(np.abs(grad[smooth] - fdv[smooth]) / np.abs(fdv[smooth])))
     s.destroy()
     return out
 
@@ -367,7 +377,7 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
     the device-memory budget (the first of the budgets that can be allocated). Finite differences
     (h = 1e-3) use the FP32-storage forward, whose own adjoint gradient is reported alongside."""
     from cells_amd import shapes
-    from cells_amd.adjoint import ShapeAdjoint, parameter_gradient
+    from cells_amd.adjoint import ShapeAdjoint, bouzidi_branch_switches, parameter_gradient
     from cells_amd.lbm import Solver3D
 
     W, H, D, u, T, window, K, r = 256, 128, 128, 0.1, 3000, 1000, 100, 12.0
@@ -445,14 +455,18 @@ def adjoint3d_case(fd_coeffs=(0, 1, 2, 3, 6, 12), budget_gb=(210, 120, 0)):
         out["grad_fp32"] = g32.tolist()
         a32.release()
         h = 1e-3
-        fdv = {}
+        fdv, switches = {}, {}
         for p in fd_coeffs:
             e = np.zeros(16)
             e[p] = h
-            fdv[p] = (a32.forward(sdf_of(coeffs + e)[0]) - a32.forward(sdf_of(coeffs - e)[0])) / (2 * h)
+            sp, sm = sdf_of(coeffs + e)[0], sdf_of(coeffs - e)[0]
+            switches[str(p)] = bouzidi_branch_switches(sp, sm, W, H, D)
+            fdv[p] = (a32.forward(sp) - a32.forward(sm)) / (2 * h)
         s32.destroy()
         scale = max(abs(v) for v in fdv.values())
         out["fd_fp32"] = {str(p): v for p, v in fdv.items()}
+        # Bouzidi links whose q changes side of 1/2 (or of a clamp bound) between coeffs +- h e_p
+        out["fd_branch_switches"] = switches
         # relative error over the components above 1% of the largest (the others are zero by the
         # body's symmetry, where differences only see rounding), and the largest error over all of
         # them relative to the largest component

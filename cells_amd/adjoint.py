@@ -64,6 +64,59 @@ def parameter_gradient(dJ_dsdf: np.ndarray, dsdf_dtheta: np.ndarray) -> np.ndarr
     return np.asarray(dsdf_dtheta, dtype=np.float64) @ np.asarray(dJ_dsdf, dtype=np.float64)
 
 
+_C2 = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
+_C3 = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0],
+                [-1, -1, 0], [1, 0, 1], [-1, 0, -1], [0, 1, 1], [0, -1, -1], [1, -1, 0], [-1, 1, 0], [1, 0, -1],
+                [-1, 0, 1], [0, 1, -1], [0, -1, 1]])
+
+
+def bouzidi_links(sdf, W: int, H: int, D: int = 1):
+    """The Bouzidi links of a geometry: (fluid node, direction) pairs whose upstream node x - c_i lies
+    on the grid and is solid (solid: sdf < 0 except on the open x = 0 and x = W - 1 planes, as the
+    flag kernels decide), with q = clamp(dF / max(dF - dS, 1e-6), 1e-3, 1) in float32, dF and dS the
+    SDF at the node and at its upstream node (src/shaders/step.ts:138-145, step3d.ts:149-156).
+    Returns {direction: (node indices, q)} for D2Q9 (D = 1) or D3Q19."""
+    s = np.asarray(sdf, dtype=np.float32).reshape(D, H, W)
+    solid = s < 0
+    solid[:, :, 0] = solid[:, :, W - 1] = False
+    cs = _C2 if D == 1 else _C3
+    out = {}
+    for i in range(1, len(cs)):
+        cx, cy = int(cs[i][0]), int(cs[i][1])
+        cz = int(cs[i][2]) if D > 1 else 0
+        # node (x, y, z) with upstream (x - cx, y - cy, z - cz) on the grid
+        xs = slice(max(cx, 0), W + min(cx, 0))
+        ys = slice(max(cy, 0), H + min(cy, 0))
+        zs = slice(max(cz, 0), D + min(cz, 0))
+        us = (slice(zs.start - cz, zs.stop - cz), slice(ys.start - cy, ys.stop - cy), slice(xs.start - cx, xs.stop - cx))
+        node_fluid = ~solid[zs, ys, xs]
+        up_solid = solid[us]
+        link = node_fluid & up_solid
+        dF, dS = s[zs, ys, xs][link], s[us][link]
+        q = np.clip(dF / np.maximum(dF - dS, np.float32(1e-6)), np.float32(1e-3), np.float32(1.0))
+        z, y, x = np.nonzero(link)
+        idx = (x + xs.start) + W * ((y + ys.start) + H * (z + zs.start))
+        out[i] = (idx.astype(np.int64), q.astype(np.float32))
+    return out
+
+
+def bouzidi_branch_switches(sdf_a, sdf_b, W: int, H: int, D: int = 1) -> int:
+    """Number of Bouzidi links of one geometry that the other has with q on the other side of 1/2 or of
+    a clamp bound (the interpolation switches formula there, src/shaders/step.ts:146-156, so J(theta)
+    has a kink: a central difference between the two geometries straddles a change of slope), plus
+    the links only one of them has (a node flipping between fluid and solid)."""
+    la, lb = bouzidi_links(sdf_a, W, H, D), bouzidi_links(sdf_b, W, H, D)
+    n = 0
+    for i in la:
+        ia, qa = la[i]
+        ib, qb = lb[i]
+        common, pa, pb = np.intersect1d(ia, ib, assume_unique=True, return_indices=True)
+        n += (len(ia) - len(common)) + (len(ib) - len(common))
+        a, b = qa[pa], qb[pb]
+        n += int(np.count_nonzero(((a < 0.5) != (b < 0.5)) | ((a <= 1e-3) != (b <= 1e-3)) | ((a >= 1.0) != (b >= 1.0))))
+    return n
+
+
 try:
     import torch
 

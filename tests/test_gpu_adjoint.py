@@ -191,6 +191,26 @@ def test_adjoint2d_naca_384x128_matches_fp64_fixture():
           f"{fd['J']:.12f} (oracle)")
     assert abs(fd["J"] - ref["J"]) <= 1e-9 * abs(ref["J"])  # two FP64 references of one objective
     assert rel.max() < 0.01
+    # at the configured h = 1e-3: within 1% on every parameter whose difference quotient does not
+    # straddle a Bouzidi branch switch (only a_1 does: one link's q crosses 1/2 between theta +- h e_1,
+    # tests/test_adjoint_fd_diagnostics.py)
+    from cells_amd import shapes
+    from cells_amd.adjoint import bouzidi_branch_switches
+
+    theta = np.array([5.0] + [0.0] * 8)
+    switches = []
+    for p in range(9):
+        e = np.zeros(9)
+        e[p] = 1e-3
+        sp, sm = (shapes.naca_bumps(384, 128, 96.0, 63.5, 64.0, t[0], t[1:])[0] for t in (theta + e, theta - e))
+        switches.append(bouzidi_branch_switches(sp, sm, 384, 128))
+    assert switches == [0, 1, 0, 0, 0, 0, 0, 0, 0]
+    smooth = [p for p in range(9) if switches[p] == 0]
+    f3 = np.array(fd["fd_h0.001"])
+    rel3 = np.abs(g[smooth] - f3[smooth]) / np.abs(f3[smooth])
+    print(f"vs FP64 central differences (h 1e-3) on the {len(smooth)} parameters without a branch switch: "
+          f"max rel {rel3.max():.2e}")
+    assert rel3.max() < 0.01
     # the configured memory plan (recompute from checkpoints) and the recorded one agree bit for bit
     assert out["info"]["stored_segments"] == 0 and out["recorded"]["info"]["stored_segments"] == 30
     print(f"adjoint / forward: {out['ratio']:.2f} (recompute), {out['recorded']['ratio']:.2f} (recorded)")
