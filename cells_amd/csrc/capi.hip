@@ -628,8 +628,10 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials,
   if (s->nd == 2) {
     if (s->ramp.kind) s->p2.uIn = float(s->ramp.at(s->step));
     const ListTables lt{s->list, s->gtab, s->bmask, s->xmask, s->lbase, s->links, s->n_link, s->n_body};
+    const int order = s->sweep & 3;  // row order, as in 3D below
+    const bool descend = order == 1 ? (src > dst) : order == 2 ? (src < dst) : false;
     launch_step2(static_cast<const float*>(src), static_cast<float*>(dst), s->flags, lt, s->p2, s->xtab, fs,
-                 s->bulk_block, s->stream, rec);
+                 s->bulk_block, descend, s->stream, rec);
   } else if (s->Q == 27) {
     if (s->ramp.kind) s->p3.uIn = float(s->ramp.at(s->step));
     float fin[27];
@@ -1212,8 +1214,8 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
 }
 
 // Autotuner for the step kernel's launch shape: each distinct bulk block width the launcher can use
-// for this W (and, in 3D, the three row orders, two store variants and two pull orders; D3Q27: its one
-// block width in the three row orders) is timed over
+// for this W with the three row orders (and, in 3D, two store variants and two pull orders; D3Q27: its
+// one block width in the three row orders) is timed over
 // trial_steps steps (after eight untimed ones; three interleaved rounds, the fastest round counting)
 // on a copy of the solver's own state, the fastest is kept, and the
 // populations, step, parity and force ring are restored. Results are cached per (lattice, velocity
@@ -1321,7 +1323,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
           for (int ea : {0, SWEEP_EARLY})
             if (!(ea && smag)) cand.push_back({bx, sw | mg | ea});  // the subgrid-model kernels have one pull order
     } else {
-      cand.push_back({bx, s->sweep});
+      for (int sw : {1, 2, 0}) cand.push_back({bx, sw});  // D2Q9: the three row orders
     }
   }
   std::vector<double> best_of(cand.size(), 1e300);
@@ -1364,7 +1366,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (block) *block = int32_t(s->Q == 27 ? step27_block_x(s->W) : bulk_block_x(s->W, s->bulk_block));
-  if (sweep) *sweep = s->nd == 3 ? s->sweep : 0;
+  if (sweep) *sweep = s->sweep;
   return LBM_OK;
 }
 
@@ -1372,8 +1374,8 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   const int bb = bulk_block_for(block);
   if (bb < 0 || block <= 0) return fail(LBM_ERR_INVALID, "block must be a multiple of 64 up to 1024");
-  if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE | SWEEP_EARLY) || (sweep & 3) == 3) : sweep != 0)
-    return fail(LBM_ERR_INVALID, "variant must be 0-2, 4-6, 8-10 or 12-14 (3D), 0 (2D)");
+  if (s->nd == 3 ? (sweep < 0 || sweep > (3 | SWEEP_MERGE | SWEEP_EARLY) || (sweep & 3) == 3) : (sweep < 0 || sweep > 2))
+    return fail(LBM_ERR_INVALID, "variant must be 0-2, 4-6, 8-10 or 12-14 (3D), 0-2 (2D)");
   if (s->Q == 27) {  // one block width; the row order is the only variant
     if (block != int32_t(step27_block_x(s->W)) || (sweep & ~3) != 0)
       return fail(LBM_ERR_UNSUPPORTED, "D3Q27: block must be " + std::to_string(step27_block_x(s->W)) +
@@ -1382,7 +1384,7 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
     return LBM_OK;
   }
   s->bulk_block = bb;
-  if (s->nd == 3) s->sweep = sweep;
+  s->sweep = sweep;
   return LBM_OK;
 }
 

@@ -96,9 +96,10 @@ void launch_flags27(int prec, const float* sdf, uint32_t* flags, void* fa, void*
                     const Params3& p, hipStream_t st);
 
 // ---- D2Q9 (FP32 storage) ----
+// descend: bulk rows walked from the top (y reversed) in this step.
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt,
-                  const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block, hipStream_t st,
-                  const Record* rec = nullptr);
+                  const Params2& p, const float4* xtab, const ForceSink& fs, int bulk_block, bool descend,
+                  hipStream_t st, const Record* rec = nullptr);
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st);
 void launch_macro2(const float* f, const uint32_t* flags, float4* mac, const Params2& p, hipStream_t st);
 void launch_init2(float* fa, float* fb, const float4* mac, const Params2& p, hipStream_t st);

@@ -17,13 +17,15 @@ namespace lbm {
 // blocks [0, nlb) take the nodes without body links, the nbb after them the nodes with body links,
 // which carry the force sample. The remaining rows are the bulk rows, where x and y come from the
 // launch coordinates and per direction the source address is a wave-uniform base
-// (src + i*N - delta_i) plus the lane's byte offset.
+// (src + i*N - delta_i) plus the lane's byte offset. With descend set the bulk rows are walked from
+// the top: alternating the direction every step makes a step begin on the rows the previous one wrote
+// last, which are still in the 256 MB Infinity Cache when the two population buffers do not fit in it.
 template <bool REC>
 __global__ __launch_bounds__(1024) void step2_kernel(const float* __restrict__ src, float* __restrict__ dst,
                                                      const uint32_t* __restrict__ flags,
                                                      const float4* __restrict__ xtab, Params2 p, ListTables lt,
                                                      uint32_t yl, uint32_t nlb, uint32_t nbb, ForceSink fs,
-                                                     Record rc) {
+                                                     Record rc, uint32_t descend) {
   if (blockIdx.y < yl) {
     const uint32_t lb = blockIdx.y * gridDim.x + blockIdx.x;
     float force[2] = {0.0f, 0.0f};
@@ -44,7 +46,7 @@ __global__ __launch_bounds__(1024) void step2_kernel(const float* __restrict__ s
   }
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= uint32_t(p.W)) return;
-  const uint32_t y = blockIdx.y - yl;
+  const uint32_t y = descend ? uint32_t(p.H) - 1u - (blockIdx.y - yl) : blockIdx.y - yl;
   bulk_cell2<REC>(src, dst, flags, xtab, p, x, int(y), x + uint32_t(p.W) * y, rc);
 }
 
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(256) void flags2_kernel(const float* __restrict__ s
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 void launch_step2(const float* src, float* dst, const uint32_t* flags, const ListTables& lt, const Params2& p,
-                  const float4* xtab, const ForceSink& fs, int bb, hipStream_t st, const Record* rec) {
+                  const float4* xtab, const ForceSink& fs, int bb, bool descend, hipStream_t st, const Record* rec) {
   const uint32_t bx = bulk_block_x(p.W, bb);
   const uint32_t gx = (uint32_t(p.W) + bx - 1) / bx;
   const uint32_t nlb = list_blocks(lt.n_link, bx);
@@ -179,9 +181,10 @@ void launch_step2(const float* src, float* dst, const uint32_t* flags, const Lis
   const uint32_t yl = (nlb + nbb + gx - 1) / gx;
   const dim3 grid(gx, yl + uint32_t(p.H));
   if (rec)
-    step2_kernel<true><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, *rec);
+    step2_kernel<true><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, *rec, descend ? 1u : 0u);
   else
-    step2_kernel<false><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, Record{nullptr, nullptr});
+    step2_kernel<false><<<grid, bx, 0, st>>>(src, dst, flags, xtab, p, lt, yl, nlb, nbb, fs, Record{nullptr, nullptr},
+                                             descend ? 1u : 0u);
 }
 void launch_xtab2(const Params2& p, float4* tab, hipStream_t st) {
   xtab2_kernel<<<blocks_for(uint32_t(p.W), 256), 256, 0, st>>>(p, tab);

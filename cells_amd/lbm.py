@@ -359,8 +359,8 @@ class _Base:
         return int(c.value)
 
     def launch_info(self):
-        """(threads per bulk block, 3D variant: bits 0-1 row order 0 fixed, 1 / 2 alternating; bit 2
-        inlet/outlet nodes share their wave's stores; bit 3 loads issued before the node kind is
+        """(threads per bulk block, variant: bits 0-1 row order 0 fixed, 1 / 2 alternating; in 3D bit
+        2 inlet/outlet nodes share their wave's stores, bit 3 loads issued before the node kind is
         known) in use."""
         b, w = C.c_int32(), C.c_int32()
         _check(_lib.lbm_launch_info(self._h, C.byref(b), C.byref(w)))

@@ -226,8 +226,8 @@ int lbm_time_steps(lbm_solver* s, int64_t n, double* elapsed_ms);
 int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_ms, double* max_step_ms);
 /* No counterpart in the reference (its bench sweeps workgroup sizes 64/128/256 by hand,
  * src/bench.ts:110-118): times every bulk-kernel block width usable for this grid (in 3D, each with
- * the three row orders, the two store variants and the two pull orders of lbm_launch_info; D3Q27:
- * its one block width in the three row orders) on a copy of the solver's
+ * the three row orders, in 3D also the two store variants and the two pull orders of lbm_launch_info;
+ * D3Q27: its one block width in the three row orders) on a copy of the solver's
  * state, with force samples as configured, keeps the fastest (cached for the process per lattice,
  * velocity set, precision, grid size, open-face kind, subgrid model on/off and force cadence), and restores
  * populations, step and force ring (also when it fails part way). chosen (may be NULL)
@@ -235,16 +235,17 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
  * two pull orders of the shared-store variant, agree to floating-point rounding (their compiled code
  * may fuse different multiply-adds). */
 int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen);
-/* No counterpart in the reference: the launch shape in use, threads per bulk block and (3D) the
+/* No counterpart in the reference: the launch shape in use, threads per bulk block and the
  * variant: bits 0-1 the row order, 0 always bottom-up, 1 (2) top-down on the steps that read the
- * population buffer at the higher (lower) address, bottom-up on the others; bit 2 set when the
- * inlet and outlet nodes share their wave's stores; bit 3 set when a node's 19 loads are issued
- * before its kind (flag word) is known (no effect with the subgrid model or on recording steps).
- * D3Q27 has one block width (whole rows of at most 256 threads) and the row order as its variant. */
+ * population buffer at the higher (lower) address, bottom-up on the others; in 3D also bit 2, set
+ * when the inlet and outlet nodes share their wave's stores, and bit 3, set when a node's 19 loads
+ * are issued before its kind (flag word) is known (no effect with the subgrid model or on recording
+ * steps). D3Q27 has one block width (whole rows of at most 256 threads) and the row order as its
+ * variant; D2Q9 the row order. */
 int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
 /* No counterpart in the reference: sets the launch shape by hand instead of lbm_autotune (block as
- * workgroup_size, variant as lbm_launch_info reports it; 0 in 2D; D3Q27: its one block width and a
- * row order 0-2), e.g. to pin a run's shape. */
+ * workgroup_size, variant as lbm_launch_info reports it; a row order 0-2 in 2D; D3Q27: its one block
+ * width and a row order 0-2), e.g. to pin a run's shape. */
 int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
 /* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
  * structure-of-arrays populations of elem_bytes each plus a u32 flag word per cell and writes the q

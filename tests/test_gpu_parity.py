@@ -354,6 +354,46 @@ def test_d2q9_brush_strokes_match_oracle():
         s.destroy()
 
 
+@pytest.mark.parametrize("name", ["tunnel", "channel", "periodic"])
+def test_d2q9_launch_shapes_give_identical_bits(name):
+    """The D2Q9 launch shapes the autotuner picks from (block widths that split the rows differently,
+    rows walked bottom-up or top-down on alternate steps): cells are independent within a step and the
+    force sample is summed per wave of body nodes in node order, so populations and force samples are
+    the same bits in every shape; the autotuner restores the state it times on."""
+    cfg, sdf_fn = CASES2[name]
+    cfg = dict(cfg, width=150, height=40)
+    W, H = cfg["width"], cfg["height"]
+    sdf = sdf_fn(W, H)
+    outs = {}
+    for block, variant in ((64, 0), (128, 1), (256, 2), (192, 1), (64, 2), ("auto", None)):
+        s = Solver.create(dict(cfg))
+        s.set_sdf(sdf)
+        s.init_field()
+        early = (np.zeros(0, np.int64), np.zeros((0, 2)))
+        if block == "auto":
+            s.run(7)
+            f7 = s.read_populations()
+            early = s.read_forces_raw()
+            s.autotune(5)
+            np.testing.assert_array_equal(s.read_populations().view(np.uint32), f7.view(np.uint32))
+            assert len(s.read_forces_raw()[0]) == 0
+            assert s.launch_info()[1] in (0, 1, 2)
+            s.run(33)
+        else:
+            s.set_launch(block, variant)
+            assert s.launch_info()[1] == variant
+            s.run(40)
+        late = s.read_forces_raw()
+        outs[(block, variant)] = (s.read_populations(), np.concatenate([early[0], late[0]]),
+                                  np.concatenate([early[1], late[1]]))
+        s.destroy()
+    ref = outs[(64, 0)]
+    for k, v in outs.items():
+        np.testing.assert_array_equal(v[0].view(np.uint32), ref[0].view(np.uint32), err_msg=str(k))
+        np.testing.assert_array_equal(v[1], ref[1], err_msg=str(k))
+        np.testing.assert_array_equal(v[2].view(np.uint64), ref[2].view(np.uint64), err_msg=str(k))
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp16"])
 def test_d3q19_launch_shapes_give_identical_bits(precision):
     """The launch shapes the autotuner picks from (a row of 150 nodes, so the block widths split it
