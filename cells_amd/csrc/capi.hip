@@ -1229,11 +1229,13 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   static std::mutex cache_mu;
   static std::map<std::string, std::pair<int, int>> cache;
   // Everything that selects a different kernel instantiation or changes the work of a step is part
-  // of the key: lattice, precision, grid, open-face kind, subgrid model on/off, force cadence.
+  // of the key: lattice and velocity set, precision, grid, open-face kind, subgrid model on/off,
+  // force cadence.
   const bool smag = (s->nd == 2 ? s->cfg2.smagorinsky : s->cfg3.smagorinsky) > 0;
-  const std::string key = std::to_string(s->nd) + ":Q" + std::to_string(s->Q) + ":" + std::to_string(s->prec) + ":" + std::to_string(s->W) + "x" +
-                          std::to_string(s->H) + "x" + std::to_string(s->D) + ":faces" + std::to_string(s->faces) +
-                          ":smag" + std::to_string(int(smag)) + ":every" + std::to_string(s->force_every());
+  const std::string key = std::to_string(s->nd) + ":Q" + std::to_string(s->Q) + ":" + std::to_string(s->prec) + ":" +
+                          std::to_string(s->W) + "x" + std::to_string(s->H) + "x" + std::to_string(s->D) + ":faces" +
+                          std::to_string(s->faces) + ":smag" + std::to_string(int(smag)) + ":every" +
+                          std::to_string(s->force_every());
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     auto it = cache.find(key);
