@@ -2,16 +2,17 @@
 //
 // Restates src/shaders/step3d.ts:24-225 and src/shaders/aux3d.ts:20-152 for HIP, re-organised for
 // wave64 hardware: instead of one kernel in which the x-boundary and face nodes of every row take
-// a divergent slow path, a step is two launches on the same stream:
-//   * step3_bulk: one thread per cell over the whole grid in cell order. It handles plain fluid
-//     nodes (flags == 0) and plain inlet / outlet nodes (only the off-grid x bits set), which sit
-//     in the first and last wave of each x row, so every wave streams fully coalesced loads and
-//     stores. Other nodes (solid, faces, body links) are skipped.
-//   * step3_list: one thread per node of a list built once per geometry change (every non-solid
-//     node the bulk kernel skips, grouped by kind, in cell order within a kind). The reference's
-//     per-direction boundary rules (pullSlow3) are resolved on the host into gather indices and
-//     Bouzidi records, so the kernel issues independent loads only; it fuses the momentum-exchange
-//     force reduction in FP64 (reduce.hpp).
+// a divergent slow path, a step is one launch (step3_kernel) of two kinds of blocks:
+//   * bulk rows: one thread per cell, one block row per grid row. They handle plain fluid nodes,
+//     plain inlet / outlet nodes (only the off-grid x bits set), which sit in the first and last wave
+//     of each x row, and whole slip-face rows, whose pull pattern is uniform along the row, so every
+//     wave streams fully coalesced loads and stores. Other nodes (solid, next to a body, irregular
+//     face nodes) are skipped.
+//   * boundary-list blocks, dispatched first: one thread per node of a list built once per geometry
+//     change (every non-solid node the bulk rows skip, grouped by kind, in cell order within a kind).
+//     The reference's per-direction boundary rules (pullSlow3) are resolved on the host into gather
+//     indices and Bouzidi records, so these threads issue independent loads only; the blocks of nodes
+//     with body links carry the momentum-exchange force reduction in FP64 (reduce.hpp).
 // Storage is structure-of-arrays f[i * N + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16;
 // arithmetic is FP32 throughout, as in the reference.
 #include <type_traits>
