@@ -287,7 +287,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
   const uint32_t outlet_flag = d2 ? FLAG_OUTLET2 : FLAG_OUTLET3;
   const uint32_t plain_in = d2 ? (FLAG_INLET2 | INLET_DIRS2) : (FLAG_INLET3 | INLET_DIRS3);
   const uint32_t plain_out = d2 ? (FLAG_OUTLET2 | OUTLET_DIRS2) : (FLAG_OUTLET3 | OUTLET_DIRS3);
-  // Boundary list: every non-solid node the bulk kernel skips. Nodes without body links come first,
+  // Boundary list: every non-solid node the bulk rows of the step skip. Nodes without body links come first,
   // grouped inlet / outlet / other, then nodes with a body link (Bouzidi, or in 3D a slip face whose
   // mirror node is solid); cell order inside each group. Every pull is resolved here once.
   const int Q = s->Q;
@@ -309,7 +309,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
         continue;
       }
     } else {
-      // A node the bulk kernel would take stays there only if every pull it makes under its row's
+      // A node the bulk rows would take stays there only if every pull it makes under its row's
       // pattern is the one the reference's rules give; otherwise it is marked for the list.
       const uint32_t WH = uint32_t(s->W) * uint32_t(s->H);
       const Row3 r = row3(s, (k % WH) / uint32_t(s->W), k / WH);

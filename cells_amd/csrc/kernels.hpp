@@ -66,7 +66,7 @@ void launch_step3(int prec, const void* src, void* dst, const uint32_t* flags, c
                   const Params3& p, const float* feq_in, const float4* xtab, const ForceSink& fs, int bulk_block,
                   bool descend, bool merge, bool early, const uint8_t* row_clean, hipStream_t st,
                   const Record* rec = nullptr);
-// Per-x relaxation table (float4 per column) for the bulk kernel; rebuilt whenever parameters change.
+// Per-x relaxation table (float4 per column) for the step kernels; rebuilt whenever parameters change.
 void launch_xtab3(const Params3& p, float4* tab, hipStream_t st);
 uint32_t bulk_block_x(int W, int requested);
 void launch_macro3(int prec, const void* f, const uint32_t* flags, float4* mac, const Params3& p, hipStream_t st);

@@ -1,7 +1,7 @@
 // D2Q9 TRT kernels for gfx950: fused pull-stream + boundaries + collide, flags, init, macro.
 //
 // Restates src/shaders/step.ts:9-229 and src/shaders/aux.ts:14-141 for HIP with the same bulk /
-// boundary-list split as the D3Q19 kernels (kernels3d.hip): the bulk kernel streams plain fluid
+// boundary-list split as the D3Q19 kernels (kernels3d.hip): the bulk rows stream plain fluid
 // nodes and plain Zou-He inlet / outlet nodes with coalesced loads and stores; every other
 // non-solid node (walls, periodic seams, body links) runs the reference's pullSlow rules from a
 // list, with the momentum-exchange force reduced in FP64 in the same launch.

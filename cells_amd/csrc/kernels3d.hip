@@ -303,7 +303,7 @@ struct FeqIn {
   float v[19];
 };
 
-// ============================================================================ bulk kernel
+// ============================================================================ bulk rows
 // Grid (ceil(W / blockDim.x), H, D): one block row of threads per x row, so x, y, z come from the
 // launch coordinates. Per direction the source address is a wave-uniform base (src + d_i*N - delta_i,
 // kept in SGPRs) plus the lane's byte offset, shared by all 19 loads; stores likewise.
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void xtab3_kernel(Params3 p, float4* __restric
   tab[x] = make_float4(r.omBase, r.omM, r.sigma, r.t0);
 }
 
-// ============================================================================ boundary-list kernel
+// ============================================================================ boundary-list blocks
 // One node of the boundary list (k is its position in the list). Every pull is a gather through the
 // table the host resolved from pullSlow3's rules (src/shaders/step3d.ts:119-168), so the 19 loads are
 // independent. Nodes with body links (BODY) then apply them: a Bouzidi link (step3d.ts:147-166)
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void init3_kernel(typename S::elem* __restrict
 }
 
 // src/shaders/aux3d.ts:32-100: flags from the SDF and refill of cells the body uncovered. Force
-// slots are not needed here (the list kernel reduces per block), so none are assigned.
+// slots are not needed here (the boundary-list blocks reduce per block), so none are assigned.
 template <class S>
 __global__ __launch_bounds__(256) void flags3_kernel(const float* __restrict__ sdf, uint32_t* __restrict__ flags,
                                                      typename S::elem* __restrict__ fa, typename S::elem* __restrict__ fb,

@@ -34,7 +34,7 @@ constexpr float W3[19] = {1.0f / 3,  1.0f / 18, 1.0f / 18, 1.0f / 18, 1.0f / 18,
 constexpr uint32_t FLAG_SOLID3 = 1u << 19;
 constexpr uint32_t FLAG_INLET3 = 1u << 20;
 constexpr uint32_t FLAG_OUTLET3 = 1u << 21;
-// Set by the host on nodes whose pulls the bulk kernel's row patterns do not cover; never reported.
+// Set by the host on nodes whose pulls the bulk rows' patterns do not cover; never reported.
 constexpr uint32_t FLAG_LIST3 = 1u << 31;
 constexpr uint32_t INLET_DIRS3 = (1u << 1) | (1u << 7) | (1u << 9) | (1u << 13) | (1u << 15);
 constexpr uint32_t OUTLET_DIRS3 = (1u << 2) | (1u << 8) | (1u << 10) | (1u << 14) | (1u << 16);

@@ -108,7 +108,7 @@ struct lbm_solver {
   uint32_t* counter = nullptr;
   double* ring = nullptr;
   uint32_t* state = nullptr;
-  float4* xtab = nullptr;  // per-x relaxation coefficients of the bulk kernel
+  float4* xtab = nullptr;  // per-x relaxation coefficients of the step kernels
 
   int64_t step = 0;
   int parity = 0;
@@ -148,14 +148,14 @@ int enqueue_step(lbm_solver* s, const Record* rec, int sample, double* partials 
 // Force-reduction partials (3 doubles each) the next step of the solver writes, if it samples.
 size_t force_partials_needed(const lbm_solver* s);
 
-// The bulk kernel's view of a D3Q19 node (step3_kernel): the face bits of its row, whether the flag
+// The bulk rows' view of a D3Q19 node (step3_kernel): the face bits of its row, whether the flag
 // word selects the bulk pass, and the gather index of each pull under the row's pattern.
 struct Row3 {
   bool loY, hiY, loZ, hiZ;
   uint32_t mask;
 };
 Row3 row3(const lbm_solver* s, uint32_t y, uint32_t z);
-// 0: not taken by the bulk kernel, 1: fluid pulls, 2: inlet (stores the inlet equilibrium)
+// 0: not taken by the bulk rows, 1: fluid pulls, 2: inlet (stores the inlet equilibrium)
 int bulk_kind3(const Row3& r, uint32_t v);
 uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outlet, int i);
 
