@@ -96,6 +96,7 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
     nb, nl, ns = s.node_counts()
     out["nodes"] = dict(bulk=nb, list=nl, solid=ns)
     out["bulk_block"], out["row_order"] = s.launch_info()
+    out["population_stride"] = s.population_stride()  # N, or the padded stride the autotuner chose
     # One kernel launch per step, back to back on the solver's stream: the step kernel's average
     # duration is the time between the HIP events around the timed steps over their count (a kernel
     # trace of the same region gives the same average, the launches leave no gaps).
@@ -598,7 +599,8 @@ def main():
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "step_kernel_us", "evented_step_us", "slowest_step_us",
                                                "kernel_achieved_gbs", "bytes_per_cell",
-                                               "bulk_block", "row_order", "nodes", "finite") if kk in v}
+                                               "bulk_block", "row_order", "population_stride", "nodes",
+                                               "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],
         "fp16sr_over_fp32": variants.get("fp16sr_192x96x96", main_run)["mlups"] / main_run["mlups"],

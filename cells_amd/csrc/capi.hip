@@ -53,7 +53,7 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
   const int64_t delta = CZ3[i] * WH + CY3[i] * W + CX3[i];
   const bool outY = (r.loY && CY3[i] > 0) || (r.hiY && CY3[i] < 0);
   const bool outZ = (r.loZ && CZ3[i] > 0) || (r.hiZ && CZ3[i] < 0);
-  if (outlet && CX3[i] < 0) return uint32_t(i) * s->N + idx - 1u;
+  if (outlet && CX3[i] < 0) return uint32_t(i) * s->NS + idx - 1u;
   int sd = i;
   int64_t sdelta = delta;
   if (outY && outZ) {
@@ -66,7 +66,7 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
     sd = mirror_z3(i);
     sdelta = CY3[i] * W + CX3[i];
   }
-  return uint32_t(int64_t(sd) * s->N + int64_t(idx) - sdelta);
+  return uint32_t(int64_t(sd) * s->NS + int64_t(idx) - sdelta);
 }
 
 }  // namespace lbm
@@ -128,6 +128,7 @@ void write_params3(lbm_solver* s) {
   p.H = s->H;
   p.D = s->D;
   p.N = s->N;
+  p.NS = s->NS;
   p.WH = uint32_t(s->W) * uint32_t(s->H);
   p.tau = float(c.tau);
   p.lambda = float(std::isnan(c.lambda) ? std::min(MAGIC_LAMBDA, 50.0 * (c.tau - 0.5) * (c.tau - 0.5)) : c.lambda);
@@ -178,7 +179,7 @@ float link_q(float dF, float dS) {
 
 int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H, D = s->D;
-  const uint32_t N = s->N;
+  const uint32_t N = s->NS;  // population stride of the gather indices
   const uint32_t WH = uint32_t(W) * uint32_t(H);
   const int z = int(idx / WH), y = int((idx % WH) / uint32_t(W)), x = int(idx % uint32_t(W));
   const int j = opp3(i);
@@ -379,7 +380,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
     }
   }
   // Every index must address a population of the grid before anything is launched with it.
-  const uint64_t lim = uint64_t(Q) * N;
+  const uint64_t lim = uint64_t(Q) * s->NS;
   for (uint32_t v : gt)
     if (v >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: gather index out of range");
   for (const BzLink& r : recs)
@@ -555,6 +556,65 @@ int tables_done(lbm_solver* s, int rc) {
 
 int rebuild_flags_impl(lbm_solver* s) { return tables_done(s, rebuild_core(s)); }
 
+// Moves both population buffers to population stride ns (a pitched device copy into new buffers,
+// their padding zeroed) and rebuilds the boundary tables, whose gather indices carry the stride.
+// A failure leaves the buffers, the stride and the tables as they were.
+int set_stride_impl(lbm_solver* s, uint32_t ns) {
+  if (ns == s->NS) return LBM_OK;
+  const size_t e = s->elem;
+  void* nf[2] = {nullptr, nullptr};
+  auto drop = [&]() {
+    (void)hipStreamSynchronize(s->stream);
+    for (void* p : nf)
+      if (p) (void)hipFree(p);
+  };
+  const int rc = [&]() -> int {
+    for (int b = 0; b < 2; b++) {
+      HIP_TRY(hipMalloc(&nf[b], size_t(s->Q) * ns * e));
+      HIP_TRY(hipMemsetAsync(nf[b], 0, size_t(s->Q) * ns * e, s->stream));
+      HIP_TRY(hipMemcpy2DAsync(nf[b], size_t(ns) * e, s->f[b], size_t(s->NS) * e, size_t(s->N) * e, size_t(s->Q),
+                               hipMemcpyDeviceToDevice, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return LBM_OK;
+  }();
+  if (rc) {
+    drop();
+    return rc;
+  }
+  const uint32_t old = s->NS;
+  s->NS = ns;
+  std::vector<uint32_t> fl = s->h_flags;
+  if (const int rc2 = build_links(s, fl)) {
+    s->NS = old;
+    drop();
+    return rc2;
+  }
+  for (int b = 0; b < 2; b++) {
+    (void)hipFree(s->f[b]);
+    s->f[b] = nf[b];
+  }
+  s->p3.NS = ns;
+  return LBM_OK;
+}
+
+// Population strides the autotuner times for a D3Q19 solver, N first. The stride decides which
+// memory channels and Infinity Cache sets the 19 streams of one node share: measured on the step
+// (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other grids), N rounded up to 1024
+// elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us and 256 x 128 x 128 FP16
+// 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and 256 x 256 x 128 (FP16
+// 128.9 -> 124.2 us) where 69,632 lost 6%, and neither helped at 192 x 96 x 96 (FP32 45.3 us at N,
+// 45.6 and 50.5 padded). So the choice is timed per grid. None while an adjoint state exists (its
+// checkpoints hold buffers at the current stride) and none for small grids (N < 65,536).
+std::vector<uint32_t> stride_candidates(const lbm_solver* s) {
+  std::vector<uint32_t> c{s->N};
+  if (s->nd != 3 || s->Q != 19 || s->adj || s->N < 65536u) return c;
+  const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
+  for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
+    if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back(uint32_t(base + pad));
+  return c;
+}
+
 
 int alloc_common(lbm_solver* s) {
   const size_t N = s->N;
@@ -563,7 +623,7 @@ int alloc_common(lbm_solver* s) {
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->owns_stream = true;
   }
-  const size_t fbytes = size_t(s->Q) * N * s->elem;
+  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;
   HIP_TRY(hipMalloc(&s->f[0], fbytes));
   HIP_TRY(hipMalloc(&s->f[1], fbytes));
   HIP_TRY(hipMalloc(&s->flags, 4 * N));
@@ -737,6 +797,7 @@ int lbm::create_2d_on(const lbm_config2d* cfg, int device, hipStream_t stream, l
   s->W = cfg->width;
   s->H = cfg->height;
   s->N = uint32_t(N);
+  s->NS = s->N;
   s->Q = 9;
   s->elem = 4;
   s->cfg2 = *cfg;
@@ -788,6 +849,7 @@ int lbm_create_3d(const lbm_config3d* cfg, int device, lbm_solver** out) {
   s->H = cfg->height;
   s->D = cfg->depth;
   s->N = uint32_t(N);
+  s->NS = s->N;
   s->Q = Q;
   s->prec = cfg->precision;
   s->elem = elem;
@@ -1152,7 +1214,13 @@ int lbm_info(const lbm_solver* s, int32_t* dims, int64_t* n_cells, int64_t* step
 int lbm_read_populations(lbm_solver* s, void* out, size_t bytes) {
   const size_t need = size_t(s ? s->Q : 0) * (s ? s->N : 0) * (s ? s->elem : 0);
   if (!s || !out || bytes < need) return fail(LBM_ERR_INVALID, "out too small");
-  HIP_TRY(hipMemcpyAsync(out, s->f[s->parity], need, hipMemcpyDeviceToHost, s->stream));
+  // Q x N, population-major, whatever the device stride: a padded stride is copied row by row
+  if (s->NS == s->N) {
+    HIP_TRY(hipMemcpyAsync(out, s->f[s->parity], need, hipMemcpyDeviceToHost, s->stream));
+  } else {
+    HIP_TRY(hipMemcpy2DAsync(out, size_t(s->N) * s->elem, s->f[s->parity], size_t(s->NS) * s->elem,
+                             size_t(s->N) * s->elem, size_t(s->Q), hipMemcpyDeviceToHost, s->stream));
+  }
   HIP_TRY(hipStreamSynchronize(s->stream));
   return LBM_OK;
 }
@@ -1227,7 +1295,11 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   if (!s || trial_steps <= 0) return fail(LBM_ERR_INVALID, "bad arguments");
   if (int rc = check_usable(s)) return rc;
   static std::mutex cache_mu;
-  static std::map<std::string, std::pair<int, int>> cache;
+  struct Pick {
+    int block, sweep;
+    uint32_t stride;
+  };
+  static std::map<std::string, Pick> cache;
   // Everything that selects a different kernel instantiation or changes the work of a step is part
   // of the key: lattice and velocity set, precision, grid, open-face kind, subgrid model on/off,
   // force cadence.
@@ -1240,17 +1312,22 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     std::lock_guard<std::mutex> lk(cache_mu);
     auto it = cache.find(key);
     if (it != cache.end()) {
-      s->bulk_block = it->second.first;
-      s->sweep = it->second.second;
+      const Pick pk = it->second;
+      const std::vector<uint32_t> strides = stride_candidates(s);
+      if (std::find(strides.begin(), strides.end(), pk.stride) != strides.end())
+        if (int rc = set_stride_impl(s, pk.stride)) return rc;
+      s->bulk_block = pk.block;
+      s->sweep = pk.sweep;
       if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : s->bulk_block;
       return LBM_OK;
     }
   }
   // Trials run on the solver's own buffers and sample forces as it is configured to (the sampling
   // changes which launch shape wins). The populations, the force ring and its counter are saved
-  // first; the guard restores them, the step, parity and pending sample steps, and frees the
-  // copies, whether the trials finish or a HIP call fails part way.
-  const size_t fbytes = size_t(s->Q) * s->N * s->elem;
+  // first; the guard restores them (into the buffers at whatever population stride is current), the
+  // step, parity and pending sample steps, and frees the copies, whether the trials finish or a HIP
+  // call fails part way; after a failure it also moves the populations back to the stride they had.
+  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;
   const size_t ring_bytes = sizeof(double) * 3 * HISTORY_LEN;
   struct Guard {
     lbm_solver* s;
@@ -1261,14 +1338,17 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     hipEvent_t a = nullptr, b = nullptr;
     int64_t step0;
     int parity0, block0, sweep0;
+    uint32_t ns0;
     int64_t pushed0;
     std::vector<int64_t> steps0;
     bool saved = false;  // every copy was enqueued: from here on the state must be restored
     bool keep_shape = false;
     int restore() {
       saved = false;
-      HIP_TRY(hipMemcpyAsync(s->f[0], save[0], fbytes, hipMemcpyDeviceToDevice, s->stream));
-      HIP_TRY(hipMemcpyAsync(s->f[1], save[1], fbytes, hipMemcpyDeviceToDevice, s->stream));
+      const size_t e = s->elem;
+      for (int b = 0; b < 2; b++)
+        HIP_TRY(hipMemcpy2DAsync(s->f[b], size_t(s->NS) * e, save[b], size_t(ns0) * e, size_t(s->N) * e,
+                                 size_t(s->Q), hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipMemcpyAsync(s->ring, ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipMemcpyAsync(s->state, state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1276,6 +1356,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     }
     ~Guard() {
       if (saved) (void)restore();
+      if (!keep_shape && s->NS != ns0) (void)set_stride_impl(s, ns0);
       (void)hipStreamSynchronize(s->stream);
       for (void* p : {save[0], save[1], ring, state})
         if (p) (void)hipFree(p);
@@ -1295,6 +1376,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   g.parity0 = s->parity;
   g.block0 = s->bulk_block;
   g.sweep0 = s->sweep;
+  g.ns0 = s->NS;
   g.pushed0 = s->samples_pushed;
   g.steps0 = s->sample_steps;
   HIP_TRY(hipMalloc(&g.save[0], fbytes));
@@ -1328,24 +1410,26 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       for (int sw : {1, 2, 0}) cand.push_back({bx, sw});  // D2Q9: the three row orders
     }
   }
+  auto trial = [&](size_t c, double* out) -> int {
+    s->bulk_block = cand[c].first;
+    s->sweep = cand[c].second;
+    // untimed steps first: the previous candidate's row order leaves the Infinity Cache holding
+    // other rows than this one's steady state does
+    for (int k = 0; k < 8; k++) enqueue_one(s);
+    HIP_TRY(hipEventRecord(g.a, s->stream));
+    for (int k = 0; k < trial_steps; k++) enqueue_one(s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g.b, s->stream));
+    HIP_TRY(hipEventSynchronize(g.b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, g.a, g.b));
+    *out = std::min(*out, double(ms));
+    return LBM_OK;
+  };
   std::vector<double> best_of(cand.size(), 1e300);
-  for (int rep = 0; rep < 3; rep++) {
-    for (size_t c = 0; c < cand.size(); c++) {
-      s->bulk_block = cand[c].first;
-      s->sweep = cand[c].second;
-      // untimed steps first: the previous candidate's row order leaves the Infinity Cache holding
-      // other rows than this one's steady state does
-      for (int k = 0; k < 8; k++) enqueue_one(s);
-      HIP_TRY(hipEventRecord(g.a, s->stream));
-      for (int k = 0; k < trial_steps; k++) enqueue_one(s);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipEventRecord(g.b, s->stream));
-      HIP_TRY(hipEventSynchronize(g.b));
-      float ms = 0;
-      HIP_TRY(hipEventElapsedTime(&ms, g.a, g.b));
-      best_of[c] = std::min(best_of[c], double(ms));
-    }
-  }
+  for (int rep = 0; rep < 3; rep++)
+    for (size_t c = 0; c < cand.size(); c++)
+      if (int rc = trial(c, &best_of[c])) return rc;
   for (size_t c = 0; c < cand.size(); c++) {
     if (best_of[c] < best_ms) {
       best_ms = best_of[c];
@@ -1353,13 +1437,44 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       best_sweep = cand[c].second;
     }
   }
+  // The population stride (D3Q19): the three fastest shapes again at every candidate stride, the
+  // strides visited twice in opposite orders; the fastest (stride, shape) pair of these runs wins.
+  uint32_t best_ns = s->NS;
+  const std::vector<uint32_t> strides = stride_candidates(s);
+  if (strides.size() > 1) {
+    std::vector<size_t> top(cand.size());
+    for (size_t c = 0; c < cand.size(); c++) top[c] = c;
+    std::sort(top.begin(), top.end(), [&](size_t a, size_t b) { return best_of[a] < best_of[b]; });
+    top.resize(std::min<size_t>(3, top.size()));
+    std::vector<double> t2(strides.size() * top.size(), 1e300);
+    for (int rep = 0; rep < 2; rep++) {
+      for (size_t k = 0; k < strides.size(); k++) {
+        const size_t si = rep == 0 ? k : strides.size() - 1 - k;
+        if (int rc = set_stride_impl(s, strides[si])) return rc;
+        for (size_t t = 0; t < top.size(); t++)
+          if (int rc = trial(top[t], &t2[si * top.size() + t])) return rc;
+      }
+    }
+    double b2 = 1e300;
+    for (size_t si = 0; si < strides.size(); si++) {
+      for (size_t t = 0; t < top.size(); t++) {
+        if (t2[si * top.size() + t] < b2) {
+          b2 = t2[si * top.size() + t];
+          best_ns = strides[si];
+          best = cand[top[t]].first;
+          best_sweep = cand[top[t]].second;
+        }
+      }
+    }
+  }
+  if (int rc = set_stride_impl(s, best_ns)) return rc;
   if (int rc = g.restore()) return rc;
   s->bulk_block = best;
   s->sweep = best_sweep;
   g.keep_shape = true;
   {
     std::lock_guard<std::mutex> lk(cache_mu);
-    cache[key] = {best, best_sweep};
+    cache[key] = {best, best_sweep, best_ns};
   }
   if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : best;
   return LBM_OK;
@@ -1388,6 +1503,27 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep) {
   s->bulk_block = bb;
   s->sweep = sweep;
   return LBM_OK;
+}
+
+int lbm_population_stride(lbm_solver* s, int64_t* stride) {
+  if (!s || !stride) return fail(LBM_ERR_INVALID, "bad arguments");
+  *stride = s->NS;
+  return LBM_OK;
+}
+
+int lbm_set_population_stride(lbm_solver* s, int64_t stride) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (int rc = check_usable(s)) return rc;
+  if (stride == 0) stride = s->N;
+  if (stride == int64_t(s->NS)) return LBM_OK;
+  if (stride != int64_t(s->N)) {
+    if (s->nd != 3 || s->Q != 19) return fail(LBM_ERR_UNSUPPORTED, "population stride: a padded stride is D3Q19 only");
+    if (stride < int64_t(s->N) || stride % 64 != 0 || uint64_t(s->Q) * uint64_t(stride) >= (uint64_t(1) << 32))
+      return fail(LBM_ERR_INVALID, "population stride: a multiple of 64 elements, at least N, with 19 x stride below 2^32");
+  }
+  if (s->adj)
+    return fail(LBM_ERR_UNSUPPORTED, "population stride: release the adjoint state first (its checkpoints hold the buffers)");
+  return set_stride_impl(s, uint32_t(stride));
 }
 
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid) {

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     if (outY || outZ) M |= 1u << i;
   }
   const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
-  const size_t N = p.N;
+  const size_t N = p.NS;  // population stride
   const uint32_t off = idx * uint32_t(sizeof(E));
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[19];
@@ -410,8 +410,9 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
     // Only x = W-1 off the faces can hold an outlet node (face-row outlet nodes are on the boundary
     // list). Every address lies inside the population buffer: a pull that leaves the grid through
     // x = 0 or W-1 reads the neighbouring row's end of the same population, at worst element -1 or
-    // N of it, the previous / next population's end; the directions with c_x != 0 are numbered 1-16,
-    // so that stays within elements 0 .. 19 N - 1.
+    // N of it, the previous population's end (or pad) / this population's pad or the next one's first
+    // element; the directions with c_x != 0 are numbered 1-16, so that stays within elements
+    // 0 .. 19 NS - 1.
     clean = *clean_byte;
     pull(!face && x == uint32_t(p.W - 1));
   } else {
@@ -539,7 +540,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
                                            const uint32_t* __restrict__ flags, const ListTables& lt, uint32_t k,
                                            const Params3& p, const FeqIn& fin, const float4* __restrict__ xtab,
                                            const Record& rc, float (&force)[3]) {
-  const uint32_t N = p.N;
+  const uint32_t N = p.NS;  // population stride
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
   const uint32_t fl = flags[idx];
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(256) void macro3_kernel(const typename S::elem* __r
   float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 19; i++) {
-    const float v = S::load(f + size_t(i) * p.N, idx);
+    const float v = S::load(f + size_t(i) * p.NS, idx);
     drho += v;
     jx += float(CX3[i]) * v;
     jy += float(CY3[i]) * v;
@@ -646,8 +647,8 @@ __global__ __launch_bounds__(256) void init3_kernel(typename S::elem* __restrict
 #pragma unroll
   for (int i = 0; i < 19; i++) {
     const float v = solid ? 0.0f : feq3(i, 1.0f, p.uIn, 0.0f, 0.0f);
-    S::store(fa + size_t(i) * p.N, idx, v);
-    S::store(fb + size_t(i) * p.N, idx, v);
+    S::store(fa + size_t(i) * p.NS, idx, v);
+    S::store(fb + size_t(i) * p.NS, idx, v);
   }
 }
 
@@ -715,8 +716,8 @@ __global__ __launch_bounds__(256) void flags3_kernel(const float* __restrict__ s
 #pragma unroll
     for (int i = 0; i < 19; i++) {
       const float v = feq3(i, rho, ux, uy, uz);
-      S::store(fa + size_t(i) * p.N, idx, v);
-      S::store(fb + size_t(i) * p.N, idx, v);
+      S::store(fa + size_t(i) * p.NS, idx, v);
+      S::store(fb + size_t(i) * p.NS, idx, v);
     }
   }
 }

@@ -189,6 +189,10 @@ struct Params3 {
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
   uint32_t seed;   // lowbias32(step index): the stochastic codec's per-step seed
   uint32_t faces;  // open x planes: 0 equilibrium (the reference's), 1 Zou-He faces (Hecht-Harting)
+  // Elements from one population's first cell to the next one's: N, or for D3Q19 N padded (the
+  // autotuner times a few strides; population i of cell idx is element i * NS + idx, the elements
+  // NS - N after each population are never read). N stays the cell count.
+  uint32_t NS;
 };
 
 // Where the step kernels put a force sample: per-block FP64 partials, a completion counter,

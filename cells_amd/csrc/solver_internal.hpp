@@ -58,6 +58,9 @@ struct lbm_solver {
   bool owns_stream = true;
   int W = 1, H = 1, D = 1;
   uint32_t N = 0;
+  // Population stride in elements (population i of cell idx is element i * NS + idx of a buffer):
+  // N, except for a D3Q19 solver whose autotuner or caller chose a padded stride (lbm_set_population_stride).
+  uint32_t NS = 0;
   int Q = 9;
   int prec = LBM_FP32;
   size_t elem = 4;

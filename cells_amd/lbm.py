@@ -176,6 +176,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_autotune": ([P, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
         "lbm_launch_info": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
         "lbm_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
+        "lbm_population_stride": ([P, C.POINTER(C.c_int64)], C.c_int),
+        "lbm_set_population_stride": ([P, C.c_int64], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
                               C.c_int),
         "lbm_copy_roofline_filled": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -369,6 +371,17 @@ class _Base:
     def set_launch(self, block: int, variant: int = 0):
         """Pins the launch shape instead of autotune (same meaning as launch_info)."""
         _check(_lib.lbm_set_launch(self._h, int(block), int(variant)))
+
+    def population_stride(self) -> int:
+        """Elements between consecutive populations in the device buffers: N, or for D3Q19 a padded
+        stride the autotuner or set_population_stride chose (host-visible layouts stay Q x N)."""
+        v = C.c_int64()
+        _check(_lib.lbm_population_stride(self._h, C.byref(v)))
+        return int(v.value)
+
+    def set_population_stride(self, stride: int = 0):
+        """Moves the populations to another stride (0 or N: unpadded; D3Q19: a multiple of 64 >= N)."""
+        _check(_lib.lbm_set_population_stride(self._h, int(stride)))
 
     def node_counts(self):
         """(bulk, list, solid) node counts."""

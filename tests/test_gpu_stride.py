@@ -1,0 +1,196 @@
+"""The D3Q19 population stride (lbm_set_population_stride, and the stride stage of lbm_autotune): the
+device buffers may hold each population at a padded stride, which changes which memory channels a
+node's 19 streams share and nothing a caller can see. Every test here runs a padded solver beside an
+unpadded one on the same inputs and requires the same bits: populations, force samples, flags, macro
+fields, refilled cells after a geometry change, and the adjoint's gradient."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from cells_amd import geometry as g
+from cells_amd import shapes
+from cells_amd.adjoint import ShapeAdjoint
+from cells_amd.lbm import LbmError, Solver, Solver3D
+from tests.parity_cases import case3
+
+pytestmark = pytest.mark.gpu
+
+
+def _padded(N, pad):
+    return (N + 1023) // 1024 * 1024 + pad
+
+
+def _make(cfg, sdf, launch=None, faces=None):
+    s = Solver3D.create(dict(cfg))
+    s.set_sdf(sdf)
+    if faces:
+        s.set_open_faces(faces)
+    s.init_field()
+    if launch:
+        s.set_launch(*launch)
+    return s
+
+
+def _state(s):
+    s.encode_macro()
+    return dict(pop=s.read_populations_raw(), forces=s.read_forces_raw(), flags=s.read_flags(), macro=s.read_macro())
+
+
+def _same(a, b):
+    assert np.array_equal(a["pop"], b["pop"]), int(np.count_nonzero(a["pop"] != b["pop"]))
+    assert np.array_equal(a["forces"][0], b["forces"][0])
+    assert np.array_equal(a["forces"][1], b["forces"][1])
+    assert np.array_equal(a["flags"], b["flags"])
+    assert np.array_equal(a["macro"], b["macro"])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16", "bf16", "fp16sr"])
+def test_padded_stride_gives_identical_bits(precision):
+    """A row of 150 nodes (blocks split it unevenly), N = 144,000 (not a multiple of 64), two bodies
+    (Bouzidi links, a slip face through a solid node): the stride moved three times during the run."""
+    cfg, sdf = case3(precision, W=150, H=40, D=24)
+    ref = _make(cfg, sdf, launch=(64, 1))
+    pad = _make(cfg, sdf, launch=(64, 1))
+    N = pad.N
+    assert pad.population_stride() == N
+    s1, s2 = _padded(N, 4096), _padded(N, 17 * 4096)
+    pad.set_population_stride(s1)
+    assert pad.population_stride() == s1
+    ref.run(25)
+    pad.run(25)
+    pad.set_population_stride(s2)
+    ref.run(20)
+    pad.run(20)
+    pad.set_population_stride(s1)
+    ref.run(15)
+    pad.run(15)
+    assert pad.population_stride() == s1
+    assert pad.node_counts() == ref.node_counts()
+    _same(_state(ref), _state(pad))
+    pad.set_population_stride(0)
+    assert pad.population_stride() == N
+    ref.run(5)
+    pad.run(5)
+    _same(_state(ref), _state(pad))
+    ref.destroy()
+    pad.destroy()
+
+
+@pytest.mark.parametrize("faces,smag,variant", [("equilibrium", 0.0, 0), ("equilibrium", 0.0, 14),
+                                                ("zou-he", 0.0, 5), ("zou-he", 0.0, 12), ("equilibrium", 0.12, 2)])
+def test_padded_stride_kernel_variants(faces, smag, variant):
+    """Each kernel instantiation reads the stride: per-node and shared stores, both pull orders, the
+    Zou-He faces and the subgrid model."""
+    cfg, sdf = case3("fp32", W=150, H=40, D=24, smag=smag)
+    ref = _make(cfg, sdf, launch=(128, variant), faces=faces)
+    pad = _make(cfg, sdf, launch=(128, variant), faces=faces)
+    pad.set_population_stride(_padded(pad.N, 17 * 4096))
+    ref.run(30)
+    pad.run(30)
+    _same(_state(ref), _state(pad))
+    ref.destroy()
+    pad.destroy()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_padded_stride_geometry_change_refills(precision):
+    """setSdf on a developed flow at a padded stride: the flag kernel refills uncovered cells into the
+    padded buffers and the boundary tables are rebuilt with the stride."""
+    cfg, sdf = case3(precision, W=150, H=40, D=24)
+    W, H, D = 150, 40, 24
+    ref = _make(cfg, sdf, launch=(256, 2))
+    pad = _make(cfg, sdf, launch=(256, 2))
+    pad.set_population_stride(_padded(pad.N, 4096))
+    ref.run(30)
+    pad.run(30)
+    sdf2 = g.empty_sdf3(W, H, D)
+    g.add_sphere(sdf2, W, H, D, 11.0, 7.5, 6.5, 3.5)
+    g.add_sphere(sdf2, W, H, D, 40, 30, 0.5, 4.2)
+    for s in (ref, pad):
+        s.set_sdf(sdf2)
+    _same(_state(ref), _state(pad))
+    ref.run(15)
+    pad.run(15)
+    _same(_state(ref), _state(pad))
+    ref.destroy()
+    pad.destroy()
+
+
+def test_autotune_times_strides_and_restores_state():
+    """N = 144,000 >= 65,536: the autotuner's stride stage runs. Its pick is one of the candidates,
+    the state it leaves is the state it found (at whatever stride it chose), a second solver of the
+    same configuration gets the same pick from the process cache, and the run continues bit for bit
+    like an unpadded solver pinned to the same launch shape."""
+    cfg, sdf = case3("fp16", W=150, H=40, D=24)
+    a = _make(cfg, sdf)
+    a.run(20)
+    before = _state(a)
+    a.autotune(10)
+    N = a.N
+    stride = a.population_stride()
+    assert stride in (N, _padded(N, 4096), _padded(N, 17 * 4096))
+    after = _state(a)
+    assert np.array_equal(before["pop"], after["pop"])
+    assert np.array_equal(before["macro"], after["macro"])
+    block, variant = a.launch_info()
+    b = _make(cfg, sdf, launch=(block, variant))
+    b.run(20)
+    b.read_forces_raw()
+    a.run(25)
+    b.run(25)
+    _same(_state(a), _state(b))
+    c = _make(cfg, sdf)
+    c.autotune(10)
+    assert c.population_stride() == stride and c.launch_info() == (block, variant)
+    for s in (a, b, c):
+        s.destroy()
+
+
+def test_adjoint_gradient_identical_at_padded_stride():
+    """The adjoint's checkpoints hold whole buffers at the solver's stride and its tables decode the
+    forward gather indices through it: J and dJ/dsdf are the same bits with a padded forward state.
+    A stride change is refused while the adjoint state exists."""
+    W, H, D, u, r = 48, 24, 24, 0.05, 5.0
+    sdf0, _ = shapes.sphere_sh(W, H, D, (12.0, 11.5, 11.5), r, np.zeros(16))
+    out = {}
+    for padded in (False, True):
+        cfg = dict(width=W, height=H, depth=D, precision="fp16", tau=0.56, uIn=u, spongeFraction=0.15,
+                   spongeTau=1, absorb=0.02, forceEvery=1)
+        s = Solver3D.create(cfg)
+        if padded:
+            s.set_population_stride(_padded(s.N, 17 * 4096))
+        adj = ShapeAdjoint(s, 150, 50, (1.0, 0.0, 0.0), 1.0, checkpoint_every=50, ramp=("smoothstep", u, 50))
+        J = adj.forward(sdf0)
+        out[padded] = (J, adj.backward())
+        if padded:
+            with pytest.raises(LbmError):
+                s.set_population_stride(0)
+            adj.release()
+            s.set_population_stride(0)
+            assert s.population_stride() == s.N
+        s.destroy()
+    assert out[False][0] == out[True][0]
+    assert np.array_equal(out[False][1], out[True][1])
+
+
+def test_stride_argument_checks():
+    cfg, sdf = case3("fp32", W=150, H=40, D=24)
+    s = _make(cfg, sdf)
+    N = s.N
+    for bad in (N - 64, _padded(N, 4096) + 32, 1 << 30):
+        with pytest.raises(LbmError):
+            s.set_population_stride(bad)
+    assert s.population_stride() == N
+    s.destroy()
+    cfg27 = dict(cfg, velocitySet=27)
+    q = _make(cfg27, sdf)
+    with pytest.raises(LbmError):
+        q.set_population_stride(_padded(q.N, 4096))
+    q.set_population_stride(0)
+    assert q.population_stride() == q.N
+    q.destroy()
+    d2 = Solver.create(dict(width=128, height=64))
+    with pytest.raises(LbmError):
+        d2.set_population_stride(_padded(d2.N, 4096))
+    d2.destroy()
