@@ -64,14 +64,18 @@ def tunnel_sdf(W, H, D):
 
 # Launch shapes pinned from the command line (--launch / --batch-launch) instead of autotuned, so that
 # counter passes, which slow every dispatch and can change the autotuner's choice, measure one shape.
-PIN = {"launch": None, "batch": None}
+PIN = {"launch": None, "batch": None, "stride": None}
 
 
 def tune_or_pin(s, trial_steps):
     if PIN["launch"] is not None and s.Q == 19:
         s.set_launch(*PIN["launch"])
-        return s.launch_info()[0]
-    return s.autotune(trial_steps)
+        block = s.launch_info()[0]
+    else:
+        block = s.autotune(trial_steps)
+    if PIN["stride"] is not None and s.Q == 19:
+        s.set_population_stride(PIN["stride"])
+    return block
 
 
 def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
@@ -539,11 +543,15 @@ def main():
                     "(for kernel traces and counter passes of that workload)")
     ap.add_argument("--launch", help="pin the D2Q9 / D3Q19 launch shape: BLOCK,VARIANT (lbm_set_launch)")
     ap.add_argument("--batch-launch", help="pin the batch launch shape: TILE,ORDER (lbm_batch_set_launch)")
+    ap.add_argument("--stride", type=int, help="pin the D3Q19 population stride in elements, 0 = N "
+                    "(lbm_set_population_stride)")
     args = ap.parse_args()
     if args.launch:
         PIN["launch"] = tuple(int(v) for v in args.launch.split(","))
     if args.batch_launch:
         PIN["batch"] = tuple(int(v) for v in args.batch_launch.split(","))
+    if args.stride is not None:
+        PIN["stride"] = args.stride
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
               f"across GPUs; see the sharding design notes).", file=sys.stderr)
