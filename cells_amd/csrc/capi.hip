@@ -557,23 +557,45 @@ int tables_done(lbm_solver* s, int rc) {
 int rebuild_flags_impl(lbm_solver* s) { return tables_done(s, rebuild_core(s)); }
 
 // Moves both population buffers to population stride ns (a pitched device copy into new buffers,
-// their padding zeroed) and rebuilds the boundary tables, whose gather indices carry the stride.
-// A failure leaves the buffers, the stride and the tables as they were.
+// their padding zeroed) and re-strides the boundary tables. Which nodes the bulk rows take does not
+// depend on the stride (a bulk node's pulls are the same (population, cell) pairs at any stride), so
+// only the indices that carry it change: every gather index and Bouzidi second operand is
+// population * NS + cell with cell < N, and becomes population * ns + cell. A failure leaves the
+// buffers, the stride and the tables as they were.
 int set_stride_impl(lbm_solver* s, uint32_t ns) {
   if (ns == s->NS) return LBM_OK;
+  const uint32_t old = s->NS;
   const size_t e = s->elem;
+  auto restride = [&](uint32_t g) { return (g / old) * ns + g % old; };
+  std::vector<uint32_t> gt(s->h_gtab.size());
+  for (size_t k = 0; k < gt.size(); k++) gt[k] = restride(s->h_gtab[k]);
+  std::vector<BzLink> lk = s->h_links;
+  for (BzLink& r : lk)
+    if (r.gb != NO_OPERAND) r.gb = restride(r.gb);
   void* nf[2] = {nullptr, nullptr};
+  uint32_t* ngt = nullptr;
+  BzLink* nlk = nullptr;
   auto drop = [&]() {
     (void)hipStreamSynchronize(s->stream);
-    for (void* p : nf)
+    for (void* p : {nf[0], nf[1], (void*)ngt, (void*)nlk})
       if (p) (void)hipFree(p);
   };
   const int rc = [&]() -> int {
     for (int b = 0; b < 2; b++) {
       HIP_TRY(hipMalloc(&nf[b], size_t(s->Q) * ns * e));
       HIP_TRY(hipMemsetAsync(nf[b], 0, size_t(s->Q) * ns * e, s->stream));
-      HIP_TRY(hipMemcpy2DAsync(nf[b], size_t(ns) * e, s->f[b], size_t(s->NS) * e, size_t(s->N) * e, size_t(s->Q),
+      HIP_TRY(hipMemcpy2DAsync(nf[b], size_t(ns) * e, s->f[b], size_t(old) * e, size_t(s->N) * e, size_t(s->Q),
                                hipMemcpyDeviceToDevice, s->stream));
+    }
+    if (s->gtab) {
+      HIP_TRY(hipMalloc(&ngt, sizeof(uint32_t) * std::max<size_t>(1, gt.size())));
+      if (!gt.empty())
+        HIP_TRY(hipMemcpyAsync(ngt, gt.data(), sizeof(uint32_t) * gt.size(), hipMemcpyHostToDevice, s->stream));
+    }
+    if (s->links) {
+      HIP_TRY(hipMalloc(&nlk, sizeof(BzLink) * std::max<size_t>(1, lk.size())));
+      if (!lk.empty())
+        HIP_TRY(hipMemcpyAsync(nlk, lk.data(), sizeof(BzLink) * lk.size(), hipMemcpyHostToDevice, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return LBM_OK;
@@ -582,18 +604,23 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
     drop();
     return rc;
   }
-  const uint32_t old = s->NS;
-  s->NS = ns;
-  std::vector<uint32_t> fl = s->h_flags;
-  if (const int rc2 = build_links(s, fl)) {
-    s->NS = old;
-    drop();
-    return rc2;
-  }
+  // the stream is idle: no step still reads the old buffers
   for (int b = 0; b < 2; b++) {
     (void)hipFree(s->f[b]);
     s->f[b] = nf[b];
   }
+  if (s->gtab) {
+    (void)hipFree(s->gtab);
+    s->gtab = ngt;
+  }
+  if (s->links) {
+    (void)hipFree(s->links);
+    s->links = nlk;
+  }
+  s->h_gtab = std::move(gt);
+  s->h_links = std::move(lk);
+  s->tables_gen++;
+  s->NS = ns;
   s->p3.NS = ns;
   return LBM_OK;
 }
