@@ -117,6 +117,7 @@ struct lbm_batch {
   int ramp_kind = 0;
   double ramp_steps = 0;
   bool dirty = true;
+  std::vector<uint64_t> seen_gen;  // each tunnel's boundary-table generation at the last refresh
   uint32_t tile = 256;  // cells (bulk) or list nodes per block
   int order = 0;        // bit 0: tunnels in creation order (0) or largest first (1); bit 1: bulk tiles
                         // walked from the end of the table on the steps of parity 1
@@ -141,9 +142,24 @@ void drop_graphs(lbm_batch* b) {
     }
 }
 
+// Whether a tunnel changed since the descriptors were built, also through a handle the caller kept
+// instead of fetching it with lbm_batch_tunnel: a geometry rebuild or a population-stride change
+// replaces its table and population buffers (and bumps tables_gen), a parameter update its Params2.
+bool tunnels_changed(const lbm_batch* b) {
+  if (b->desc.size() != b->t.size() || b->seen_gen.size() != b->t.size()) return true;
+  for (size_t i = 0; i < b->t.size(); i++) {
+    const lbm_solver* s = b->t[i];
+    const TunnelDesc& d = b->desc[i];
+    if (s->tables_gen != b->seen_gen[i] || std::memcmp(&s->p2, &d.p, sizeof(Params2)) != 0 ||
+        d.force_every != s->cfg2.force_every || d.u_target != s->cfg2.u_in)
+      return true;
+  }
+  return false;
+}
+
 // Rebuilds the descriptor and block tables (after geometry, parameter, ramp or launch-shape changes).
 int refresh(lbm_batch* b) {
-  if (!b->dirty) return LBM_OK;
+  if (!b->dirty && !tunnels_changed(b)) return LBM_OK;
   const size_t n = b->t.size();
   const uint32_t T = b->tile;
   std::vector<size_t> perm(n);
@@ -210,6 +226,8 @@ int refresh(lbm_batch* b) {
   HIP_TRY(hipMemcpyAsync(b->d_block_tunnel, b->block_tunnel.data(), sizeof(uint16_t) * blocks, hipMemcpyHostToDevice,
                          b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));  // host tables are reused on the next refresh
+  b->seen_gen.resize(n);
+  for (size_t i = 0; i < n; i++) b->seen_gen[i] = b->t[i]->tables_gen;
   b->dirty = false;
   return LBM_OK;
 }
@@ -417,7 +435,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
       saved = false;
       for (size_t i = 0; i < b->t.size(); i++) {
         lbm_solver* s = b->t[i];
-        const size_t fb = size_t(s->Q) * s->N * s->elem;
+        const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
         HIP_TRY(hipMemcpyAsync(s->f[0], sv[i].f[0], fb, hipMemcpyDeviceToDevice, b->stream));
         HIP_TRY(hipMemcpyAsync(s->f[1], sv[i].f[1], fb, hipMemcpyDeviceToDevice, b->stream));
         HIP_TRY(hipMemcpyAsync(s->ring, sv[i].ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));
@@ -446,7 +464,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   g.order0 = b->order;
   for (size_t i = 0; i < b->t.size(); i++) {
     lbm_solver* s = b->t[i];
-    const size_t fb = size_t(s->Q) * s->N * s->elem;
+    const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
     HIP_TRY(hipMalloc(&g.sv[i].f[0], fb));
     HIP_TRY(hipMalloc(&g.sv[i].f[1], fb));
     HIP_TRY(hipMalloc(&g.sv[i].ring, sizeof(double) * 3 * HISTORY_LEN));
@@ -456,7 +474,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   HIP_TRY(hipEventCreate(&g.e1));
   for (size_t i = 0; i < b->t.size(); i++) {
     lbm_solver* s = b->t[i];
-    const size_t fb = size_t(s->Q) * s->N * s->elem;
+    const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
     HIP_TRY(hipMemcpyAsync(g.sv[i].f[0], s->f[0], fb, hipMemcpyDeviceToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(g.sv[i].f[1], s->f[1], fb, hipMemcpyDeviceToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(g.sv[i].ring, s->ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));

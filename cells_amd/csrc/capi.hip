@@ -100,6 +100,7 @@ void write_params2(lbm_solver* s) {
   p.W = s->W;
   p.H = s->H;
   p.N = s->N;
+  p.NS = s->NS;
   p.left = c.left;
   p.right = c.right;
   p.bottom = c.bottom;
@@ -230,7 +231,7 @@ int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
 
 int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H;
-  const uint32_t N = s->N;
+  const uint32_t N = s->NS;  // population stride of the gather indices
   const Params2& p = s->p2;
   const int y = int(idx / uint32_t(W)), x = int(idx % uint32_t(W));
   const int j = OPP2[i];
@@ -621,12 +622,13 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
   s->h_links = std::move(lk);
   s->tables_gen++;
   s->NS = ns;
-  s->p3.NS = ns;
+  if (s->nd == 2) s->p2.NS = ns;
+  else s->p3.NS = ns;
   return LBM_OK;
 }
 
-// Population strides the autotuner times for a D3Q19 solver, N first. The stride decides which
-// memory channels and Infinity Cache sets the 19 streams of one node share: measured on the step
+// Population strides the autotuner times for a D3Q19 or D2Q9 solver, N first. The stride decides
+// which memory channels and Infinity Cache sets the streams of one node share: measured on the D3Q19 step
 // (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other grids), N rounded up to 1024
 // elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us and 256 x 128 x 128 FP16
 // 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and 256 x 256 x 128 (FP16
@@ -635,7 +637,7 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
 // checkpoints hold buffers at the current stride) and none for small grids (N < 65,536).
 std::vector<uint32_t> stride_candidates(const lbm_solver* s) {
   std::vector<uint32_t> c{s->N};
-  if (s->nd != 3 || s->Q != 19 || s->adj || s->N < 65536u) return c;
+  if (s->Q == 27 || s->adj || s->N < 65536u) return c;
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
     if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back(uint32_t(base + pad));
@@ -1544,9 +1546,9 @@ int lbm_set_population_stride(lbm_solver* s, int64_t stride) {
   if (stride == 0) stride = s->N;
   if (stride == int64_t(s->NS)) return LBM_OK;
   if (stride != int64_t(s->N)) {
-    if (s->nd != 3 || s->Q != 19) return fail(LBM_ERR_UNSUPPORTED, "population stride: a padded stride is D3Q19 only");
+    if (s->Q == 27) return fail(LBM_ERR_UNSUPPORTED, "population stride: D3Q27 keeps the stride N");
     if (stride < int64_t(s->N) || stride % 64 != 0 || uint64_t(s->Q) * uint64_t(stride) >= (uint64_t(1) << 32))
-      return fail(LBM_ERR_INVALID, "population stride: a multiple of 64 elements, at least N, with 19 x stride below 2^32");
+      return fail(LBM_ERR_INVALID, "population stride: a multiple of 64 elements, at least N, with Q x stride below 2^32");
   }
   if (s->adj)
     return fail(LBM_ERR_UNSUPPORTED, "population stride: release the adjoint state first (its checkpoints hold the buffers)");

@@ -156,7 +156,7 @@ __device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, i
 
 // One plain fluid / plain Zou-He node of the bulk pass (src/shaders/step.ts:177-227 for flags that
 // are 0 or only the off-grid x bits of an open column). Per direction the source address is a
-// wave-uniform base (src + i*N - delta_i) plus the lane's byte offset.
+// wave-uniform base (src + i*NS - delta_i) plus the lane's byte offset.
 template <bool REC>
 __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
@@ -177,7 +177,7 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
     outlet = fl == (FLAG_OUTLET2 | OUTLET_DIRS2);
     if (fl != 0u && !inlet && !outlet) return;
   }
-  const size_t N = p.N;
+  const size_t N = p.NS;  // population stride
   const uint32_t off = idx * 4u;
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[9];
@@ -224,7 +224,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
                                            const ListTables& lt, uint32_t k, const Params2& p, const Record& rc,
                                            float (&force)[2]) {
-  const uint32_t N = p.N;
+  const uint32_t N = p.NS;  // population stride
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
   const uint32_t fl = flags[idx];

@@ -181,6 +181,7 @@ struct Params2 {
   uint32_t N;
   int32_t left, right, bottom, top, parabolic, forcing;
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, uLid, fx, fy, absorb, inletLayer;
+  uint32_t NS;  // population stride in elements, as Params3::NS
 };
 // Params3 (src/shaders/common3d.ts:49-56) after Solver3D.writeParams (src/solver3d.ts:197-208).
 struct Params3 {
@@ -189,9 +190,9 @@ struct Params3 {
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
   uint32_t seed;   // lowbias32(step index): the stochastic codec's per-step seed
   uint32_t faces;  // open x planes: 0 equilibrium (the reference's), 1 Zou-He faces (Hecht-Harting)
-  // Elements from one population's first cell to the next one's: N, or for D3Q19 N padded (the
-  // autotuner times a few strides; population i of cell idx is element i * NS + idx, the elements
-  // NS - N after each population are never read). N stays the cell count.
+  // Elements from one population's first cell to the next one's: N, or N padded (D3Q19 and D2Q9; the
+  // autotuners time a few strides; population i of cell idx is element i * NS + idx, the elements
+  // NS - N after each population hold no population). N stays the cell count.
   uint32_t NS;
 };
 

@@ -1,18 +1,19 @@
-"""The D3Q19 population stride (lbm_set_population_stride, and the stride stage of lbm_autotune): the
-device buffers may hold each population at a padded stride, which changes which memory channels a
-node's 19 streams share and nothing a caller can see. Every test here runs a padded solver beside an
-unpadded one on the same inputs and requires the same bits: populations, force samples, flags, macro
-fields, refilled cells after a geometry change, and the adjoint's gradient."""
+"""The population stride (lbm_set_population_stride, and the stride stage of lbm_autotune; D3Q19 and
+D2Q9): the device buffers may hold each population at a padded stride, which changes which memory
+channels a node's streams share and nothing a caller can see. Every test here runs a padded solver
+beside an unpadded one on the same inputs and requires the same bits: populations, force samples,
+flags, macro fields, refilled cells after a geometry change, batched tunnels, and the adjoint's
+gradient."""
 from __future__ import annotations
 
 import numpy as np
 import pytest
 
 from cells_amd import geometry as g
-from cells_amd import shapes
+from cells_amd import shapes, sweep
 from cells_amd.adjoint import ShapeAdjoint
-from cells_amd.lbm import LbmError, Solver, Solver3D
-from tests.parity_cases import case3
+from cells_amd.lbm import Batch, LbmError, Solver, Solver3D
+from tests.parity_cases import CASES2, case3
 
 pytestmark = pytest.mark.gpu
 
@@ -191,6 +192,69 @@ def test_stride_argument_checks():
     assert q.population_stride() == q.N
     q.destroy()
     d2 = Solver.create(dict(width=128, height=64))
+    d2.set_population_stride(_padded(d2.N, 4096))
+    assert d2.population_stride() == _padded(d2.N, 4096)
     with pytest.raises(LbmError):
-        d2.set_population_stride(_padded(d2.N, 4096))
+        d2.set_population_stride(d2.N - 64)
     d2.destroy()
+
+
+@pytest.mark.parametrize("name", ["tunnel", "channel", "periodic", "cavity"])
+def test_d2q9_padded_stride_gives_identical_bits(name):
+    """D2Q9: Zou-He planes and a Bouzidi cylinder, no-slip walls with the subgrid model, periodic seams
+    with a body across them and a body force, a moving lid; the stride moved twice during the run."""
+    cfg, sdf_fn = CASES2[name]
+    W, H = cfg["width"], cfg["height"]
+
+    def make():
+        s = Solver.create(dict(cfg))
+        if sdf_fn:
+            s.set_sdf(sdf_fn(W, H))
+        s.init_field()
+        s.set_launch(64, 1)
+        return s
+
+    ref, pad = make(), make()
+    N = pad.N
+    pad.set_population_stride(_padded(N, 4096))
+    ref.run(20)
+    pad.run(20)
+    pad.set_population_stride(_padded(N, 17 * 4096))
+    ref.run(25)
+    pad.run(25)
+    assert pad.node_counts() == ref.node_counts()
+    _same(_state(ref), _state(pad))
+    ref.destroy()
+    pad.destroy()
+
+
+def test_batch_with_padded_tunnels_gives_identical_bits():
+    """Batched tunnels whose strides change through the handles the caller kept (without fetching them
+    again from the batch): the batch notices the new buffers and tables before its next launch, its
+    autotuner saves and restores the padded buffers, and every tunnel stores the bits of the unpadded
+    batch."""
+    cases = sweep.sweep_cases(n=5, seed=7, sizes=(64, 96, 160))
+
+    def run(pads):
+        b = Batch.create([c["config"] for c in cases])
+        for i, c in enumerate(cases):
+            b.tunnel(i).set_sdf(sweep.case_sdf(c))
+            b.tunnel(i).init_field()
+        b.reset_clock()
+        b.set_launch(256, 2)
+        b.run(40)
+        for i, t in enumerate(b.tunnels):
+            if pads:
+                t.set_population_stride(_padded(t.N, pads[i % len(pads)]))
+        b.autotune(5)
+        b.run(70)
+        out = [(t.read_populations_raw(), t.read_forces_raw()[1], t.population_stride()) for t in b.tunnels]
+        b.destroy()
+        return out
+
+    ref = run(())
+    got = run((4096, 17 * 4096, 0))
+    assert any(g[2] != r[2] for g, r in zip(got, ref))
+    for (pg, fg, _), (pr, fr, _) in zip(got, ref):
+        np.testing.assert_array_equal(pg, pr)
+        np.testing.assert_array_equal(fg, fr)
