@@ -627,17 +627,19 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
   return LBM_OK;
 }
 
-// Population strides the autotuner times for a D3Q19 or D2Q9 solver, N first. The stride decides
-// which memory channels and Infinity Cache sets the streams of one node share: measured on the D3Q19 step
+// Population strides the autotuner times for a D3Q19 solver, N first. The stride decides which memory
+// channels and Infinity Cache sets the streams of one node share: measured on the D3Q19 step
 // (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other grids), N rounded up to 1024
 // elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us and 256 x 128 x 128 FP16
 // 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and 256 x 256 x 128 (FP16
 // 128.9 -> 124.2 us) where 69,632 lost 6%, and neither helped at 192 x 96 x 96 (FP32 45.3 us at N,
 // 45.6 and 50.5 padded). So the choice is timed per grid. None while an adjoint state exists (its
-// checkpoints hold buffers at the current stride) and none for small grids (N < 65,536).
+// checkpoints hold buffers at the current stride) and none for small grids (N < 65,536). D2Q9 takes a
+// padded stride by hand (lbm_set_population_stride) but is not timed here: on the D2Q9 grids measured
+// (2048 x 1024, 3072 x 1024, 4096 x 2048, the 64-tunnel sweep) no pad was faster than N.
 std::vector<uint32_t> stride_candidates(const lbm_solver* s) {
   std::vector<uint32_t> c{s->N};
-  if (s->Q == 27 || s->adj || s->N < 65536u) return c;
+  if (s->nd != 3 || s->Q != 19 || s->adj || s->N < 65536u) return c;
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
     if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back(uint32_t(base + pad));
@@ -1468,6 +1470,8 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   }
   // The population stride (D3Q19): the three fastest shapes again at every candidate stride, the
   // strides visited twice in opposite orders; the fastest (stride, shape) pair of these runs wins.
+  // A stride change leaves the new buffers, just written by the copy, in the Infinity Cache: each
+  // visit first runs untimed steps until the step's own traffic has replaced that state.
   uint32_t best_ns = s->NS;
   const std::vector<uint32_t> strides = stride_candidates(s);
   if (strides.size() > 1) {
@@ -1480,6 +1484,9 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       for (size_t k = 0; k < strides.size(); k++) {
         const size_t si = rep == 0 ? k : strides.size() - 1 - k;
         if (int rc = set_stride_impl(s, strides[si])) return rc;
+        s->bulk_block = cand[top[0]].first;
+        s->sweep = cand[top[0]].second;
+        for (int k = 0; k < 2 * trial_steps; k++) enqueue_one(s);
         for (size_t t = 0; t < top.size(); t++)
           if (int rc = trial(top[t], &t2[si * top.size() + t])) return rc;
       }
