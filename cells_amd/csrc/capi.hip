@@ -627,19 +627,21 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
   return LBM_OK;
 }
 
-// Population strides the autotuner times for a D3Q19 solver, N first. The stride decides which memory
+// Population strides the autotuner times for a 3D solver, N first. The stride decides which memory
 // channels and Infinity Cache sets the streams of one node share: measured on the D3Q19 step
 // (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other grids), N rounded up to 1024
 // elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us and 256 x 128 x 128 FP16
 // 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and 256 x 256 x 128 (FP16
 // 128.9 -> 124.2 us) where 69,632 lost 6%, and neither helped at 192 x 96 x 96 (FP32 45.3 us at N,
-// 45.6 and 50.5 padded). So the choice is timed per grid. None while an adjoint state exists (its
+// 45.6 and 50.5 padded); on the D3Q27 step 256 x 128 x 64 FP32 109.5 -> 98.1 us and 256 x 128 x 128
+// FP16 117.3 -> 109.9 (+ 69,632), nothing at 192 x 96 x 96. So the choice is timed per grid. None
+// while an adjoint state exists (its
 // checkpoints hold buffers at the current stride) and none for small grids (N < 65,536). D2Q9 takes a
 // padded stride by hand (lbm_set_population_stride) but is not timed here: on the D2Q9 grids measured
 // (2048 x 1024, 3072 x 1024, 4096 x 2048, the 64-tunnel sweep) no pad was faster than N.
 std::vector<uint32_t> stride_candidates(const lbm_solver* s) {
   std::vector<uint32_t> c{s->N};
-  if (s->nd != 3 || s->Q != 19 || s->adj || s->N < 65536u) return c;
+  if (s->nd != 3 || s->adj || s->N < 65536u) return c;
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
     if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back(uint32_t(base + pad));
@@ -1553,7 +1555,6 @@ int lbm_set_population_stride(lbm_solver* s, int64_t stride) {
   if (stride == 0) stride = s->N;
   if (stride == int64_t(s->NS)) return LBM_OK;
   if (stride != int64_t(s->N)) {
-    if (s->Q == 27) return fail(LBM_ERR_UNSUPPORTED, "population stride: D3Q27 keeps the stride N");
     if (stride < int64_t(s->N) || stride % 64 != 0 || uint64_t(s->Q) * uint64_t(stride) >= (uint64_t(1) << 32))
       return fail(LBM_ERR_INVALID, "population stride: a multiple of 64 elements, at least N, with Q x stride below 2^32");
   }

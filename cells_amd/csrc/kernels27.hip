@@ -14,7 +14,8 @@
 // keep a uniform flow uniform. Force samples: only FLAG_SLOW27 nodes carry momentum-exchange force,
 // so only the waves holding one write an FP64 partial, and a one-wave kernel sums those partials in
 // wave order (a list the host keeps per geometry) into the history ring, the same order every run.
-// Storage is structure-of-arrays f[i * N + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16.
+// Storage is structure-of-arrays f[i * NS + cell] of (f_i - w_i) in FP32, FP16 x 2^15 or BF16 (NS the
+// population stride, Params3::NS).
 #include "kernels.hpp"
 #include "reduce.hpp"
 
@@ -138,7 +139,7 @@ template <class S>
 __device__ __forceinline__ float pull27(const typename S::elem* __restrict__ src, const uint32_t* __restrict__ flags,
                                         const float* __restrict__ sdf, const Params3& p, int i, int x, int y, int z,
                                         uint32_t idx, bool outlet, float (&force)[3]) {
-  const uint32_t N = p.N;
+  const uint32_t N = p.NS;  // population stride
   const int W = p.W, H = p.H, D = p.D;
   const int j = opp27(i);
   auto at = [&](int ax, int ay, int az) { return uint32_t(ax) + uint32_t(W) * (uint32_t(ay) + uint32_t(H) * uint32_t(az)); };
@@ -196,7 +197,7 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
                                        const float* __restrict__ sdf, const float4* __restrict__ xtab, const Params3& p,
                                        const FeqIn27& fin, int x, int y, int z, uint32_t idx, uint32_t fl,
                                        float (&force)[3], float (&out)[27]) {
-  const uint32_t N = p.N;
+  const uint32_t N = p.NS;  // population stride
   const bool slow = (fl & FLAG_SLOW27) != 0u;
   if (fl & FLAG_INLET27) {  // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0)
 #pragma unroll
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
       float out[27];
       slow = node27<S, SMAG>(src, flags, sdf, xtab, p, fin, x, y, z, idx, fl, force, out);
 #pragma unroll
-      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * p.N, idx, out[i]);
+      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * p.NS, idx, out[i]);
     }
   }
   if (!sample || __ballot(slow) == 0ull) return;  // uniform over the wave
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(256) void macro27_kernel(const typename S::elem* __
   float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 27; i++) {
-    const float v = S::load(f + size_t(i) * p.N, idx);
+    const float v = S::load(f + size_t(i) * p.NS, idx);
     drho += v;
     jx += float(CX27[i]) * v;
     jy += float(CY27[i]) * v;
@@ -358,8 +359,8 @@ __global__ __launch_bounds__(256) void init27_kernel(typename S::elem* __restric
 #pragma unroll
   for (int i = 0; i < 27; i++) {
     const float v = solid ? 0.0f : feq27(i, 1.0f, p.uIn, 0.0f, 0.0f);
-    S::store(fa + size_t(i) * p.N, idx, v);
-    S::store(fb + size_t(i) * p.N, idx, v);
+    S::store(fa + size_t(i) * p.NS, idx, v);
+    S::store(fb + size_t(i) * p.NS, idx, v);
   }
 }
 
@@ -434,8 +435,8 @@ __global__ __launch_bounds__(256) void flags27_kernel(const float* __restrict__ 
 #pragma unroll
     for (int i = 0; i < 27; i++) {
       const float v = feq27(i, rho, ux, uy, uz);
-      S::store(fa + size_t(i) * p.N, idx, v);
-      S::store(fb + size_t(i) * p.N, idx, v);
+      S::store(fa + size_t(i) * p.NS, idx, v);
+      S::store(fb + size_t(i) * p.NS, idx, v);
     }
   }
 }

@@ -230,7 +230,7 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
  * D3Q27: its one block width in the three row orders) on a copy of the solver's
  * state, with force samples as configured, keeps the fastest (cached for the process per lattice,
  * velocity set, precision, grid size, open-face kind, subgrid model on/off and force cadence), and restores
- * populations, step and force ring (also when it fails part way). For D3Q19 grids of at least 65,536
+ * populations, step and force ring (also when it fails part way). For 3D grids of at least 65,536
  * cells it then times the three fastest shapes at three population strides (N, and N rounded up to
  * 1024 plus 4096 or plus 69,632; see lbm_set_population_stride) and keeps the fastest pair; not
  * while an adjoint state exists. chosen (may be NULL) receives the width. Block width, row order
@@ -252,14 +252,13 @@ int lbm_launch_info(lbm_solver* s, int32_t* block, int32_t* sweep);
 int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
 /* No counterpart in the reference (its distribution buffers are N-strided, src/solver3d.ts:131-132): the
  * device buffers' population stride in elements, population i of cell idx at element i * stride +
- * idx. N unless lbm_autotune (D3Q19) or lbm_set_population_stride (D3Q19 and D2Q9, also tunnels of a
- * batch) chose a padded stride, which changes which memory channels and cache sets a node's streams
- * share. Every host-visible layout stays Q x N (lbm_read_populations copies the padded buffers row by
+ * idx. N unless lbm_autotune (3D) or lbm_set_population_stride (any lattice, also tunnels of a batch)
+ * chose a padded stride, which changes which memory channels and cache sets a node's streams share. Every host-visible layout stays Q x N (lbm_read_populations copies the padded buffers row by
  * row) and every result the same bits. Setting 0 or N restores the unpadded layout; a padded stride
  * is a multiple of 64 elements, at least N, with Q x stride below 2^32. The populations are moved to
  * the new stride (a device copy into two new buffers) and the boundary tables' indices re-strided; on
- * failure the old buffers, stride and tables stay. Refused for D3Q19 / D2Q9 while an adjoint state
- * exists (lbm_adjoint_release first) and for D3Q27 (stride N only). */
+ * failure the old buffers, stride and tables stay. Refused while an adjoint state exists
+ * (lbm_adjoint_release first). */
 int lbm_population_stride(lbm_solver* s, int64_t* stride);
 int lbm_set_population_stride(lbm_solver* s, int64_t stride);
 /* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
