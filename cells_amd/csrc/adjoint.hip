@@ -458,9 +458,9 @@ struct Src {
   float w, v;
 };
 
-// The sources of pull (x, i) of fluid node idx as population * NS + cell, NS the solver's population
-// stride; returns how many (0 when the reconstruction discards the pull). pos is the node's forward
-// list position or NO_OPERAND for bulk nodes.
+// The sources of pull (x, i) of fluid node idx as elements of the solver's population layout (the
+// forward tables' indices); returns how many (0 when the reconstruction discards the pull). pos is
+// the node's forward list position or NO_OPERAND for bulk nodes.
 int pull_sources_at_stride(const lbm_solver* s, uint32_t idx, uint32_t pos, int i, const double* dir, Src* out) {
   const uint32_t N = s->NS;
   const bool d2 = s->nd == 2;
@@ -470,7 +470,7 @@ int pull_sources_at_stride(const lbm_solver* s, uint32_t idx, uint32_t pos, int 
     if ((fl & FLAG_OUTLET2) && (OUTLET_DIRS2 & (1u << i))) return 0;
   }
   if (i == 0) {
-    out[0] = {idx, 1.0f, 0.0f};
+    out[0] = {pelem(s, 0, idx), 1.0f, 0.0f};
     return 1;
   }
   if (pos == NO_OPERAND) {
@@ -515,11 +515,17 @@ int pull_sources_at_stride(const lbm_solver* s, uint32_t idx, uint32_t pos, int 
 }
 
 // pull_sources with every source as population * N + cell: the forward tables carry the solver's
-// population stride NS, the adjoint tables index by N (and re-stride to their own state's stride).
+// population layout, the adjoint tables index by N (and re-stride to their own state's stride).
 int pull_sources(const lbm_solver* s, uint32_t idx, uint32_t pos, int i, const double* dir, Src* out) {
   const int n = pull_sources_at_stride(s, idx, pos, i, dir, out);
-  if (s->NS != s->N)
-    for (int e = 0; e < n; e++) out[e].src = (out[e].src / s->NS) * s->N + out[e].src % s->NS;
+  if (s->NA != 0 || s->NS != s->N) {
+    for (int e = 0; e < n; e++) {
+      int pi;
+      uint32_t c;
+      pelem_decode(s->NA, s->NS, s->Q, out[e].src, &pi, &c);
+      out[e].src = uint32_t(pi) * s->N + c;
+    }
+  }
   return n;
 }
 
@@ -845,7 +851,7 @@ int lbm_adjoint_forward(lbm_solver* s, const lbm_adjoint_config* cfg, double* J)
   if (!s->adj) s->adj = new lbm_adjoint();
   lbm_adjoint* a = s->adj;
   const int64_t T = cfg->steps, K = cfg->checkpoint_every, nseg = (T + K - 1) / K;
-  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;  // whole population buffers, at the solver's stride
+  const size_t fbytes = pop_elems(s) * s->elem;  // whole population buffers, in the solver's layout
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
   const size_t seg_bytes = size_t(K) * s->N * mom_bytes(s) + size_t(K) * nlk * 4;
   const int64_t nstored = std::min<int64_t>(nseg, std::max<int64_t>(0, cfg->record_budget_bytes) / int64_t(seg_bytes));
@@ -941,7 +947,7 @@ int lbm_adjoint_backward(lbm_solver* s, double seed, double* dJ_dsdf, size_t n) 
   if (a->tables_gen != s->tables_gen)
     if (int rc = build_tables(s, a, a->cfg.direction)) return rc;
   const uint32_t N = s->N;
-  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;
+  const size_t fbytes = pop_elems(s) * s->elem;
   const size_t nlk = std::max<uint32_t>(s->n_links, 1);
   for (float*& m : a->mu) {
     if (!m) HIP_TRY(hipMalloc(&m, size_t(s->Q) * a->Ns * 4));

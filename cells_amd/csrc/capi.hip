@@ -53,7 +53,7 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
   const int64_t delta = CZ3[i] * WH + CY3[i] * W + CX3[i];
   const bool outY = (r.loY && CY3[i] > 0) || (r.hiY && CY3[i] < 0);
   const bool outZ = (r.loZ && CZ3[i] > 0) || (r.hiZ && CZ3[i] < 0);
-  if (outlet && CX3[i] < 0) return uint32_t(i) * s->NS + idx - 1u;
+  if (outlet && CX3[i] < 0) return pelem(s, i, idx - 1u);
   int sd = i;
   int64_t sdelta = delta;
   if (outY && outZ) {
@@ -66,7 +66,7 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
     sd = mirror_z3(i);
     sdelta = CY3[i] * W + CX3[i];
   }
-  return uint32_t(int64_t(sd) * s->NS + int64_t(idx) - sdelta);
+  return pelem(s, sd, uint32_t(int64_t(idx) - sdelta));
 }
 
 }  // namespace lbm
@@ -130,6 +130,7 @@ void write_params3(lbm_solver* s) {
   p.D = s->D;
   p.N = s->N;
   p.NS = s->NS;
+  p.NA = s->NA;
   p.WH = uint32_t(s->W) * uint32_t(s->H);
   p.tau = float(c.tau);
   p.lambda = float(std::isnan(c.lambda) ? std::min(MAGIC_LAMBDA, 50.0 * (c.tau - 0.5) * (c.tau - 0.5)) : c.lambda);
@@ -180,7 +181,6 @@ float link_q(float dF, float dS) {
 
 int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H, D = s->D;
-  const uint32_t N = s->NS;  // population stride of the gather indices
   const uint32_t WH = uint32_t(W) * uint32_t(H);
   const int z = int(idx / WH), y = int((idx % WH) / uint32_t(W)), x = int(idx % uint32_t(W));
   const int j = opp3(i);
@@ -188,41 +188,41 @@ int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
   auto at = [&](int ax, int ay, int az) { return uint32_t(ax) + uint32_t(W) * (uint32_t(ay) + uint32_t(H) * uint32_t(az)); };
   const uint32_t f = fl[idx];
   if ((f & (1u << i)) == 0u) {
-    *g = uint32_t(i) * N + at(sx, sy, sz);
+    *g = pelem(s, i, at(sx, sy, sz));
     return PULL_GATHER;
   }
   const bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
   const bool outlet = (f & FLAG_OUTLET3) != 0u;
   if (outX && (f & FLAG_INLET3)) {
     // entering from x = -1: a Zou-He inlet face rebuilds it (the equilibrium inlet pulls nothing)
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
   } else if (outX && outlet) {
-    *g = uint32_t(i) * N + at(W - 2, y, z);
+    *g = pelem(s, i, at(W - 2, y, z));
   } else if (outY && outZ) {
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
   } else if (outY || outZ) {
     const uint32_t rs = outY ? at(sx, y, sz) : at(sx, sy, z);
     if (fl[rs] & FLAG_SOLID3) {
-      *g = uint32_t(j) * N + idx;
+      *g = pelem(s, j, idx);
       return PULL_MARKED;
     }
-    *g = uint32_t(outY ? mirror_y3(i) : mirror_z3(i)) * N + rs;
+    *g = pelem(s, outY ? mirror_y3(i) : mirror_z3(i), rs);
   } else if (outX) {
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
   } else {
     const uint32_t sc = at(sx, sy, sz);
     if (!(fl[sc] & FLAG_SOLID3)) {
-      *g = uint32_t(i) * N + sc;
+      *g = pelem(s, i, sc);
       return PULL_GATHER;
     }
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
     rec->q = link_q(s->sdf_host[idx], s->sdf_host[sc]);
     if (rec->q < 0.5f) {
       const int nx = x + CX3[i], ny = y + CY3[i], nz = z + CZ3[i];
       const bool ok = nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D && !(fl[at(nx, ny, nz)] & FLAG_SOLID3);
-      rec->gb = ok ? uint32_t(j) * N + at(nx, ny, nz) : NO_OPERAND;
+      rec->gb = ok ? pelem(s, j, at(nx, ny, nz)) : NO_OPERAND;
     } else {
-      rec->gb = uint32_t(i) * N + idx;
+      rec->gb = pelem(s, i, idx);
     }
     return PULL_BOUZIDI;
   }
@@ -381,7 +381,7 @@ int build_links(lbm_solver* s, std::vector<uint32_t>& fl) {
     }
   }
   // Every index must address a population of the grid before anything is launched with it.
-  const uint64_t lim = uint64_t(Q) * s->NS;
+  const uint64_t lim = pop_elems(s);
   for (uint32_t v : gt)
     if (v >= lim) return lbm::fail(LBM_ERR_DEVICE, "boundary list: gather index out of range");
   for (const BzLink& r : recs)
@@ -557,22 +557,49 @@ int tables_done(lbm_solver* s, int rc) {
 
 int rebuild_flags_impl(lbm_solver* s) { return tables_done(s, rebuild_core(s)); }
 
-// Moves both population buffers to population stride ns (a pitched device copy into new buffers,
-// their padding zeroed) and re-strides the boundary tables. Which nodes the bulk rows take does not
-// depend on the stride (a bulk node's pulls are the same (population, cell) pairs at any stride), so
-// only the indices that carry it change: every gather index and Bouzidi second operand is
-// population * NS + cell with cell < N, and becomes population * ns + cell. A failure leaves the
-// buffers, the stride and the tables as they were.
-int set_stride_impl(lbm_solver* s, uint32_t ns) {
-  if (ns == s->NS) return LBM_OK;
-  const uint32_t old = s->NS;
-  const size_t e = s->elem;
-  auto restride = [&](uint32_t g) { return (g / old) * ns + g % old; };
+// Copies the Q populations of every cell from layout (na0, ns0) to layout (na1, ns1), element words of
+// W bytes (the codecs' bits are moved, not converted). One thread per cell.
+template <class Word>
+__global__ __launch_bounds__(256) void relayout_kernel(const Word* __restrict__ src, Word* __restrict__ dst, uint32_t N,
+                                                       int Q, uint32_t na0, uint32_t ns0, uint32_t na1, uint32_t ns1) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  for (int i = 0; i < Q; i++) dst[pelem(na1, ns1, i, c)] = src[pelem(na0, ns0, i, c)];
+}
+
+int launch_relayout(const lbm_solver* s, const void* src, void* dst, uint32_t na0, uint32_t ns0, uint32_t na1,
+                    uint32_t ns1, hipStream_t st) {
+  const uint32_t grid = (s->N + 255u) / 256u;
+  if (s->elem == 2)
+    relayout_kernel<uint16_t><<<grid, 256, 0, st>>>(static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), s->N,
+                                                    s->Q, na0, ns0, na1, ns1);
+  else
+    relayout_kernel<uint32_t><<<grid, 256, 0, st>>>(static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), s->N,
+                                                    s->Q, na0, ns0, na1, ns1);
+  HIP_TRY(hipGetLastError());
+  return LBM_OK;
+}
+
+// Moves both population buffers to layout (na, ns) (new buffers, their unused elements zeroed) and
+// re-encodes the boundary tables' element indices. Which nodes the bulk rows take does not depend
+// on the layout (a bulk node's pulls are the same (population, cell) pairs in any layout), so only
+// the gather indices and Bouzidi second operands change: each is decoded to (population, cell) and
+// encoded again. A failure leaves the buffers, the layout and the tables as they were.
+int set_layout_impl(lbm_solver* s, uint32_t na, uint32_t ns) {
+  if (na == s->NA && ns == s->NS) return LBM_OK;
+  const uint32_t na0 = s->NA, ns0 = s->NS;
+  auto recode = [&](uint32_t g) {
+    int i;
+    uint32_t c;
+    pelem_decode(na0, ns0, s->Q, g, &i, &c);
+    return pelem(na, ns, i, c);
+  };
   std::vector<uint32_t> gt(s->h_gtab.size());
-  for (size_t k = 0; k < gt.size(); k++) gt[k] = restride(s->h_gtab[k]);
+  for (size_t k = 0; k < gt.size(); k++) gt[k] = recode(s->h_gtab[k]);
   std::vector<BzLink> lk = s->h_links;
   for (BzLink& r : lk)
-    if (r.gb != NO_OPERAND) r.gb = restride(r.gb);
+    if (r.gb != NO_OPERAND) r.gb = recode(r.gb);
+  const size_t bytes = (na ? size_t(s->Q) * ((size_t(s->N) + 63u) / 64u * 64u) : size_t(s->Q) * ns) * s->elem;
   void* nf[2] = {nullptr, nullptr};
   uint32_t* ngt = nullptr;
   BzLink* nlk = nullptr;
@@ -583,10 +610,9 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
   };
   const int rc = [&]() -> int {
     for (int b = 0; b < 2; b++) {
-      HIP_TRY(hipMalloc(&nf[b], size_t(s->Q) * ns * e));
-      HIP_TRY(hipMemsetAsync(nf[b], 0, size_t(s->Q) * ns * e, s->stream));
-      HIP_TRY(hipMemcpy2DAsync(nf[b], size_t(ns) * e, s->f[b], size_t(old) * e, size_t(s->N) * e, size_t(s->Q),
-                               hipMemcpyDeviceToDevice, s->stream));
+      HIP_TRY(hipMalloc(&nf[b], bytes));
+      HIP_TRY(hipMemsetAsync(nf[b], 0, bytes, s->stream));
+      if (int r = launch_relayout(s, s->f[b], nf[b], na0, ns0, na, ns, s->stream)) return r;
     }
     if (s->gtab) {
       HIP_TRY(hipMalloc(&ngt, sizeof(uint32_t) * std::max<size_t>(1, gt.size())));
@@ -622,29 +648,43 @@ int set_stride_impl(lbm_solver* s, uint32_t ns) {
   s->h_links = std::move(lk);
   s->tables_gen++;
   s->NS = ns;
-  if (s->nd == 2) s->p2.NS = ns;
-  else s->p3.NS = ns;
+  s->NA = na;
+  if (s->nd == 2) {
+    s->p2.NS = ns;
+  } else {
+    s->p3.NS = ns;
+    s->p3.NA = na;
+  }
   return LBM_OK;
 }
 
-// Population strides the autotuner times for a 3D solver, N first. The stride decides which memory
-// channels and Infinity Cache sets the streams of one node share: measured on the D3Q19 step
-// (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other grids), N rounded up to 1024
-// elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us and 256 x 128 x 128 FP16
-// 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and 256 x 256 x 128 (FP16
-// 128.9 -> 124.2 us) where 69,632 lost 6%, and neither helped at 192 x 96 x 96 (FP32 45.3 us at N,
-// 45.6 and 50.5 padded); on the D3Q27 step 256 x 128 x 64 FP32 109.5 -> 98.1 us and 256 x 128 x 128
-// FP16 117.3 -> 109.9 (+ 69,632), nothing at 192 x 96 x 96. So the choice is timed per grid. None
-// while an adjoint state exists (its
-// checkpoints hold buffers at the current stride) and none for small grids (N < 65,536). D2Q9 takes a
-// padded stride by hand (lbm_set_population_stride) but is not timed here: on the D2Q9 grids measured
-// (2048 x 1024, 3072 x 1024, 4096 x 2048, the 64-tunnel sweep) no pad was faster than N.
-std::vector<uint32_t> stride_candidates(const lbm_solver* s) {
-  std::vector<uint32_t> c{s->N};
+// Whether a D3Q19 solver can take tiles of 64 cells: rows of a multiple of 64 cells (so a wave of
+// the bulk rows is one tile) and byte offsets within a buffer that fit 32 bits.
+bool tiles_allowed(const lbm_solver* s) {
+  return s->nd == 3 && s->Q == 19 && s->W % 64 == 0 && uint64_t(s->Q) * s->N * s->elem < (uint64_t(1) << 32);
+}
+
+// Population layouts the autotuner times for a 3D solver, (NA, NS) with the unpadded one first. The
+// layout decides which memory channels and Infinity Cache sets the streams of one node share:
+// measured on the D3Q19 step (MI355X, the 192 x 96 x 96 and 256 x 128 x 128 tunnels and six other
+// grids), N rounded up to 1024 elements plus 69,632 (17 x 4096) gave 256 x 128 x 64 FP32 67.0 -> 59.0 us
+// and 256 x 128 x 128 FP16 72.7 -> 64.8 us, plus 4096 was the better pad at 512 x 128 x 128 and
+// 256 x 256 x 128 (FP16 128.9 -> 124.2 us) where 69,632 lost 6%, and neither helped at 192 x 96 x 96
+// (FP32 45.3 us at N, 45.6 and 50.5 padded); on the D3Q27 step 256 x 128 x 64 FP32 109.5 -> 98.1 us
+// and 256 x 128 x 128 FP16 117.3 -> 109.9 (+ 69,632), nothing at 192 x 96 x 96. Tiles of 64 cells
+// (D3Q19) keep a tile's 19 populations in one 64 x 19-element block, which a copy of the step's
+// shifted streams moved 4-14% faster than one array per population. So the choice is timed per
+// grid. None while an adjoint state exists (its checkpoints hold buffers in the current layout) and
+// none for small grids (N < 65,536). D2Q9 takes a padded stride by hand (lbm_set_population_stride)
+// but is not timed here: on the D2Q9 grids measured (2048 x 1024, 3072 x 1024, 4096 x 2048, the
+// 64-tunnel sweep) no pad was faster than N.
+std::vector<std::pair<uint32_t, uint32_t>> layout_candidates(const lbm_solver* s) {
+  std::vector<std::pair<uint32_t, uint32_t>> c{{0u, s->N}};
   if (s->nd != 3 || s->adj || s->N < 65536u) return c;
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
-    if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back(uint32_t(base + pad));
+    if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back({0u, uint32_t(base + pad)});
+  if (tiles_allowed(s)) c.push_back({uint32_t(s->Q - 1), 64u});
   return c;
 }
 
@@ -656,7 +696,7 @@ int alloc_common(lbm_solver* s) {
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->owns_stream = true;
   }
-  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;
+  const size_t fbytes = pop_elems(s) * s->elem;
   HIP_TRY(hipMalloc(&s->f[0], fbytes));
   HIP_TRY(hipMalloc(&s->f[1], fbytes));
   HIP_TRY(hipMalloc(&s->flags, 4 * N));
@@ -1247,8 +1287,18 @@ int lbm_info(const lbm_solver* s, int32_t* dims, int64_t* n_cells, int64_t* step
 int lbm_read_populations(lbm_solver* s, void* out, size_t bytes) {
   const size_t need = size_t(s ? s->Q : 0) * (s ? s->N : 0) * (s ? s->elem : 0);
   if (!s || !out || bytes < need) return fail(LBM_ERR_INVALID, "out too small");
-  // Q x N, population-major, whatever the device stride: a padded stride is copied row by row
-  if (s->NS == s->N) {
+  // Q x N, population-major, whatever the device layout: a padded stride is copied row by row, tiles
+  // are copied whole and taken apart on the host
+  if (s->NA) {
+    std::vector<uint8_t> raw(pop_elems(s) * s->elem);
+    HIP_TRY(hipMemcpyAsync(raw.data(), s->f[s->parity], raw.size(), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    uint8_t* o = static_cast<uint8_t*>(out);
+    for (int i = 0; i < s->Q; i++)
+      for (uint32_t c = 0; c < s->N; c++)
+        std::memcpy(o + (size_t(i) * s->N + c) * s->elem, raw.data() + size_t(pelem(s, i, c)) * s->elem, s->elem);
+    return LBM_OK;
+  } else if (s->NS == s->N) {
     HIP_TRY(hipMemcpyAsync(out, s->f[s->parity], need, hipMemcpyDeviceToHost, s->stream));
   } else {
     HIP_TRY(hipMemcpy2DAsync(out, size_t(s->N) * s->elem, s->f[s->parity], size_t(s->NS) * s->elem,
@@ -1330,7 +1380,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   static std::mutex cache_mu;
   struct Pick {
     int block, sweep;
-    uint32_t stride;
+    std::pair<uint32_t, uint32_t> layout;  // (NA, NS)
   };
   static std::map<std::string, Pick> cache;
   // Everything that selects a different kernel instantiation or changes the work of a step is part
@@ -1346,9 +1396,9 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     auto it = cache.find(key);
     if (it != cache.end()) {
       const Pick pk = it->second;
-      const std::vector<uint32_t> strides = stride_candidates(s);
-      if (std::find(strides.begin(), strides.end(), pk.stride) != strides.end())
-        if (int rc = set_stride_impl(s, pk.stride)) return rc;
+      const auto layouts = layout_candidates(s);
+      if (std::find(layouts.begin(), layouts.end(), pk.layout) != layouts.end())
+        if (int rc = set_layout_impl(s, pk.layout.first, pk.layout.second)) return rc;
       s->bulk_block = pk.block;
       s->sweep = pk.sweep;
       if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : s->bulk_block;
@@ -1357,10 +1407,10 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   }
   // Trials run on the solver's own buffers and sample forces as it is configured to (the sampling
   // changes which launch shape wins). The populations, the force ring and its counter are saved
-  // first; the guard restores them (into the buffers at whatever population stride is current), the
+  // first; the guard restores them (into the buffers in whatever population layout is current), the
   // step, parity and pending sample steps, and frees the copies, whether the trials finish or a HIP
-  // call fails part way; after a failure it also moves the populations back to the stride they had.
-  const size_t fbytes = size_t(s->Q) * s->NS * s->elem;
+  // call fails part way; after a failure it also moves the populations back to the layout they had.
+  const size_t fbytes = pop_elems(s) * s->elem;
   const size_t ring_bytes = sizeof(double) * 3 * HISTORY_LEN;
   struct Guard {
     lbm_solver* s;
@@ -1371,17 +1421,20 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     hipEvent_t a = nullptr, b = nullptr;
     int64_t step0;
     int parity0, block0, sweep0;
-    uint32_t ns0;
+    uint32_t na0, ns0;
     int64_t pushed0;
     std::vector<int64_t> steps0;
     bool saved = false;  // every copy was enqueued: from here on the state must be restored
     bool keep_shape = false;
     int restore() {
       saved = false;
-      const size_t e = s->elem;
-      for (int b = 0; b < 2; b++)
-        HIP_TRY(hipMemcpy2DAsync(s->f[b], size_t(s->NS) * e, save[b], size_t(ns0) * e, size_t(s->N) * e,
-                                 size_t(s->Q), hipMemcpyDeviceToDevice, s->stream));
+      for (int b = 0; b < 2; b++) {
+        if (s->NA == na0 && s->NS == ns0) {
+          HIP_TRY(hipMemcpyAsync(s->f[b], save[b], fbytes, hipMemcpyDeviceToDevice, s->stream));
+        } else if (int rc = launch_relayout(s, save[b], s->f[b], na0, ns0, s->NA, s->NS, s->stream)) {
+          return rc;
+        }
+      }
       HIP_TRY(hipMemcpyAsync(s->ring, ring, ring_bytes, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipMemcpyAsync(s->state, state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, s->stream));
       HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1389,7 +1442,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     }
     ~Guard() {
       if (saved) (void)restore();
-      if (!keep_shape && s->NS != ns0) (void)set_stride_impl(s, ns0);
+      if (!keep_shape && (s->NA != na0 || s->NS != ns0)) (void)set_layout_impl(s, na0, ns0);
       (void)hipStreamSynchronize(s->stream);
       for (void* p : {save[0], save[1], ring, state})
         if (p) (void)hipFree(p);
@@ -1409,6 +1462,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
   g.parity0 = s->parity;
   g.block0 = s->bulk_block;
   g.sweep0 = s->sweep;
+  g.na0 = s->NA;
   g.ns0 = s->NS;
   g.pushed0 = s->samples_pushed;
   g.steps0 = s->sample_steps;
@@ -1470,12 +1524,12 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       best_sweep = cand[c].second;
     }
   }
-  // The population stride (D3Q19): the three fastest shapes again at every candidate stride, the
-  // strides visited twice in opposite orders; the fastest (stride, shape) pair of these runs wins.
-  // A stride change leaves the new buffers, just written by the copy, in the Infinity Cache: each
+  // The population layout (3D): the three fastest shapes again in every candidate layout, the
+  // layouts visited twice in opposite orders; the fastest (layout, shape) pair of these runs wins.
+  // A layout change leaves the new buffers, just written by the copy, in the Infinity Cache: each
   // visit first runs untimed steps until the step's own traffic has replaced that state.
-  uint32_t best_ns = s->NS;
-  const std::vector<uint32_t> strides = stride_candidates(s);
+  std::pair<uint32_t, uint32_t> best_layout{s->NA, s->NS};
+  const auto strides = layout_candidates(s);
   if (strides.size() > 1) {
     std::vector<size_t> top(cand.size());
     for (size_t c = 0; c < cand.size(); c++) top[c] = c;
@@ -1485,7 +1539,7 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
     for (int rep = 0; rep < 2; rep++) {
       for (size_t k = 0; k < strides.size(); k++) {
         const size_t si = rep == 0 ? k : strides.size() - 1 - k;
-        if (int rc = set_stride_impl(s, strides[si])) return rc;
+        if (int rc = set_layout_impl(s, strides[si].first, strides[si].second)) return rc;
         s->bulk_block = cand[top[0]].first;
         s->sweep = cand[top[0]].second;
         for (int k = 0; k < 2 * trial_steps; k++) enqueue_one(s);
@@ -1498,21 +1552,21 @@ int lbm_autotune(lbm_solver* s, int32_t trial_steps, int32_t* chosen) {
       for (size_t t = 0; t < top.size(); t++) {
         if (t2[si * top.size() + t] < b2) {
           b2 = t2[si * top.size() + t];
-          best_ns = strides[si];
+          best_layout = strides[si];
           best = cand[top[t]].first;
           best_sweep = cand[top[t]].second;
         }
       }
     }
   }
-  if (int rc = set_stride_impl(s, best_ns)) return rc;
+  if (int rc = set_layout_impl(s, best_layout.first, best_layout.second)) return rc;
   if (int rc = g.restore()) return rc;
   s->bulk_block = best;
   s->sweep = best_sweep;
   g.keep_shape = true;
   {
     std::lock_guard<std::mutex> lk(cache_mu);
-    cache[key] = {best, best_sweep, best_ns};
+    cache[key] = {best, best_sweep, best_layout};
   }
   if (chosen) *chosen = s->Q == 27 ? int32_t(step27_block_x(s->W)) : best;
   return LBM_OK;
@@ -1549,19 +1603,38 @@ int lbm_population_stride(lbm_solver* s, int64_t* stride) {
   return LBM_OK;
 }
 
-int lbm_set_population_stride(lbm_solver* s, int64_t stride) {
+int lbm_population_layout(lbm_solver* s, int32_t* tiled, int64_t* stride) {
+  if (!s) return fail(LBM_ERR_INVALID, "null solver");
+  if (tiled) *tiled = s->NA ? 1 : 0;
+  if (stride) *stride = s->NS;
+  return LBM_OK;
+}
+
+int lbm_set_population_layout(lbm_solver* s, int32_t tiled, int64_t stride) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
   if (int rc = check_usable(s)) return rc;
-  if (stride == 0) stride = s->N;
-  if (stride == int64_t(s->NS)) return LBM_OK;
-  if (stride != int64_t(s->N)) {
-    if (stride < int64_t(s->N) || stride % 64 != 0 || uint64_t(s->Q) * uint64_t(stride) >= (uint64_t(1) << 32))
+  uint32_t na = 0, ns = 0;
+  if (tiled) {
+    if (!tiles_allowed(s))
+      return fail(LBM_ERR_UNSUPPORTED, "population layout: tiles of 64 cells need D3Q19, a row width that is a multiple "
+                                       "of 64 and 19 x N x element bytes below 2^32");
+    if (stride != 0 && stride != 64) return fail(LBM_ERR_INVALID, "population layout: tiles take stride 0 or 64");
+    na = uint32_t(s->Q - 1);
+    ns = 64u;
+  } else {
+    if (stride == 0) stride = s->N;
+    if (stride != int64_t(s->N) &&
+        (stride < int64_t(s->N) || stride % 64 != 0 || uint64_t(s->Q) * uint64_t(stride) >= (uint64_t(1) << 32)))
       return fail(LBM_ERR_INVALID, "population stride: a multiple of 64 elements, at least N, with Q x stride below 2^32");
+    ns = uint32_t(stride);
   }
+  if (na == s->NA && ns == s->NS) return LBM_OK;
   if (s->adj)
-    return fail(LBM_ERR_UNSUPPORTED, "population stride: release the adjoint state first (its checkpoints hold the buffers)");
-  return set_stride_impl(s, uint32_t(stride));
+    return fail(LBM_ERR_UNSUPPORTED, "population layout: release the adjoint state first (its checkpoints hold the buffers)");
+  return set_layout_impl(s, na, ns);
 }
+
+int lbm_set_population_stride(lbm_solver* s, int64_t stride) { return lbm_set_population_layout(s, 0, stride); }
 
 int lbm_node_counts(lbm_solver* s, int64_t* bulk, int64_t* list, int64_t* solid) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");

@@ -373,34 +373,47 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   }
   const uint32_t idx = x + uint32_t(p.W) * (y + uint32_t(p.H) * z);
   const size_t N = p.NS;  // population stride
-  const uint32_t off = idx * uint32_t(sizeof(E));
+  constexpr uint32_t EB = uint32_t(sizeof(E));
+  // Element of population i at cell c: c + NA (c & ~63) + NS i (NA = 0: one array per population;
+  // NA = 18, NS = 64: tiles of 64 cells, whose rows start tiles because W % 64 == 0 there, so a
+  // wave is one tile). off is this node's byte offset within a population; a pull from c - 1 or
+  // c + 1 (c_x = +1 / -1) reads at offp / offm, which on a tiled layout cross into the previous /
+  // next tile at the wave's first / last lane. At x = 0 and x = W - 1 those pulls read the node's
+  // own element instead: the inlet node never uses them, and the x = W - 1 node either takes the
+  // outlet copy rule or is a boundary-list node.
+  const uint32_t off = (idx + p.NA * (idx & ~63u)) * EB;
+  const uint32_t tile_step = 64u * p.NA * EB;
+  const uint32_t offp = x == 0u ? off : off - EB - ((idx & 63u) == 0u ? tile_step : 0u);
+  const uint32_t offm = x == uint32_t(p.W - 1) ? off : off + EB + ((idx & 63u) == 63u ? tile_step : 0u);
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[19];
   // The 19 pulls; at_outlet selects the outlet copy rule for the directions entering from x = W.
+  // Per direction the source is population sd at cell idx - dyz - dx: a wave-uniform base (the
+  // population and the y / z shift dyz, a multiple of 64 cells on a tiled layout) plus off, offp or
+  // offm for dx = 0, +1, -1.
   auto pull = [&](bool at_outlet) {
 #pragma unroll
     for (int i = 0; i < 19; i++) {
-      const ptrdiff_t delta = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W + CX3[i];
       const bool outY = (loY && CY3[i] > 0) || (hiY && CY3[i] < 0);
       const bool outZ = (loZ && CZ3[i] > 0) || (hiZ && CZ3[i] < 0);
       int sd = i;
-      ptrdiff_t sdelta = delta;
+      ptrdiff_t dyz = ptrdiff_t(CZ3[i]) * p.WH + ptrdiff_t(CY3[i]) * p.W;
       if (outY && outZ) {
-        sd = opp3(i);
-        sdelta = 0;
+        sd = opp3(i);  // bounce-back: the node's own cell
+        dyz = 0;
       } else if (outY) {
         sd = mirror_y3(i);
-        sdelta = ptrdiff_t(CZ3[i]) * p.WH + CX3[i];
+        dyz = ptrdiff_t(CZ3[i]) * p.WH;
       } else if (outZ) {
         sd = mirror_z3(i);
-        sdelta = ptrdiff_t(CY3[i]) * p.W + CX3[i];
+        dyz = ptrdiff_t(CY3[i]) * p.W;
       }
-      const char* base = sb + (ptrdiff_t(sd) * ptrdiff_t(N) - sdelta) * ptrdiff_t(sizeof(E));
-      uint32_t o = off;
+      const char* base = sb + (ptrdiff_t(sd) * ptrdiff_t(N) - ptrdiff_t(1u + p.NA) * dyz) * ptrdiff_t(EB);
+      uint32_t o = CX3[i] == 0 || (outY && outZ) ? off : CX3[i] > 0 ? offp : offm;
       // Outlet copy rule (step3d.ts:130): the population entering from x = W comes from (W-2, y, z),
-      // i.e. element idx - 1 of population i: base + (idx + delta - 1) elements (rows off the faces
-      // only, where base is the plain one).
-      if (CX3[i] < 0 && at_outlet) o = off + uint32_t(delta - 1) * uint32_t(sizeof(E));
+      // i.e. population i at cell idx - 1, in the same tile (rows off the faces only, where base is
+      // the plain one).
+      if (CX3[i] < 0 && at_outlet) o = off - EB + uint32_t(ptrdiff_t(1u + p.NA) * dyz) * EB;
       f[i] = S::load_at(base, o);
     }
   };
@@ -408,11 +421,9 @@ __global__ __launch_bounds__(1024) void step3_kernel(const typename S::elem* __r
   uint8_t clean = 0;
   if constexpr (EARLY) {
     // Only x = W-1 off the faces can hold an outlet node (face-row outlet nodes are on the boundary
-    // list). Every address lies inside the population buffer: a pull that leaves the grid through
-    // x = 0 or W-1 reads the neighbouring row's end of the same population, at worst element -1 or
-    // N of it, the previous population's end (or pad) / this population's pad or the next one's first
-    // element; the directions with c_x != 0 are numbered 1-16, so that stays within elements
-    // 0 .. 19 NS - 1.
+    // list). Every address lies inside the population buffer: a pull that would leave the grid
+    // through x = 0 or W-1 reads the node's own element (offp / offm above), and the y / z shifts
+    // of a row stay on the grid (the slip faces' patterns).
     clean = *clean_byte;
     pull(!face && x == uint32_t(p.W - 1));
   } else {
@@ -543,16 +554,17 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   const uint32_t N = p.NS;  // population stride
   const uint32_t nl = lt.n_link + lt.n_body;
   const uint32_t idx = lt.list[k];
+  const uint32_t ce = idx + p.NA * (idx & ~63u);  // the node's element within each population
   const uint32_t fl = flags[idx];
   const uint32_t nw = S::stochastic ? lowbias32(idx ^ p.seed) : 0u;
   constexpr bool zh = ZH;
   if ((fl & FLAG_INLET3) && !zh) {
 #pragma unroll
-    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, fin.v[i], nw, i);
+    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, ce, fin.v[i], nw, i);
     return;
   }
   float f[19];
-  f[0] = S::load(src, idx);
+  f[0] = S::load(src, ce);
 #pragma unroll
   for (int i = 1; i < 19; i++) f[i] = S::load(src, lt.gtab[size_t(i - 1) * nl + k]);
   if (BODY) {
@@ -577,7 +589,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
     float eq[19];
     outlet_feq3(m, eq);
 #pragma unroll
-    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, eq[i], nw, i);
+    for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, ce, eq[i], nw, i);
     return;
   }
   const float4 t = xtab[idx % uint32_t(p.W)];
@@ -589,7 +601,7 @@ __device__ __forceinline__ void list_node3(const typename S::elem* __restrict__ 
   float out[19];
   collide3<SMAG>(f, m, r, p, out);
 #pragma unroll
-  for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, idx, out[i], nw, i);
+  for (int i = 0; i < 19; i++) put<S>(dst + size_t(i) * N, ce, out[i], nw, i);
 }
 
 // Boundary-list part of the step: list block lb of the launch. List blocks [0, nlb) take the nodes
@@ -627,7 +639,7 @@ __global__ __launch_bounds__(256) void macro3_kernel(const typename S::elem* __r
   float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 19; i++) {
-    const float v = S::load(f + size_t(i) * p.NS, idx);
+    const float v = S::load(f + size_t(i) * p.NS, idx + p.NA * (idx & ~63u));
     drho += v;
     jx += float(CX3[i]) * v;
     jy += float(CY3[i]) * v;
@@ -647,8 +659,8 @@ __global__ __launch_bounds__(256) void init3_kernel(typename S::elem* __restrict
 #pragma unroll
   for (int i = 0; i < 19; i++) {
     const float v = solid ? 0.0f : feq3(i, 1.0f, p.uIn, 0.0f, 0.0f);
-    S::store(fa + size_t(i) * p.NS, idx, v);
-    S::store(fb + size_t(i) * p.NS, idx, v);
+    S::store(fa + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
+    S::store(fb + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
   }
 }
 
@@ -716,8 +728,8 @@ __global__ __launch_bounds__(256) void flags3_kernel(const float* __restrict__ s
 #pragma unroll
     for (int i = 0; i < 19; i++) {
       const float v = feq3(i, rho, ux, uy, uz);
-      S::store(fa + size_t(i) * p.NS, idx, v);
-      S::store(fb + size_t(i) * p.NS, idx, v);
+      S::store(fa + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
+      S::store(fb + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
     }
   }
 }

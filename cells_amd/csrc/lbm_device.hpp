@@ -190,10 +190,12 @@ struct Params3 {
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, absorb;
   uint32_t seed;   // lowbias32(step index): the stochastic codec's per-step seed
   uint32_t faces;  // open x planes: 0 equilibrium (the reference's), 1 Zou-He faces (Hecht-Harting)
-  // Elements from one population's first cell to the next one's: N, or N padded (D3Q19 and D2Q9; the
-  // autotuners time a few strides; population i of cell idx is element i * NS + idx, the elements
-  // NS - N after each population hold no population). N stays the cell count.
+  // Population layout: population i of cell c is element c + NA (c & ~63) + NS i. NA = 0: one array
+  // per population, NS elements apart (N, or N padded: the elements NS - N after each population
+  // hold no population); NA = 18, NS = 64 (D3Q19 with W % 64 == 0): tiles of 64 cells, each holding
+  // its cells' 19 populations one after another. N stays the cell count.
   uint32_t NS;
+  uint32_t NA;
 };
 
 // Where the step kernels put a force sample: per-block FP64 partials, a completion counter,

@@ -58,9 +58,12 @@ struct lbm_solver {
   bool owns_stream = true;
   int W = 1, H = 1, D = 1;
   uint32_t N = 0;
-  // Population stride in elements (population i of cell idx is element i * NS + idx of a buffer):
-  // N, except for a D3Q19 solver whose autotuner or caller chose a padded stride (lbm_set_population_stride).
+  // Population layout of the buffers (Params3::NS / NA): population i of cell c is element
+  // c + NA (c & ~63) + NS i. NA = 0, NS = N unless the autotuner or the caller chose a padded stride
+  // (lbm_set_population_stride) or, for D3Q19 rows of a multiple of 64 cells, tiles of 64 cells
+  // (NA = 18, NS = 64; lbm_set_population_layout).
   uint32_t NS = 0;
+  uint32_t NA = 0;
   int Q = 9;
   int prec = LBM_FP32;
   size_t elem = 4;
@@ -164,4 +167,23 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
 
 // Frees the adjoint state of a solver (adjoint.hip).
 void adjoint_free(lbm_solver* s);
+
+// Element of population i at cell c in the solver's buffers, and the inverse; elements per buffer.
+__host__ __device__ inline uint32_t pelem(uint32_t na, uint32_t ns, int i, uint32_t c) {
+  return c + na * (c & ~63u) + ns * uint32_t(i);
+}
+inline uint32_t pelem(const lbm_solver* s, int i, uint32_t c) { return pelem(s->NA, s->NS, i, c); }
+inline void pelem_decode(uint32_t na, uint32_t ns, int Q, uint32_t e, int* i, uint32_t* c) {
+  if (na == 0) {
+    *i = int(e / ns);
+    *c = e % ns;
+  } else {
+    const uint32_t t = e / (64u * uint32_t(Q)), r = e % (64u * uint32_t(Q));
+    *i = int(r / 64u);
+    *c = t * 64u + r % 64u;
+  }
+}
+inline size_t pop_elems(const lbm_solver* s) {
+  return s->NA ? size_t(s->Q) * ((size_t(s->N) + 63u) / 64u * 64u) : size_t(s->Q) * s->NS;
+}
 }  // namespace lbm

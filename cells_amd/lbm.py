@@ -178,6 +178,8 @@ def load_library(path: str = LIB_PATH):
         "lbm_set_launch": ([P, C.c_int32, C.c_int32], C.c_int),
         "lbm_population_stride": ([P, C.POINTER(C.c_int64)], C.c_int),
         "lbm_set_population_stride": ([P, C.c_int64], C.c_int),
+        "lbm_population_layout": ([P, C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
+        "lbm_set_population_layout": ([P, C.c_int32, C.c_int64], C.c_int),
         "lbm_copy_roofline": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double)],
                               C.c_int),
         "lbm_copy_roofline_filled": ([C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -380,8 +382,17 @@ class _Base:
         return int(v.value)
 
     def set_population_stride(self, stride: int = 0):
-        """Moves the populations to another stride (0 or N: unpadded; D3Q19: a multiple of 64 >= N)."""
+        """Moves the populations to another stride (0 or N: unpadded; else a multiple of 64 >= N)."""
         _check(_lib.lbm_set_population_stride(self._h, int(stride)))
+
+    def population_layout(self):
+        """(tiled, stride): tiled 1 for tiles of 64 cells (D3Q19), else one array per population."""
+        t, v = C.c_int32(), C.c_int64()
+        _check(_lib.lbm_population_layout(self._h, C.byref(t), C.byref(v)))
+        return int(t.value), int(v.value)
+
+    def set_population_layout(self, tiled: bool, stride: int = 0):
+        _check(_lib.lbm_set_population_layout(self._h, int(bool(tiled)), int(stride)))
 
     def node_counts(self):
         """(bulk, list, solid) node counts."""

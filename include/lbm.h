@@ -231,9 +231,10 @@ int lbm_profile_steps(lbm_solver* s, int64_t n, double* total_ms, double* step_m
  * state, with force samples as configured, keeps the fastest (cached for the process per lattice,
  * velocity set, precision, grid size, open-face kind, subgrid model on/off and force cadence), and restores
  * populations, step and force ring (also when it fails part way). For 3D grids of at least 65,536
- * cells it then times the three fastest shapes at three population strides (N, and N rounded up to
- * 1024 plus 4096 or plus 69,632; see lbm_set_population_stride) and keeps the fastest pair; not
- * while an adjoint state exists. chosen (may be NULL) receives the width. Block width, row order
+ * cells it then times the three fastest shapes in each population layout (strides N, and N rounded
+ * up to 1024 plus 4096 or plus 69,632; for D3Q19 rows of a multiple of 64 cells also tiles of 64
+ * cells; see lbm_set_population_stride and lbm_set_population_layout) and keeps the fastest pair;
+ * not while an adjoint state exists. chosen (may be NULL) receives the width. Block width, row order
  * and population stride never change a result; the store variants, and the two pull orders of the
  * shared-store variant, agree to floating-point rounding (their compiled code may fuse different
  * multiply-adds). */
@@ -261,6 +262,17 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
  * (lbm_adjoint_release first). */
 int lbm_population_stride(lbm_solver* s, int64_t* stride);
 int lbm_set_population_stride(lbm_solver* s, int64_t stride);
+/* No counterpart in the reference: the device population layout. tiled = 0: one array per
+ * population, stride elements apart (lbm_population_stride). tiled = 1 (D3Q19 with a row width that
+ * is a multiple of 64 and 19 x N x element bytes below 2^32): tiles of 64 cells, each holding its
+ * cells' 19 populations one after another, population i of cell c at element
+ * c + 18 (c & ~63) + 64 i (stride reads 64); a tile is one wave of the step's bulk rows, and the 19
+ * streams of a wave fall in one block of memory instead of 19 arrays. lbm_autotune times it next to
+ * the strides; set by hand here (stride 0 or 64 with tiled = 1, as lbm_set_population_stride
+ * otherwise). Same guarantees as the stride: host-visible layouts and results unchanged, refused
+ * while an adjoint state exists. */
+int lbm_population_layout(lbm_solver* s, int32_t* tiled, int64_t* stride);
+int lbm_set_population_layout(lbm_solver* s, int32_t tiled, int64_t stride);
 /* Copy roofline (src/bench.ts:61-95): iters launches (after 50 untimed ones) of a kernel that reads q
  * structure-of-arrays populations of elem_bytes each plus a u32 flag word per cell and writes the q
  * populations, with no arithmetic, one thread per cell in blocks of workgroup (64, 128 or 256; 0 for

@@ -59,6 +59,8 @@ def test_host_only_calls():
     assert lib.lbm_set_launch(None, 64, 0) != 0
     assert lib.lbm_population_stride(None, None) != 0
     assert lib.lbm_set_population_stride(None, 0) != 0
+    assert lib.lbm_population_layout(None, None, None) != 0
+    assert lib.lbm_set_population_layout(None, 1, 0) != 0
     assert lib.lbm_rebuild_flags(None) != 0
     assert lib.lbm_set_open_faces(None, 1) != 0
     assert lib.lbm_brush(None, 1.0, 1.0, 1.0, 0, 1) != 0
