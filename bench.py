@@ -64,7 +64,7 @@ def tunnel_sdf(W, H, D):
 
 # Launch shapes pinned from the command line (--launch / --batch-launch) instead of autotuned, so that
 # counter passes, which slow every dispatch and can change the autotuner's choice, measure one shape.
-PIN = {"launch": None, "batch": None, "stride": None}
+PIN = {"launch": None, "batch": None, "stride": None, "tiled": False}
 
 
 def tune_or_pin(s, trial_steps):
@@ -75,6 +75,8 @@ def tune_or_pin(s, trial_steps):
         block = s.autotune(trial_steps)
     if PIN["stride"] is not None and s.Q == 19:
         s.set_population_stride(PIN["stride"])
+    if PIN["tiled"] and s.Q == 19:
+        s.set_population_layout(True)
     return block
 
 
@@ -100,7 +102,8 @@ def run_gpu(W, H, D, precision, steps, warmup, profile=True, velocity_set=19):
     nb, nl, ns = s.node_counts()
     out["nodes"] = dict(bulk=nb, list=nl, solid=ns)
     out["bulk_block"], out["row_order"] = s.launch_info()
-    out["population_stride"] = s.population_stride()  # N, or the padded stride the autotuner chose
+    # N, the padded stride or the tiles of 64 cells the autotuner chose
+    out["population_tiled"], out["population_stride"] = s.population_layout()
     # One kernel launch per step, back to back on the solver's stream: the step kernel's average
     # duration is the time between the HIP events around the timed steps over their count (a kernel
     # trace of the same region gives the same average, the launches leave no gaps).
@@ -545,6 +548,8 @@ def main():
     ap.add_argument("--batch-launch", help="pin the batch launch shape: TILE,ORDER (lbm_batch_set_launch)")
     ap.add_argument("--stride", type=int, help="pin the D3Q19 population stride in elements, 0 = N "
                     "(lbm_set_population_stride)")
+    ap.add_argument("--tiled", action="store_true", help="pin the D3Q19 tiled population layout "
+                    "(lbm_set_population_layout)")
     args = ap.parse_args()
     if args.launch:
         PIN["launch"] = tuple(int(v) for v in args.launch.split(","))
@@ -552,6 +557,7 @@ def main():
         PIN["batch"] = tuple(int(v) for v in args.batch_launch.split(","))
     if args.stride is not None:
         PIN["stride"] = args.stride
+    PIN["tiled"] = args.tiled
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solver does not shard "
               f"across GPUs; see the sharding design notes).", file=sys.stderr)
@@ -607,8 +613,8 @@ def main():
         "variants": {k: {kk: v[kk] for kk in ("mlups", "ms_per_step", "algorithmic_gbs", "roofline_frac",
                                                "step_kernel_us", "evented_step_us", "slowest_step_us",
                                                "kernel_achieved_gbs", "bytes_per_cell",
-                                               "bulk_block", "row_order", "population_stride", "nodes",
-                                               "finite") if kk in v}
+                                               "bulk_block", "row_order", "population_tiled", "population_stride",
+                                               "nodes", "finite") if kk in v}
                      for k, v in variants.items()},
         "fp16_over_fp32": variants.get("fp16_192x96x96", main_run)["mlups"] / main_run["mlups"],
         "fp16sr_over_fp32": variants.get("fp16sr_192x96x96", main_run)["mlups"] / main_run["mlups"],
