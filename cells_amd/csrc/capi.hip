@@ -1293,10 +1293,13 @@ int lbm_read_populations(lbm_solver* s, void* out, size_t bytes) {
     std::vector<uint8_t> raw(pop_elems(s) * s->elem);
     HIP_TRY(hipMemcpyAsync(raw.data(), s->f[s->parity], raw.size(), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
+    // population i of tile t: 64 consecutive elements at t * 64 Q + 64 i (N is a multiple of 64 here)
     uint8_t* o = static_cast<uint8_t*>(out);
-    for (int i = 0; i < s->Q; i++)
-      for (uint32_t c = 0; c < s->N; c++)
-        std::memcpy(o + (size_t(i) * s->N + c) * s->elem, raw.data() + size_t(pelem(s, i, c)) * s->elem, s->elem);
+    const size_t run = 64u * s->elem;
+    for (uint32_t t = 0; t < s->N / 64u; t++)
+      for (int i = 0; i < s->Q; i++)
+        std::memcpy(o + (size_t(i) * s->N + size_t(t) * 64u) * s->elem,
+                    raw.data() + size_t(pelem(s, i, t * 64u)) * s->elem, run);
     return LBM_OK;
   } else if (s->NS == s->N) {
     HIP_TRY(hipMemcpyAsync(out, s->f[s->parity], need, hipMemcpyDeviceToHost, s->stream));
