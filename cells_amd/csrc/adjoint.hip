@@ -462,7 +462,6 @@ struct Src {
 // forward tables' indices); returns how many (0 when the reconstruction discards the pull). pos is
 // the node's forward list position or NO_OPERAND for bulk nodes.
 int pull_sources_at_stride(const lbm_solver* s, uint32_t idx, uint32_t pos, int i, const double* dir, Src* out) {
-  const uint32_t N = s->NS;
   const bool d2 = s->nd == 2;
   const uint32_t fl = s->h_flags[idx];
   if (d2) {
@@ -475,7 +474,7 @@ int pull_sources_at_stride(const lbm_solver* s, uint32_t idx, uint32_t pos, int 
   }
   if (pos == NO_OPERAND) {
     if (d2) {
-      out[0] = {uint32_t(i) * N + idx - uint32_t(CY2[i] * s->W + CX2[i]), 1.0f, 0.0f};
+      out[0] = {pelem(s, i, idx - uint32_t(CY2[i] * s->W + CX2[i])), 1.0f, 0.0f};
     } else {
       const uint32_t WH = uint32_t(s->W) * uint32_t(s->H);
       const Row3 r = row3(s, (idx % WH) / uint32_t(s->W), idx / WH);

@@ -423,6 +423,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
     void* f[2] = {nullptr, nullptr};
     void* ring = nullptr;
     void* state = nullptr;
+    uint32_t na0 = 0, ns0 = 0;  // the tunnel's population layout when saved
   };
   struct Guard {
     lbm_batch* b;
@@ -435,9 +436,14 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
       saved = false;
       for (size_t i = 0; i < b->t.size(); i++) {
         lbm_solver* s = b->t[i];
-        const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
-        HIP_TRY(hipMemcpyAsync(s->f[0], sv[i].f[0], fb, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(s->f[1], sv[i].f[1], fb, hipMemcpyDeviceToDevice, b->stream));
+        const size_t fb = pop_elems(s) * s->elem;  // whole buffers, in the tunnel's layout
+        for (int k = 0; k < 2; k++) {
+          if (s->NA == sv[i].na0 && s->NS == sv[i].ns0) {
+            HIP_TRY(hipMemcpyAsync(s->f[k], sv[i].f[k], fb, hipMemcpyDeviceToDevice, b->stream));
+          } else if (int rc = relayout(s, sv[i].f[k], s->f[k], sv[i].na0, sv[i].ns0, s->NA, s->NS, b->stream)) {
+            return rc;
+          }
+        }
         HIP_TRY(hipMemcpyAsync(s->ring, sv[i].ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));
         HIP_TRY(hipMemcpyAsync(s->state, sv[i].state, 4 * STATE_WORDS, hipMemcpyDeviceToDevice, b->stream));
       }
@@ -446,6 +452,9 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
     }
     ~Guard() {
       if (saved) (void)restore();
+      if (!keep)
+        for (size_t i = 0; i < b->t.size(); i++)
+          if (b->t[i]->NA != sv[i].na0 || b->t[i]->NS != sv[i].ns0) (void)set_layout(b->t[i], sv[i].na0, sv[i].ns0);
       (void)hipStreamSynchronize(b->stream);
       for (Saved& x : sv)
         for (void* p : {x.f[0], x.f[1], x.ring, x.state})
@@ -464,7 +473,9 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   g.order0 = b->order;
   for (size_t i = 0; i < b->t.size(); i++) {
     lbm_solver* s = b->t[i];
-    const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
+    const size_t fb = pop_elems(s) * s->elem;  // whole buffers, in the tunnel's layout
+    g.sv[i].na0 = s->NA;
+    g.sv[i].ns0 = s->NS;
     HIP_TRY(hipMalloc(&g.sv[i].f[0], fb));
     HIP_TRY(hipMalloc(&g.sv[i].f[1], fb));
     HIP_TRY(hipMalloc(&g.sv[i].ring, sizeof(double) * 3 * HISTORY_LEN));
@@ -474,7 +485,7 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   HIP_TRY(hipEventCreate(&g.e1));
   for (size_t i = 0; i < b->t.size(); i++) {
     lbm_solver* s = b->t[i];
-    const size_t fb = size_t(s->Q) * s->NS * s->elem;  // whole buffers, at the tunnel's stride
+    const size_t fb = pop_elems(s) * s->elem;  // whole buffers, in the tunnel's layout
     HIP_TRY(hipMemcpyAsync(g.sv[i].f[0], s->f[0], fb, hipMemcpyDeviceToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(g.sv[i].f[1], s->f[1], fb, hipMemcpyDeviceToDevice, b->stream));
     HIP_TRY(hipMemcpyAsync(g.sv[i].ring, s->ring, sizeof(double) * 3 * HISTORY_LEN, hipMemcpyDeviceToDevice, b->stream));
@@ -508,6 +519,46 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   size_t best = 0;
   for (size_t c = 1; c < nc; c++)
     if (best_of[c] < best_of[best]) best = c;
+  // Population layout: the chosen shape again with every tunnel in its own layout and with the
+  // tunnels whose rows allow it in tiles of 64 cells, twice each in alternating order; a layout change
+  // leaves the new buffers in the Infinity Cache, so each visit runs untimed steps first.
+  bool any_tiles = false;
+  for (const lbm_solver* s : b->t) any_tiles = any_tiles || (tiles_ok(s) && s->NA == 0);
+  bool tiled = false;
+  if (any_tiles) {
+    b->tile = cand[best].first;
+    b->order = cand[best].second;
+    double t_lay[2] = {1e300, 1e300};
+    for (int rep = 0; rep < 2; rep++) {
+      for (int k = 0; k < 2; k++) {
+        const int tl = rep == 0 ? k : 1 - k;
+        for (size_t i = 0; i < b->t.size(); i++) {
+          lbm_solver* s = b->t[i];
+          const bool t = tl && tiles_ok(s);
+          if (int rc = set_layout(s, t ? uint32_t(s->Q - 1) : g.sv[i].na0, t ? 64u : g.sv[i].ns0)) return rc;
+        }
+        b->dirty = true;
+        if (int rc = refresh(b)) return rc;
+        set_u64_kernel<<<1, 1, 0, b->stream>>>(b->d_step, uint64_t(b->step));
+        for (int k2 = 0; k2 < 2 * trial_steps + 2; k2++) launch_step(b, b->parity ^ (k2 & 1), uint32_t(k2));
+        HIP_TRY(hipEventRecord(g.e0, b->stream));
+        for (int k2 = 0; k2 < trial_steps; k2++)
+          launch_step(b, b->parity ^ (k2 & 1), uint32_t(2 * trial_steps + 2 + k2));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(g.e1, b->stream));
+        HIP_TRY(hipEventSynchronize(g.e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, g.e0, g.e1));
+        t_lay[tl] = std::min(t_lay[tl], double(ms));
+      }
+    }
+    tiled = t_lay[1] < t_lay[0];
+    for (size_t i = 0; i < b->t.size(); i++) {
+      lbm_solver* s = b->t[i];
+      const bool t = tiled && tiles_ok(s);
+      if (int rc = set_layout(s, t ? uint32_t(s->Q - 1) : g.sv[i].na0, t ? 64u : g.sv[i].ns0)) return rc;
+    }
+  }
   if (int rc = g.restore()) return rc;
   b->tile = cand[best].first;
   b->order = cand[best].second;

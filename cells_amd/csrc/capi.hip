@@ -101,6 +101,7 @@ void write_params2(lbm_solver* s) {
   p.H = s->H;
   p.N = s->N;
   p.NS = s->NS;
+  p.NA = s->NA;
   p.left = c.left;
   p.right = c.right;
   p.bottom = c.bottom;
@@ -231,13 +232,12 @@ int resolve3(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
 
 int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint32_t* g, BzLink* rec) {
   const int W = s->W, H = s->H;
-  const uint32_t N = s->NS;  // population stride of the gather indices
   const Params2& p = s->p2;
   const int y = int(idx / uint32_t(W)), x = int(idx % uint32_t(W));
   const int j = OPP2[i];
   int sx = x - CX2[i], sy = y - CY2[i];
   if ((fl[idx] & (1u << i)) == 0u) {
-    *g = uint32_t(i) * N + uint32_t(sy * W + sx);
+    *g = pelem(s, i, uint32_t(sy * W + sx));
     return PULL_GATHER;
   }
   auto wrap = [&](int& qx, int& qy) {
@@ -251,18 +251,18 @@ int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
   if (inX && inY) {
     const uint32_t sc = uint32_t(sy * W + sx);
     if (!(fl[sc] & FLAG_SOLID2)) {
-      *g = uint32_t(i) * N + sc;
+      *g = pelem(s, i, sc);
       return PULL_GATHER;
     }
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
     rec->q = link_q(s->sdf_host[idx], s->sdf_host[sc]);
     if (rec->q < 0.5f) {
       int nx = x + CX2[i], ny = y + CY2[i];
       wrap(nx, ny);
       const bool ok = nx >= 0 && nx < W && ny >= 0 && ny < H && !(fl[uint32_t(ny * W + nx)] & FLAG_SOLID2);
-      rec->gb = ok ? uint32_t(j) * N + uint32_t(ny * W + nx) : NO_OPERAND;
+      rec->gb = ok ? pelem(s, j, uint32_t(ny * W + nx)) : NO_OPERAND;
     } else {
-      rec->gb = uint32_t(i) * N + idx;
+      rec->gb = pelem(s, i, idx);
     }
     return PULL_BOUZIDI;
   }
@@ -270,13 +270,13 @@ int resolve2(const lbm_solver* s, const uint32_t* fl, uint32_t idx, int i, uint3
     const int mode = sy < 0 ? p.bottom : p.top;
     if (mode == 2) {
       const int rx = std::min(std::max(sx, 0), W - 1);
-      *g = uint32_t(MIRROR_Y2[i]) * N + uint32_t(y) * uint32_t(W) + uint32_t(rx);
+      *g = pelem(s, MIRROR_Y2[i], uint32_t(y) * uint32_t(W) + uint32_t(rx));
       return PULL_GATHER;
     }
-    *g = uint32_t(j) * N + idx;
+    *g = pelem(s, j, idx);
     return mode == 3 ? PULL_MARKED : PULL_GATHER;
   }
-  *g = uint32_t(j) * N + idx;
+  *g = pelem(s, j, idx);
   return PULL_GATHER;
 }
 
@@ -651,6 +651,7 @@ int set_layout_impl(lbm_solver* s, uint32_t na, uint32_t ns) {
   s->NA = na;
   if (s->nd == 2) {
     s->p2.NS = ns;
+    s->p2.NA = na;
   } else {
     s->p3.NS = ns;
     s->p3.NA = na;
@@ -658,10 +659,10 @@ int set_layout_impl(lbm_solver* s, uint32_t na, uint32_t ns) {
   return LBM_OK;
 }
 
-// Whether a D3Q19 solver can take tiles of 64 cells: rows of a multiple of 64 cells (so a wave of
-// the bulk rows is one tile) and byte offsets within a buffer that fit 32 bits.
+// Whether a D3Q19 or D2Q9 solver can take tiles of 64 cells: rows of a multiple of 64 cells (so a wave
+// of the bulk rows is one tile) and byte offsets within a buffer that fit 32 bits.
 bool tiles_allowed(const lbm_solver* s) {
-  return s->nd == 3 && s->Q == 19 && s->W % 64 == 0 && uint64_t(s->Q) * s->N * s->elem < (uint64_t(1) << 32);
+  return (s->Q == 19 || s->Q == 9) && s->W % 64 == 0 && uint64_t(s->Q) * s->N * s->elem < (uint64_t(1) << 32);
 }
 
 // Population layouts the autotuner times for a 3D solver, (NA, NS) with the unpadded one first. The
@@ -680,7 +681,11 @@ bool tiles_allowed(const lbm_solver* s) {
 // 64-tunnel sweep) no pad was faster than N.
 std::vector<std::pair<uint32_t, uint32_t>> layout_candidates(const lbm_solver* s) {
   std::vector<std::pair<uint32_t, uint32_t>> c{{0u, s->N}};
-  if (s->nd != 3 || s->adj || s->N < 65536u) return c;
+  if (s->adj || s->N < 65536u) return c;
+  if (s->nd == 2) {  // D2Q9: no pad measured faster than N (below), tiles were (9-17% on the pull copy)
+    if (tiles_allowed(s)) c.push_back({uint32_t(s->Q - 1), 64u});
+    return c;
+  }
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
     if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back({0u, uint32_t(base + pad)});
@@ -742,6 +747,12 @@ int enqueue_one(lbm_solver* s) { return enqueue_step(s, nullptr, -1); }
 namespace lbm {
 
 int rebuild_flags(lbm_solver* s) { return rebuild_flags_impl(s); }
+int set_layout(lbm_solver* s, uint32_t na, uint32_t ns) { return set_layout_impl(s, na, ns); }
+bool tiles_ok(const lbm_solver* s) { return tiles_allowed(s); }
+int relayout(const lbm_solver* s, const void* src, void* dst, uint32_t na0, uint32_t ns0, uint32_t na1, uint32_t ns1,
+             hipStream_t st) {
+  return launch_relayout(s, src, dst, na0, ns0, na1, ns1, st);
+}
 
 int check_usable(const lbm_solver* s) {
   if (!s) return fail(LBM_ERR_INVALID, "null solver");
@@ -1619,8 +1630,8 @@ int lbm_set_population_layout(lbm_solver* s, int32_t tiled, int64_t stride) {
   uint32_t na = 0, ns = 0;
   if (tiled) {
     if (!tiles_allowed(s))
-      return fail(LBM_ERR_UNSUPPORTED, "population layout: tiles of 64 cells need D3Q19, a row width that is a multiple "
-                                       "of 64 and 19 x N x element bytes below 2^32");
+      return fail(LBM_ERR_UNSUPPORTED, "population layout: tiles of 64 cells need D3Q19 or D2Q9, a row width that is a "
+                                       "multiple of 64 and Q x N x element bytes below 2^32");
     if (stride != 0 && stride != 64) return fail(LBM_ERR_INVALID, "population layout: tiles take stride 0 or 64");
     na = uint32_t(s->Q - 1);
     ns = 64u;

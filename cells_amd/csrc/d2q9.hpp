@@ -156,7 +156,7 @@ __device__ __forceinline__ void collide2(const float (&f)[9], const Relax2& r, i
 
 // One plain fluid / plain Zou-He node of the bulk pass (src/shaders/step.ts:177-227 for flags that
 // are 0 or only the off-grid x bits of an open column). Per direction the source address is a
-// wave-uniform base (src + i*NS - delta_i) plus the lane's byte offset.
+// wave-uniform base (population i, shifted by whole rows) plus the lane's byte offset.
 template <bool REC>
 __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float* __restrict__ dst,
                                            const uint32_t* __restrict__ flags, const float4* __restrict__ xtab,
@@ -178,18 +178,26 @@ __device__ __forceinline__ void bulk_cell2(const float* __restrict__ src, float*
     if (fl != 0u && !inlet && !outlet) return;
   }
   const size_t N = p.NS;  // population stride
-  const uint32_t off = idx * 4u;
+  // Element of population i at cell c: c + NA (c & ~63) + NS i (one array per population, or tiles of
+  // 64 cells when W % 64 == 0, so that a wave is one tile): the y shift moves whole tiles and stays in
+  // the wave-uniform base, an x shift crosses into the previous / next tile only at the wave's first /
+  // last lane (offp, offm). At x = 0 and x = W - 1 those pulls read the node's own element: the
+  // populations entering from off the grid are placeholders (step.ts:173-174) that Zou-He overwrites,
+  // and a bulk node there is always a plain inlet or outlet node.
+  const uint32_t off = (idx + p.NA * (idx & ~63u)) * 4u;
+  const uint32_t tile_step = 64u * p.NA * 4u;
+  const uint32_t offp = x == 0u ? off : off - 4u - ((idx & 63u) == 0u ? tile_step : 0u);
+  const uint32_t offm = x == uint32_t(p.W - 1) ? off : off + 4u + ((idx & 63u) == 63u ? tile_step : 0u);
   const char* const sb = reinterpret_cast<const char*>(src);
   float f[9];
 #pragma unroll
   for (int i = 0; i < 9; i++) {
-    const ptrdiff_t delta = ptrdiff_t(CY2[i]) * p.W + CX2[i];
-    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - delta) * 4;
-    uint32_t o = off;
-    // The populations entering from off the grid are placeholders (step.ts:173-174) that Zou-He
-    // overwrites; on the boundary rows read the node's own population instead so the address stays
-    // on the grid (on interior rows the plain address is in the buffer and its value is overwritten).
-    if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) o = off + uint32_t(delta) * 4u;
+    const ptrdiff_t dy = ptrdiff_t(CY2[i]) * p.W;
+    const char* base = sb + (ptrdiff_t(i) * ptrdiff_t(N) - ptrdiff_t(1u + p.NA) * dy) * 4;
+    uint32_t o = CX2[i] > 0 ? offp : CX2[i] < 0 ? offm : off;
+    // On the boundary rows the off-grid populations of an inlet / outlet node read the node's own
+    // element, so the address stays on the grid (on interior rows any address above is in the buffer).
+    if ((inlet && CX2[i] > 0) || (outlet && CX2[i] < 0)) o = off + uint32_t(ptrdiff_t(1u + p.NA) * dy) * 4u;
     f[i] = *reinterpret_cast<const float*>(base + o);
   }
   if (interior) {
@@ -231,7 +239,8 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
   const int y = int(idx / uint32_t(p.W));
   const int x = int(idx - uint32_t(y) * uint32_t(p.W));
   float f[9];
-  f[0] = src[idx];
+  const uint32_t ce = idx + p.NA * (idx & ~63u);  // the node's element within each population
+  f[0] = src[ce];
 #pragma unroll
   for (int i = 1; i < 9; i++) f[i] = src[lt.gtab[size_t(i - 1) * nl + k]];
   if (BODY) {
@@ -288,7 +297,7 @@ __device__ __forceinline__ void list_cell2(const float* __restrict__ src, float*
   float out[9];
   collide2(f, r, y, p, out);
 #pragma unroll
-  for (int i = 0; i < 9; i++) dst[size_t(i) * N + idx] = out[i];
+  for (int i = 0; i < 9; i++) dst[size_t(i) * N + ce] = out[i];
 }
 
 }  // namespace lbm

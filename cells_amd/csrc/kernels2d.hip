@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void macro2_kernel(const float* __restrict__ f
   }
   float v[9];
 #pragma unroll
-  for (int i = 0; i < 9; i++) v[i] = f[size_t(i) * p.NS + idx];
+  for (int i = 0; i < 9; i++) v[i] = f[size_t(i) * p.NS + idx + p.NA * (idx & ~63u)];
   const float rho = 1.0f + v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7] + v[8];
   float jx = v[1] - v[3] + v[5] - v[6] - v[7] + v[8];
   float jy = v[2] - v[4] + v[5] + v[6] - v[7] - v[8];
@@ -89,8 +89,8 @@ __global__ __launch_bounds__(256) void init2_kernel(float* __restrict__ fa, floa
 #pragma unroll
   for (int i = 0; i < 9; i++) {
     const float v = feq2(i, m.x, m.y, m.z);
-    fa[size_t(i) * p.NS + idx] = v;
-    fb[size_t(i) * p.NS + idx] = v;
+    fa[size_t(i) * p.NS + idx + p.NA * (idx & ~63u)] = v;
+    fb[size_t(i) * p.NS + idx + p.NA * (idx & ~63u)] = v;
   }
 }
 
@@ -163,8 +163,8 @@ __global__ __launch_bounds__(256) void flags2_kernel(const float* __restrict__ s
 #pragma unroll
     for (int i = 0; i < 9; i++) {
       const float v = feq2(i, rho, ux, uy);
-      fa[size_t(i) * p.NS + idx] = v;
-      fb[size_t(i) * p.NS + idx] = v;
+      fa[size_t(i) * p.NS + idx + p.NA * (idx & ~63u)] = v;
+      fb[size_t(i) * p.NS + idx + p.NA * (idx & ~63u)] = v;
     }
   }
 }

@@ -181,7 +181,7 @@ struct Params2 {
   uint32_t N;
   int32_t left, right, bottom, top, parabolic, forcing;
   float tau, lambda, smagC2, uIn, spongeStart, tauSponge, uLid, fx, fy, absorb, inletLayer;
-  uint32_t NS;  // population stride in elements, as Params3::NS
+  uint32_t NS, NA;  // population layout, as Params3::NS / NA (NA = 8 for tiles of 64 cells)
 };
 // Params3 (src/shaders/common3d.ts:49-56) after Solver3D.writeParams (src/solver3d.ts:197-208).
 struct Params3 {
@@ -192,8 +192,8 @@ struct Params3 {
   uint32_t faces;  // open x planes: 0 equilibrium (the reference's), 1 Zou-He faces (Hecht-Harting)
   // Population layout: population i of cell c is element c + NA (c & ~63) + NS i. NA = 0: one array
   // per population, NS elements apart (N, or N padded: the elements NS - N after each population
-  // hold no population); NA = 18, NS = 64 (D3Q19 with W % 64 == 0): tiles of 64 cells, each holding
-  // its cells' 19 populations one after another. N stays the cell count.
+  // hold no population); NA = Q - 1, NS = 64 (rows of a multiple of 64 cells): tiles of 64 cells, each
+  // holding its cells' Q populations one after another. N stays the cell count.
   uint32_t NS;
   uint32_t NA;
 };

@@ -168,6 +168,14 @@ uint32_t row_gather3(const lbm_solver* s, const Row3& r, uint32_t idx, bool outl
 // Frees the adjoint state of a solver (adjoint.hip).
 void adjoint_free(lbm_solver* s);
 
+// Population layouts (capi.hip): move a solver's buffers and tables to layout (na, ns); whether tiles
+// of 64 cells are allowed; copy the populations of buffer src in layout (na0, ns0) to dst in
+// (na1, ns1) on stream st.
+int set_layout(lbm_solver* s, uint32_t na, uint32_t ns);
+bool tiles_ok(const lbm_solver* s);
+int relayout(const lbm_solver* s, const void* src, void* dst, uint32_t na0, uint32_t ns0, uint32_t na1, uint32_t ns1,
+             hipStream_t st);
+
 // Element of population i at cell c in the solver's buffers, and the inverse; elements per buffer.
 __host__ __device__ inline uint32_t pelem(uint32_t na, uint32_t ns, int i, uint32_t c) {
   return c + na * (c & ~63u) + ns * uint32_t(i);
