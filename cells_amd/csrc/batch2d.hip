@@ -519,46 +519,6 @@ int lbm_batch_autotune(lbm_batch* b, int32_t trial_steps, int32_t* tile, int32_t
   size_t best = 0;
   for (size_t c = 1; c < nc; c++)
     if (best_of[c] < best_of[best]) best = c;
-  // Population layout: the chosen shape again with every tunnel in its own layout and with the
-  // tunnels whose rows allow it in tiles of 64 cells, twice each in alternating order; a layout change
-  // leaves the new buffers in the Infinity Cache, so each visit runs untimed steps first.
-  bool any_tiles = false;
-  for (const lbm_solver* s : b->t) any_tiles = any_tiles || (tiles_ok(s) && s->NA == 0);
-  bool tiled = false;
-  if (any_tiles) {
-    b->tile = cand[best].first;
-    b->order = cand[best].second;
-    double t_lay[2] = {1e300, 1e300};
-    for (int rep = 0; rep < 2; rep++) {
-      for (int k = 0; k < 2; k++) {
-        const int tl = rep == 0 ? k : 1 - k;
-        for (size_t i = 0; i < b->t.size(); i++) {
-          lbm_solver* s = b->t[i];
-          const bool t = tl && tiles_ok(s);
-          if (int rc = set_layout(s, t ? uint32_t(s->Q - 1) : g.sv[i].na0, t ? 64u : g.sv[i].ns0)) return rc;
-        }
-        b->dirty = true;
-        if (int rc = refresh(b)) return rc;
-        set_u64_kernel<<<1, 1, 0, b->stream>>>(b->d_step, uint64_t(b->step));
-        for (int k2 = 0; k2 < 2 * trial_steps + 2; k2++) launch_step(b, b->parity ^ (k2 & 1), uint32_t(k2));
-        HIP_TRY(hipEventRecord(g.e0, b->stream));
-        for (int k2 = 0; k2 < trial_steps; k2++)
-          launch_step(b, b->parity ^ (k2 & 1), uint32_t(2 * trial_steps + 2 + k2));
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(g.e1, b->stream));
-        HIP_TRY(hipEventSynchronize(g.e1));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, g.e0, g.e1));
-        t_lay[tl] = std::min(t_lay[tl], double(ms));
-      }
-    }
-    tiled = t_lay[1] < t_lay[0];
-    for (size_t i = 0; i < b->t.size(); i++) {
-      lbm_solver* s = b->t[i];
-      const bool t = tiled && tiles_ok(s);
-      if (int rc = set_layout(s, t ? uint32_t(s->Q - 1) : g.sv[i].na0, t ? 64u : g.sv[i].ns0)) return rc;
-    }
-  }
   if (int rc = g.restore()) return rc;
   b->tile = cand[best].first;
   b->order = cand[best].second;

@@ -682,10 +682,10 @@ bool tiles_allowed(const lbm_solver* s) {
 std::vector<std::pair<uint32_t, uint32_t>> layout_candidates(const lbm_solver* s) {
   std::vector<std::pair<uint32_t, uint32_t>> c{{0u, s->N}};
   if (s->adj || s->N < 65536u) return c;
-  if (s->nd == 2) {  // D2Q9: no pad measured faster than N (below), tiles were (9-17% on the pull copy)
-    if (tiles_allowed(s)) c.push_back({uint32_t(s->Q - 1), 64u});
-    return c;
-  }
+  // D2Q9 takes a padded stride or tiles by hand but is not timed over them: on the D2Q9 step no pad
+  // was faster than N, and tiles moved it by -2% to +5% (4096 x 2048 112.1 -> 109.6 us, 2048 x 1024
+  // 24.7 -> 25.9, the 64-tunnel sweep within 0.3%), about the size of the trials' own noise.
+  if (s->nd == 2) return c;
   const uint64_t base = (uint64_t(s->N) + 1023u) / 1024u * 1024u;
   for (uint64_t pad : {uint64_t(4096), uint64_t(17 * 4096)})
     if (uint64_t(s->Q) * (base + pad) < (uint64_t(1) << 32)) c.push_back({0u, uint32_t(base + pad)});

@@ -461,8 +461,8 @@ def test_d2q9_tiled_layout_gives_identical_bits(name):
 
 def test_batch_tiled_layout_and_its_autotuning():
     """Batched tunnels (rows of 64, 128 and 192 nodes) tiled through the handles the caller kept, and the
-    batch autotuner's layout stage (it restores every tunnel's state into the layout it keeps): the
-    same bits as the batch with one array per population."""
+    batch autotuner on tiled tunnels (it saves and restores each tunnel in its layout): the same bits
+    as the batch with one array per population."""
     cases = sweep.sweep_cases(n=5, seed=11, sizes=(64, 128, 192))
 
     def run(mode):
@@ -473,7 +473,7 @@ def test_batch_tiled_layout_and_its_autotuning():
         b.reset_clock()
         b.set_launch(256, 2)
         b.run(40)
-        if mode == "tiled":
+        if mode in ("tiled", "autotune"):
             for t in b.tunnels:
                 t.set_population_layout(True)
         if mode == "autotune":
@@ -489,8 +489,7 @@ def test_batch_tiled_layout_and_its_autotuning():
     ref = run("plain")
     for mode in ("tiled", "autotune"):
         got = run(mode)
-        if mode == "tiled":
-            assert all(g[2] == (1, 64) for g in got)
+        assert all(g[2] == (1, 64) for g in got)
         for (pg, fg, _), (pr, fr, _) in zip(got, ref):
             np.testing.assert_array_equal(pg, pr)
             np.testing.assert_array_equal(fg, fr)
