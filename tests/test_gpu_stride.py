@@ -493,3 +493,25 @@ def test_batch_tiled_layout_and_its_autotuning():
         for (pg, fg, _), (pr, fr, _) in zip(got, ref):
             np.testing.assert_array_equal(pg, pr)
             np.testing.assert_array_equal(fg, fr)
+
+
+def test_d2q9_adjoint_gradient_identical_in_tiles():
+    """The 2D adjoint's tables take the forward sources through the layout: J and dJ/dsdf are the same
+    bits with the forward state in tiles of 64 cells."""
+    from cells_amd.lbm import X_OPEN, Y_SLIP
+
+    W, H, u = 128, 32, 0.05
+    sdf = g.empty_sdf(W, H)
+    g.add_circle(sdf, W, H, 30, 15.5, 4)
+    out = {}
+    for tiled in (False, True):
+        s = Solver.create(dict(width=W, height=H, left=X_OPEN, right=X_OPEN, bottom=Y_SLIP, top=Y_SLIP, tau=0.56,
+                               uIn=u, spongeFraction=0.15, spongeTau=1, absorb=0.02, forceEvery=1))
+        if tiled:
+            s.set_population_layout(True)
+        adj = ShapeAdjoint(s, 200, 80, (1.0, 0.0), 1.0, checkpoint_every=50, ramp=("smoothstep", u, 60))
+        out[tiled] = (adj.forward(sdf), adj.backward())
+        adj.release()
+        s.destroy()
+    assert out[False][0] == out[True][0]
+    assert np.array_equal(out[False][1], out[True][1])
