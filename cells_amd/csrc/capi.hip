@@ -659,10 +659,10 @@ int set_layout_impl(lbm_solver* s, uint32_t na, uint32_t ns) {
   return LBM_OK;
 }
 
-// Whether a D3Q19 or D2Q9 solver can take tiles of 64 cells: rows of a multiple of 64 cells (so a wave
-// of the bulk rows is one tile) and byte offsets within a buffer that fit 32 bits.
+// Whether a solver can take tiles of 64 cells: rows of a multiple of 64 cells (so a wave of the bulk
+// rows is one tile) and byte offsets within a buffer that fit 32 bits.
 bool tiles_allowed(const lbm_solver* s) {
-  return (s->Q == 19 || s->Q == 9) && s->W % 64 == 0 && uint64_t(s->Q) * s->N * s->elem < (uint64_t(1) << 32);
+  return s->W % 64 == 0 && uint64_t(s->Q) * s->N * s->elem < (uint64_t(1) << 32);
 }
 
 // Population layouts the autotuner times for a 3D solver, (NA, NS) with the unpadded one first. The
@@ -1630,8 +1630,8 @@ int lbm_set_population_layout(lbm_solver* s, int32_t tiled, int64_t stride) {
   uint32_t na = 0, ns = 0;
   if (tiled) {
     if (!tiles_allowed(s))
-      return fail(LBM_ERR_UNSUPPORTED, "population layout: tiles of 64 cells need D3Q19 or D2Q9, a row width that is a "
-                                       "multiple of 64 and Q x N x element bytes below 2^32");
+      return fail(LBM_ERR_UNSUPPORTED, "population layout: tiles of 64 cells need a row width that is a multiple of 64 "
+                                       "and Q x N x element bytes below 2^32");
     if (stride != 0 && stride != 64) return fail(LBM_ERR_INVALID, "population layout: tiles take stride 0 or 64");
     na = uint32_t(s->Q - 1);
     ns = 64u;

@@ -143,7 +143,7 @@ __device__ __forceinline__ float pull27(const typename S::elem* __restrict__ src
   const int W = p.W, H = p.H, D = p.D;
   const int j = opp27(i);
   auto at = [&](int ax, int ay, int az) { return uint32_t(ax) + uint32_t(W) * (uint32_t(ay) + uint32_t(H) * uint32_t(az)); };
-  auto ld = [&](int dir, uint32_t cell) { return S::load(src + size_t(dir) * N, cell); };
+  auto ld = [&](int dir, uint32_t cell) { return S::load(src + size_t(dir) * N, cell + p.NA * (cell & ~63u)); };
   const int sx = x - CX27[i], sy = y - CY27[i], sz = z - CZ27[i];
   const bool outX = sx < 0 || sx >= W, outY = sy < 0 || sy >= H, outZ = sz < 0 || sz >= D;
   if (outX && outlet) return ld(i, at(W - 2, y, z));
@@ -198,6 +198,7 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
                                        const FeqIn27& fin, int x, int y, int z, uint32_t idx, uint32_t fl,
                                        float (&force)[3], float (&out)[27]) {
   const uint32_t N = p.NS;  // population stride
+  const uint32_t o0 = idx + p.NA * (idx & ~63u);  // the node's element within each population
   const bool slow = (fl & FLAG_SLOW27) != 0u;
   if (fl & FLAG_INLET27) {  // src/shaders/step3d.ts:190-193: equilibrium at (1, uIn, 0, 0)
 #pragma unroll
@@ -212,26 +213,37 @@ __device__ __forceinline__ bool node27(const typename S::elem* __restrict__ src,
     // the outlet node, across x (copy from (W-2, y, z), the previous node of the row); no solid is
     // involved. So the source is a row-uniform population and cell offset (scalar registers) plus
     // the lane's x, with the outlet node's nine c_x < 0 directions replaced.
-    const int rowbase = p.W * y + int(p.WH) * z;
+    // The element of population m at cell idx - dyz - c_x is a wave-uniform base (population m, the
+    // row shift dyz, a multiple of 64 cells in tiles since W % 64 == 0 then) plus this node's element,
+    // or the one before / after it (crossing into the previous / next tile at the wave's first / last
+    // lane). Fast-path nodes have 1 <= x <= W - 1 (the inlet node pulled nothing above), and at
+    // x = W - 1 the outlet copy rule replaces the c_x < 0 pulls.
+    const uint32_t op = x == 0 ? o0 : o0 - 1u - ((idx & 63u) == 0u ? 64u * p.NA : 0u);
+    const uint32_t om = x == p.W - 1 ? o0 : o0 + 1u + ((idx & 63u) == 63u ? 64u * p.NA : 0u);
 #pragma unroll
     for (int i = 0; i < 27; i++) {
       const int sy = y - CY27[i], sz = z - CZ27[i];
       const bool outY = sy < 0 || sy >= p.H, outZ = sz < 0 || sz >= p.D;
       const int m = outY && outZ ? mirror_y27(mirror_z27(i)) : outY ? mirror_y27(i) : outZ ? mirror_z27(i) : i;
-      const int c = rowbase - (outY ? 0 : CY27[i] * p.W) - (outZ ? 0 : CZ27[i] * int(p.WH)) - CX27[i];
-      uint32_t off = uint32_t(m) * N + uint32_t(c + x);
-      if (CX27[i] < 0 && outlet) off = uint32_t(i) * N + idx - 1u;
-      f[i] = S::load(src, off);
+      const int dyz = (outY ? 0 : CY27[i] * p.W) + (outZ ? 0 : CZ27[i] * int(p.WH));
+      const typename S::elem* base = src + ptrdiff_t(m) * ptrdiff_t(N) - ptrdiff_t(1u + p.NA) * dyz;
+      uint32_t off = CX27[i] > 0 ? op : CX27[i] < 0 ? om : o0;
+      if (CX27[i] < 0 && outlet) {
+        base = src + size_t(i) * N;
+        off = o0 - 1u;
+      }
+      f[i] = S::load(base, off);
     }
   } else {
-    f[0] = S::load(src, idx);
+    f[0] = S::load(src, o0);
 #pragma unroll
     for (int i = 1; i < 27; i++) {
       if (fl & (1u << i)) {
         f[i] = pull27<S>(src, flags, sdf, p, i, x, y, z, idx, outlet, force);
       } else {
         const int delta = CZ27[i] * int(p.WH) + CY27[i] * p.W + CX27[i];
-        f[i] = S::load(src + size_t(i) * N, uint32_t(int(idx) - delta));
+        const uint32_t cell = uint32_t(int(idx) - delta);
+        f[i] = S::load(src + size_t(i) * N, cell + p.NA * (cell & ~63u));
       }
     }
   }
@@ -290,7 +302,7 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
       float out[27];
       slow = node27<S, SMAG>(src, flags, sdf, xtab, p, fin, x, y, z, idx, fl, force, out);
 #pragma unroll
-      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * p.NS, idx, out[i]);
+      for (int i = 0; i < 27; i++) S::store(dst + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), out[i]);
     }
   }
   if (!sample || __ballot(slow) == 0ull) return;  // uniform over the wave
@@ -339,7 +351,7 @@ __global__ __launch_bounds__(256) void macro27_kernel(const typename S::elem* __
   float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
 #pragma unroll
   for (int i = 0; i < 27; i++) {
-    const float v = S::load(f + size_t(i) * p.NS, idx);
+    const float v = S::load(f + size_t(i) * p.NS, idx + p.NA * (idx & ~63u));
     drho += v;
     jx += float(CX27[i]) * v;
     jy += float(CY27[i]) * v;
@@ -359,8 +371,8 @@ __global__ __launch_bounds__(256) void init27_kernel(typename S::elem* __restric
 #pragma unroll
   for (int i = 0; i < 27; i++) {
     const float v = solid ? 0.0f : feq27(i, 1.0f, p.uIn, 0.0f, 0.0f);
-    S::store(fa + size_t(i) * p.NS, idx, v);
-    S::store(fb + size_t(i) * p.NS, idx, v);
+    S::store(fa + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
+    S::store(fb + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
   }
 }
 
@@ -435,8 +447,8 @@ __global__ __launch_bounds__(256) void flags27_kernel(const float* __restrict__ 
 #pragma unroll
     for (int i = 0; i < 27; i++) {
       const float v = feq27(i, rho, ux, uy, uz);
-      S::store(fa + size_t(i) * p.NS, idx, v);
-      S::store(fb + size_t(i) * p.NS, idx, v);
+      S::store(fa + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
+      S::store(fb + size_t(i) * p.NS, idx + p.NA * (idx & ~63u), v);
     }
   }
 }

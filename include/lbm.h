@@ -263,12 +263,12 @@ int lbm_set_launch(lbm_solver* s, int32_t block, int32_t sweep);
 int lbm_population_stride(lbm_solver* s, int64_t* stride);
 int lbm_set_population_stride(lbm_solver* s, int64_t stride);
 /* No counterpart in the reference: the device population layout. tiled = 0: one array per
- * population, stride elements apart (lbm_population_stride). tiled = 1 (D3Q19 or D2Q9 with a row
- * width that is a multiple of 64 and Q x N x element bytes below 2^32): tiles of 64 cells, each
+ * population, stride elements apart (lbm_population_stride). tiled = 1 (a row width that is a
+ * multiple of 64 and Q x N x element bytes below 2^32): tiles of 64 cells, each
  * holding its cells' Q populations one after another, population i of cell c at element
  * c + (Q - 1) (c & ~63) + 64 i (stride reads 64); a tile is one wave of the step's bulk rows, and the
  * Q streams of a wave fall in one block of memory instead of Q arrays. lbm_autotune times it next to
- * the strides for D3Q19; set by hand here (stride 0 or 64 with tiled = 1, as lbm_set_population_stride
+ * the strides for D3Q19 and D3Q27; set by hand here (stride 0 or 64 with tiled = 1, as lbm_set_population_stride
  * otherwise; batched tunnels too). Same guarantees as the stride: host-visible layouts and results unchanged, refused
  * while an adjoint state exists. */
 int lbm_population_layout(lbm_solver* s, int32_t* tiled, int64_t* stride);
