@@ -317,15 +317,30 @@ __global__ __launch_bounds__(256) void step27_kernel(const typename S::elem* __r
 }
 
 // One wave: the partials of the listed waves (ascending wave index) summed lane-strided, then across
-// the wave, appended to the history ring (src/shaders/aux3d.ts:155-183's ring semantics).
+// the wave, appended to the history ring (src/shaders/aux3d.ts:155-183's ring semantics). Eight of a
+// lane's list entries are loaded at a time (their indices, then their partials) so that a chunk
+// costs two memory round trips instead of two per entry; each lane adds in list order as before.
 __global__ __launch_bounds__(64) void reduce27_kernel(const double* __restrict__ partials,
                                                       const uint32_t* __restrict__ waves, uint32_t nwaves,
                                                       double* __restrict__ ring, uint32_t* __restrict__ state,
                                                       uint64_t step) {
+  constexpr int U = 8;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (uint32_t k = threadIdx.x; k < nwaves; k += 64u)
+  for (uint32_t k0 = threadIdx.x; k0 < nwaves; k0 += 64u * U) {
+    uint32_t w[U];
 #pragma unroll
-    for (int d = 0; d < 3; d++) acc[d] += partials[size_t(waves[k]) * 3 + d];
+    for (int u = 0; u < U; u++) w[u] = k0 + 64u * u < nwaves ? waves[k0 + 64u * u] : ~0u;
+    double v[U][3];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) v[u][d] = w[u] != ~0u ? partials[size_t(w[u]) * 3 + d] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (w[u] != ~0u)
+#pragma unroll
+        for (int d = 0; d < 3; d++) acc[d] += v[u][d];
+  }
   wave_sum<3>(acc);
   if (threadIdx.x == 0) {
     const uint32_t st = state[0];
