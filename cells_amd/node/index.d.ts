@@ -58,7 +58,9 @@ declare class SolverBase {
   readPopulations(): Float32Array | Uint16Array;
   readFlags(): Uint32Array;
   timeSteps(n: number): number;
-  autotune(trialSteps?: number): { block: number; sweep: number };
+  autotune(trialSteps?: number): { block: number; sweep: number; tiled: boolean; stride: number };
+  /** Device population layout by hand (tiles of 64 cells, or a padded stride; 0 = unpadded). */
+  setPopulationLayout(tiled: boolean, stride?: number): void;
   /** First step whose force sample was non-finite since initField / resetForces, 0 while none was. */
   divergedAt(): number;
   destroy(): void;

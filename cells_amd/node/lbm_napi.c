@@ -277,7 +277,8 @@ static napi_value js_copy_roofline(napi_env env, napi_callback_info info) {
   return out;
 }
 
-/* autotune(handle, trialSteps) -> { block, sweep }: lbm_autotune then lbm_launch_info */
+/* autotune(handle, trialSteps) -> { block, sweep, tiled, stride }: lbm_autotune, then lbm_launch_info
+ * and lbm_population_layout */
 static napi_value js_autotune(napi_env env, napi_callback_info info) {
   napi_value argv[2], out, v;
   if (!get_args(env, info, 2, argv)) return NULL;
@@ -287,15 +288,38 @@ static napi_value js_autotune(napi_env env, napi_callback_info info) {
   napi_valuetype t;
   napi_typeof(env, argv[1], &t);
   if (t == napi_number) napi_get_value_int32(env, argv[1], &trial);
+  int32_t tiled = 0;
+  int64_t stride = 0;
   int rc = lbm_autotune(s, trial, NULL);
   if (!rc) rc = lbm_launch_info(s, &block, &sweep);
+  if (!rc) rc = lbm_population_layout(s, &tiled, &stride);
   if (rc) return throw_status(env, rc);
   CHECK_NAPI(env, napi_create_object(env, &out));
   napi_create_int32(env, block, &v);
   napi_set_named_property(env, out, "block", v);
   napi_create_int32(env, sweep, &v);
   napi_set_named_property(env, out, "sweep", v);
+  napi_get_boolean(env, tiled != 0, &v);
+  napi_set_named_property(env, out, "tiled", v);
+  napi_create_double(env, (double)stride, &v);
+  napi_set_named_property(env, out, "stride", v);
   return out;
+}
+
+/* setPopulationLayout(handle, tiled, stride): lbm_set_population_layout (device layout only; every
+ * array the caller reads keeps the reference's layout and bits) */
+static napi_value js_set_population_layout(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return NULL;
+  lbm_solver* s = solver_arg(env, argv[0]);
+  if (!s) return NULL;
+  bool tiled = false;
+  double stride = 0;
+  napi_get_value_bool(env, argv[1], &tiled);
+  napi_get_value_double(env, argv[2], &stride);
+  int rc = lbm_set_population_layout(s, tiled ? 1 : 0, (int64_t)stride);
+  if (rc) return throw_status(env, rc);
+  return NULL;
 }
 
 /* divergedAt(handle) -> number: lbm_diverged (waits for the work enqueued before it) */
@@ -749,6 +773,7 @@ static napi_value init(napi_env env, napi_value exports) {
       EXPORT("copyRoofline", js_copy_roofline),
       EXPORT("autotune", js_autotune),
       EXPORT("divergedAt", js_diverged_at),
+      EXPORT("setPopulationLayout", js_set_population_layout),
       EXPORT("brush", js_brush),
       EXPORT("brushStroke", js_brush_stroke),
       EXPORT("initField", js_init_field),

@@ -166,9 +166,15 @@ class Base {
   divergedAt() {
     return addon().divergedAt(this.handle);
   }
-  /** Picks the fastest launch shape for this grid (results do not change); returns { block, sweep }. */
+  /** Picks the fastest launch shape (and, for 3D grids, device population layout) for this grid
+   * (results do not change); returns { block, sweep, tiled, stride }. */
   autotune(trialSteps) {
     return addon().autotune(this.handle, trialSteps === undefined ? 200 : trialSteps);
+  }
+  /** Device population layout by hand: tiles of 64 cells, or one array per population at a stride
+   * (0: unpadded). Results and every array read back are unchanged. */
+  setPopulationLayout(tiled, stride) {
+    addon().setPopulationLayout(this.handle, !!tiled, stride === undefined ? 0 : stride);
   }
   destroy() {
     const h = this._h;

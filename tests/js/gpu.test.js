@@ -151,6 +151,31 @@ async function main() {
     console.log('ok shape adjoint');
   }
 
+  // --- device population layout: tiles of 64 cells give the same bits; autotune reports the layout
+  {
+    const mk = async () => {
+      const t = await m.Solver3D.create(0, { width: 64, height: 16, depth: 12, precision: 'fp32', tau: 0.524, uIn: 0.05,
+        spongeFraction: 0.15, spongeTau: 1, absorb: 0.02, forceEvery: 1 });
+      const sd = m.emptySdf3(64, 16, 12);
+      m.addSphere(sd, 64, 16, 12, 16, 7.5, 5.5, 3);
+      t.setSdf(sd);
+      t.initField();
+      return t;
+    };
+    const a = await mk();
+    const b = await mk();
+    b.setPopulationLayout(true);
+    await a.run(30, 10);
+    await b.run(30, 10);
+    assert.deepStrictEqual(await b.readPopulations(), await a.readPopulations());
+    assert.throws(() => b.setPopulationLayout(false, 7), /stride/);
+    const pick = a.autotune(10);
+    assert.ok(typeof pick.tiled === 'boolean' && pick.stride > 0, JSON.stringify(pick));
+    a.destroy();
+    b.destroy();
+    console.log('ok population layout');
+  }
+
   // --- allocation failure message keeps the reference's prefix
   let err = null;
   try {
